@@ -1,0 +1,279 @@
+"""ctypes front end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this module, and only as the checker or the timed CPU
+baseline. The product path (``pencil-rs_amd``) never imports it.
+
+Arrays are numpy arrays. Complex data is ``complex128`` / ``complex64``
+(interleaved re, im, as in the reference). Plan arrays are laid out
+``[extra][memory-ordered spatial axes]``; ``route(ndim, permute_dims)``
+returns the per-stage memory orders of the reference's ``build_route``
+(crates/pencil-fft/src/distributed.rs:2306-2373).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+OK, ERR_ARG, ERR_INVALID_SPECTRUM = 0, 1, 2
+DCT1, DCT2, DCT3, DCT4, DST1, DST2, DST3, DST4, DHT = range(1, 10)
+KIND_C2C, KIND_R2C, KIND_R2R = 1, 2, 3
+FORWARD, INVERSE, BACKWARD, ADJ_FORWARD, ADJ_INVERSE, ADJ_BACKWARD = range(6)
+MAXD = 4
+
+R2R_BACKWARD_KIND = {DCT1: DCT1, DCT2: DCT3, DCT3: DCT2, DCT4: DCT4, DST1: DST1,
+                     DST2: DST3, DST3: DST2, DST4: DST4, DHT: DHT}
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("ndim", ctypes.c_int),
+        ("shape", ctypes.c_size_t * MAXD),
+        ("extra", ctypes.c_size_t),
+        ("selected", ctypes.c_uint),
+        ("r2r", ctypes.c_int * MAXD),
+        ("backward_dir", ctypes.c_int * MAXD),
+        ("permute_dims", ctypes.c_int),
+    ]
+
+
+def build():
+    """Compiles liboracle.so with the Makefile next to this file."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        vp, sz = ctypes.c_void_p, ctypes.c_size_t
+        _lib.pfo_execute_f64.argtypes = [ctypes.POINTER(_Desc), ctypes.c_int, vp, vp, ctypes.c_int]
+        _lib.pfo_execute_f32.argtypes = [ctypes.POINTER(_Desc), ctypes.c_int, vp, vp, ctypes.c_int]
+        for name in ("pfo_c2c_lines_f64", "pfo_c2c_lines_f32", "pfo_c2c_lines_f80",
+                     "pfo_r2c_lines_f64", "pfo_r2c_lines_f32"):
+            getattr(_lib, name).argtypes = [vp, vp, sz, sz, ctypes.c_int]
+        for name in ("pfo_r2r_lines_f64", "pfo_r2r_lines_f32"):
+            getattr(_lib, name).argtypes = [vp, vp, sz, sz, ctypes.c_int, ctypes.c_int]
+        _lib.pfo_local_transpose.argtypes = [sz, ctypes.c_int, vp, sz, vp, vp, vp, vp]
+        _lib.pfo_alltoallv_1x1.argtypes = [sz, ctypes.c_int, vp, sz, vp, vp, vp, vp, vp]
+        for name in ("pfo_endpoint_accept_f64", "pfo_endpoint_accept_f32"):
+            getattr(_lib, name).argtypes = [ctypes.c_int, vp, vp, sz, ctypes.c_int, sz, vp,
+                                            ctypes.c_double, ctypes.c_double]
+        _lib.pfo_build_route.argtypes = [ctypes.c_int, ctypes.c_int, vp]
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _real_dtype(dtype) -> np.dtype:
+    d = np.dtype(dtype)
+    return np.dtype(np.float32) if d in (np.float32, np.complex64) else np.dtype(np.float64)
+
+
+def _sfx(dtype) -> str:
+    return "f32" if _real_dtype(dtype) == np.float32 else "f64"
+
+
+def route(ndim: int, permute_dims: bool = True):
+    perm = (ctypes.c_int * (MAXD * MAXD))()
+    lib().pfo_build_route(ndim, int(permute_dims), ctypes.cast(perm, ctypes.c_void_p))
+    return [tuple(perm[s * MAXD + p] for p in range(ndim)) for s in range(ndim)]
+
+
+# ---------------------------------------------------------------- lines --
+
+def c2c_lines(x: np.ndarray, op: int = FORWARD, extended: bool = False) -> np.ndarray:
+    """Batched contiguous C2C lines (last axis). extended=True computes in
+    long double (complex128 storage)."""
+    x = np.ascontiguousarray(x)
+    n = x.shape[-1]
+    batch = x.size // n if n else 0
+    y = np.empty_like(x)
+    if extended:
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.empty_like(x)
+        st = lib().pfo_c2c_lines_f80(_ptr(x), _ptr(y), n, batch, op)
+    else:
+        st = getattr(lib(), "pfo_c2c_lines_" + _sfx(x.dtype))(_ptr(x), _ptr(y), n, batch, op)
+    if st != OK:
+        raise ValueError(f"oracle c2c_lines status {st}")
+    return y
+
+
+def r2c_lines(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x)
+    n = x.shape[-1]
+    batch = x.size // n
+    cd = np.complex64 if x.dtype == np.float32 else np.complex128
+    y = np.empty(x.shape[:-1] + (n // 2 + 1,), dtype=cd)
+    st = getattr(lib(), "pfo_r2c_lines_" + _sfx(x.dtype))(_ptr(x), _ptr(y), n, batch, FORWARD)
+    if st != OK:
+        raise ValueError(f"oracle r2c_lines status {st}")
+    return y
+
+
+def c2r_lines(x: np.ndarray, n: int, op: int = INVERSE) -> np.ndarray:
+    x = np.ascontiguousarray(x)
+    batch = x.size // (n // 2 + 1)
+    rd = _real_dtype(x.dtype)
+    y = np.empty(x.shape[:-1] + (n,), dtype=rd)
+    st = getattr(lib(), "pfo_r2c_lines_" + _sfx(x.dtype))(_ptr(x), _ptr(y), n, batch, op)
+    if st != OK:
+        raise ValueError(f"oracle c2r_lines status {st}")
+    return y
+
+
+def r2r_lines(x: np.ndarray, kind: int, op: int = FORWARD) -> np.ndarray:
+    x = np.ascontiguousarray(x)
+    n = x.shape[-1]
+    batch = x.size // n if n else 0
+    y = np.empty_like(x)
+    st = getattr(lib(), "pfo_r2r_lines_" + _sfx(x.dtype))(_ptr(x), _ptr(y), n, batch, kind, op)
+    if st != OK:
+        raise ValueError(f"oracle r2r_lines status {st}")
+    return y
+
+
+# ---------------------------------------------------------------- plans --
+
+class InvalidSpectrum(Exception):
+    """The oracle rejected a C2R boundary (R2cError::InvalidSpectrum)."""
+
+
+def make_desc(kind, shape, extra=1, selected=None, r2r=None, backward_dir=None, permute_dims=True):
+    d = _Desc()
+    d.kind = kind
+    d.ndim = len(shape)
+    for a, n in enumerate(shape):
+        d.shape[a] = n
+    d.extra = extra
+    if selected is None:
+        selected = range(len(shape))
+    mask = 0
+    for a in selected:
+        mask |= 1 << a
+    d.selected = mask
+    for a in range(len(shape)):
+        d.r2r[a] = (r2r[a] or 0) if r2r is not None else 0
+        d.backward_dir[a] = int(bool(backward_dir[a])) if backward_dir is not None else 0
+    d.permute_dims = int(bool(permute_dims))
+    return d
+
+
+def plan_shapes(kind, shape, selected=None):
+    """(input logical shape, output logical shape) of the forward operation."""
+    shape = tuple(shape)
+    if kind != KIND_R2C:
+        return shape, shape
+    sel = list(range(len(shape))) if selected is None else list(selected)
+    red = max(sel)
+    out = list(shape)
+    out[red] = shape[red] // 2 + 1
+    return shape, tuple(out)
+
+
+def execute(desc: _Desc, op: int, src: np.ndarray, out_dtype, out_count: int, threads: int = 0):
+    src = np.ascontiguousarray(src)
+    dst = np.empty(out_count, dtype=out_dtype)
+    fn = lib().pfo_execute_f32 if _real_dtype(out_dtype) == np.float32 else lib().pfo_execute_f64
+    st = fn(ctypes.byref(desc), op, _ptr(src), _ptr(dst), threads)
+    if st == ERR_INVALID_SPECTRUM:
+        raise InvalidSpectrum()
+    if st != OK:
+        raise ValueError(f"oracle execute status {st}")
+    return dst
+
+
+def transform(kind, x: np.ndarray, op=FORWARD, shape=None, extra=1, selected=None, r2r=None,
+              backward_dir=None, permute_dims=True, threads=0):
+    """Runs one plan operation on a flat array laid out as the plan's input for
+    `op` and returns the flat output array."""
+    assert shape is not None
+    desc = make_desc(kind, shape, extra, selected, r2r, backward_dir, permute_dims)
+    real = _real_dtype(x.dtype)
+    cd = np.complex64 if real == np.float32 else np.complex128
+    shp_in, shp_out = plan_shapes(kind, shape, selected)
+    forward_like = op in (FORWARD, ADJ_INVERSE, ADJ_BACKWARD)
+    if kind == KIND_C2C:
+        count, dt = int(np.prod(shape)) * extra, cd
+    elif kind == KIND_R2R:
+        count, dt = int(np.prod(shape)) * extra, real
+    else:
+        if forward_like:
+            count, dt = int(np.prod(shp_out)) * extra, cd
+        else:
+            count, dt = int(np.prod(shp_in)) * extra, real
+    return execute(desc, op, x.reshape(-1), dt, count, threads)
+
+
+# ------------------------------------------------------------- layouts --
+
+def local_transpose(x: np.ndarray, shape, extra, src_perm, dst_perm) -> np.ndarray:
+    x = np.ascontiguousarray(x).reshape(-1)
+    y = np.empty_like(x)
+    nd = len(shape)
+    shp = (ctypes.c_size_t * MAXD)(*shape)
+    sp = (ctypes.c_int * MAXD)(*src_perm)
+    dp = (ctypes.c_int * MAXD)(*dst_perm)
+    st = lib().pfo_local_transpose(x.itemsize, nd, ctypes.cast(shp, ctypes.c_void_p), extra,
+                                   ctypes.cast(sp, ctypes.c_void_p), ctypes.cast(dp, ctypes.c_void_p),
+                                   _ptr(x), _ptr(y))
+    if st != OK:
+        raise ValueError("oracle local_transpose failed")
+    return y
+
+
+def alltoallv_1x1(x: np.ndarray, shape, extra, src_perm, dst_perm):
+    """Returns (destination, payload) of the one-peer Alltoallv transpose."""
+    x = np.ascontiguousarray(x).reshape(-1)
+    y = np.empty_like(x)
+    pay = np.empty_like(x)
+    nd = len(shape)
+    shp = (ctypes.c_size_t * MAXD)(*shape)
+    sp = (ctypes.c_int * MAXD)(*src_perm)
+    dp = (ctypes.c_int * MAXD)(*dst_perm)
+    st = lib().pfo_alltoallv_1x1(x.itemsize, nd, ctypes.cast(shp, ctypes.c_void_p), extra,
+                                 ctypes.cast(sp, ctypes.c_void_p), ctypes.cast(dp, ctypes.c_void_p),
+                                 _ptr(x), _ptr(y), _ptr(pay))
+    if st != OK:
+        raise ValueError("oracle alltoallv failed")
+    return y, pay
+
+
+def endpoint_accept(data: np.ndarray, perm, rshape, extra, axis, real_len, relative, absolute) -> bool:
+    data = np.ascontiguousarray(data)
+    nd = len(rshape)
+    shp = (ctypes.c_size_t * MAXD)(*rshape)
+    pp = (ctypes.c_int * MAXD)(*perm)
+    fn = lib().pfo_endpoint_accept_f32 if data.dtype == np.complex64 else lib().pfo_endpoint_accept_f64
+    return bool(fn(nd, ctypes.cast(pp, ctypes.c_void_p), ctypes.cast(shp, ctypes.c_void_p), extra, axis,
+                   real_len, _ptr(data), relative, absolute))
+
+
+def endpoint_thresholds(dtype, shape, selected, reduction_axis, normalize):
+    """(relative, absolute) acceptance thresholds, distributed/r2c.rs:4841-4878."""
+    eps = float(np.finfo(np.float32).eps) if _real_dtype(dtype) == np.float32 else float(np.finfo(np.float64).eps)
+    minsub = float(np.float32(1.4e-45)) if _real_dtype(dtype) == np.float32 else 5e-324
+    depth, transverse = 1.0, 1.0
+    for a, n in enumerate(shape):
+        if a != reduction_axis and a in selected:
+            if n > 1:
+                depth += (n - 1).bit_length()
+            transverse *= float(n)
+    relative = 128.0 * eps * depth
+    absolute = 128.0 * minsub * depth
+    if not normalize:
+        absolute *= transverse
+    return relative, absolute
