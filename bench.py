@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark of the single-GPU FFT path: 512^3 f32 R2C + normalized C2R
+round trips per second on one MI355X.
+
+One step = one 3D R2C forward of a 512^3 real N(0,1) field (seed 2) followed
+by the normalized C2R inverse back to real space, inputs resident in HBM,
+through the C ABI (pencil_fft_hip). Prints ONE JSON line.
+
+roofline: the dominant kernel is k_line_stage<float> (every stage of both
+directions). achieved = algorithmic bytes per launch (one read + one write of
+the stage's input and output arrays) / average launch duration, both measured
+here with HIP events on the stream the kernels run on.
+cpu_baseline: the oracle's f32 restatement of the reference route (per-line
+FFT with copy-in/out, pack / self-exchange / unpack transitions) timed on the
+host for one full round trip ("port").
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pencil-rs_amd"))
+
+METRIC = "512³ f32 R2C+C2R round-trips/s, % of HBM roofline; batched & adjoint vs plain"
+PEAK_GBPS = 8000.0
+
+
+def route_model_bytes(n: int, real_bytes: int) -> int:
+    """Route-model bytes of one round trip: (R + 5C) per direction."""
+    R = n ** 3 * real_bytes
+    C = n * n * (n // 2 + 1) * 2 * real_bytes
+    return 2 * (R + 5 * C)
+
+
+def traffic_from_profiles():
+    """HBM bytes per k_line_stage<float> launch from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "k_line_stage_512cube_f32_pmc.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return float(d["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(n: int, threads: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle as O
+
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((n, n, n), dtype=np.float32)
+    t0 = time.perf_counter()
+    spec = O.transform(O.KIND_R2C, x, O.FORWARD, shape=(n, n, n), threads=threads)
+    back = O.transform(O.KIND_R2C, spec, O.INVERSE, shape=(n, n, n), threads=threads)
+    dt = time.perf_counter() - t0
+    err = float(np.linalg.norm(back - x.reshape(-1)) / np.linalg.norm(x))
+    return {
+        "value": 1.0 / dt,
+        "unit": "round_trips/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"one full {n}^3 f32 R2C forward + normalized C2R inverse of the oracle's restatement of the "
+                  f"reference route ({dt:.2f} s, round-trip rel-L2 {err:.1e})",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timing-reps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (this build has no "
+              "multi-GPU path; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import pencil_fft_hip as pf
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    n = args.size
+    shape = (n, n, n)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2)
+    x = torch.randn(n ** 3, generator=g, device="cuda", dtype=torch.float32)
+    plan = pf.R2cPlan(shape, dtype=torch.float32)
+    spec = torch.empty(plan.output_elements(), dtype=torch.complex64, device="cuda")
+    back = torch.empty(n ** 3, dtype=torch.float32, device="cuda")
+    ws = plan.allocate_workspace()
+
+    def step():
+        plan.forward(x, spec, ws)
+        plan.inverse(spec, back, ws, flags=pf.EXEC_DEFER_STATUS)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    plan.take_status(ws)  # raises InvalidSpectrum if any inverse was rejected
+    rt_per_s = args.steps / elapsed
+    ms = elapsed / args.steps * 1e3
+
+    # accuracy of the round trip (f64 on device)
+    rt_err = float(torch.linalg.norm((back - x).double()) / torch.linalg.norm(x.double()))
+
+    # per-kernel HIP-event timing on the same stream
+    stage_ms, stage_bytes, check_ms = [], [], []
+    for _ in range(args.timing_reps):
+        _, tf = plan.execute_with_timing(pf.FORWARD, x, spec, ws)
+        _, ti = plan.execute_with_timing(pf.INVERSE, spec, back, ws)
+        for t in (tf, ti):
+            for axis, t_ms, b in t.entries():
+                if axis < 0:
+                    check_ms.append(t_ms)
+                else:
+                    stage_ms.append(t_ms)
+                    stage_bytes.append(b)
+    avg_ms = sum(stage_ms) / len(stage_ms)
+    avg_bytes = sum(stage_bytes) / len(stage_bytes)
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles()
+    rm = route_model_bytes(n, 4)
+    route_gbps = rm / (ms * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": rt_per_s,
+        "unit": "round_trips/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic i.i.d. N(0,1), seed 2 (torch cuda generator)",
+        "config": {
+            "workload": f"3D R2C forward + normalized C2R inverse, {n}x{n}x{n} f32 real -> {n}x{n}x{n // 2 + 1} "
+                        "complex, 1x1 pencil grid, permute_dims",
+            "shape": list(shape),
+            "output_layout": "[kz][ky][kx]",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_GBPS,
+            "traffic": traffic,
+            "kernel": "k_line_stage<float>",
+            "avg_launch_ms": avg_ms,
+            "algorithmic_bytes_per_launch": avg_bytes,
+        },
+        "route_model": {
+            "bytes_per_round_trip": rm,
+            "achieved_GBps": route_gbps,
+            "frac_of_peak": route_gbps / PEAK_GBPS,
+            "endpoint_check_ms": (sum(check_ms) / len(check_ms)) if check_ms else 0.0,
+        },
+        "accuracy": {"round_trip_rel_l2": rt_err},
+    }
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(n, threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

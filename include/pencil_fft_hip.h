@@ -1,0 +1,264 @@
+/*
+ * pencil_fft_hip.h - C ABI of the MI355X (gfx950) single-GPU FFT path of
+ * pencil-rs.
+ *
+ * The reference is Rust; its plugin seam for this path is the backend
+ * selection of crates/pencil-fft/src/backend.rs:12-131 (BackendKind,
+ * LocalBackend, per-line trait objects) plus the stage and transition
+ * executors of crates/pencil-fft/src/distributed.rs:3126-3867. A per-line
+ * trait object would force a host round trip per line, so this library
+ * replaces whole stages and transitions: one call runs a full plan operation
+ * on device-resident buffers. Every entry point below names the reference
+ * interface it replaces.
+ *
+ * Conventions
+ *  - All data pointers are device pointers (hipMalloc'd or equivalent).
+ *  - Complex values are interleaved (re, im), like num_complex::Complex.
+ *  - Arrays are laid out [extra...][memory-ordered spatial axes], the memory
+ *    order being the pencil permutation (crates/pencil-array/src/lib.rs:12-16).
+ *  - The process grid is one rank (a 1x1 / 1 topology); every route
+ *    transition is therefore a local permutation fused into a stage kernel.
+ *  - Validation completes before any device write: a rejected call leaves
+ *    src, dst and workspace untouched (crates/pencil-fft/src/lib.rs:1015-1077).
+ *  - Execution is stream ordered. Errors found on the device (an invalid C2R
+ *    spectrum) gate the writing kernels on the device, so dst is not written,
+ *    and are reported synchronously unless PFH_EXEC_DEFER_STATUS is passed;
+ *    deferred status is read with pfh_workspace_take_status.
+ */
+#ifndef PENCIL_FFT_HIP_H
+#define PENCIL_FFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *pfh_stream_t; /* hipStream_t */
+
+typedef struct pfh_plan_s *pfh_plan;
+
+/* Status codes. Each maps to one variant of the reference's error enums:
+ * LocalC2cError (lib.rs:337-360), LocalR2cError (r2c.rs:24-109),
+ * LocalR2rError (r2r.rs:204-235), FftError (distributed.rs:338-418),
+ * R2cError (distributed.rs:440-455), LocalTransposeError
+ * (local_transpose.rs:12-44) and TransposeError (transpose.rs:112-175). */
+typedef enum pfh_status {
+    PFH_OK = 0,
+    PFH_ERR_INVALID_LENGTH = 1,           /* Local*Error::InvalidLength */
+    PFH_ERR_LENGTH_OVERFLOW = 2,          /* Local*Error::LengthOverflow */
+    PFH_ERR_NON_INTEGRAL_BATCH = 3,       /* Local*Error::NonIntegralBatch */
+    PFH_ERR_BUFFER_LENGTH_MISMATCH = 4,   /* LocalC2cError/LocalR2rError::BufferLengthMismatch */
+    PFH_ERR_BATCH_COUNT_MISMATCH = 5,     /* LocalR2cError::BatchCountMismatch {source_count, destination_count} */
+    PFH_ERR_SCRATCH_TOO_SMALL = 6,        /* ScratchTooSmall {required, actual} */
+    PFH_ERR_INVALID_SPECTRUM_ENDPOINT = 7,/* LocalR2cError::InvalidSpectrumEndpoint {batch, index} */
+    PFH_ERR_INVALID_DIMENSIONS = 8,       /* FftError::InvalidDimensions */
+    PFH_ERR_INVALID_INPUT_LAYOUT = 9,     /* FftError::InvalidInputLayout */
+    PFH_ERR_INPUT_LAYOUT_MISMATCH = 10,   /* FftError::InputLayoutMismatch */
+    PFH_ERR_OUTPUT_LAYOUT_MISMATCH = 11,  /* FftError::OutputLayoutMismatch */
+    PFH_ERR_EXTRA_SHAPE_MISMATCH = 12,    /* FftError::ExtraShapeMismatch */
+    PFH_ERR_WORKSPACE_MISMATCH = 13,      /* FftError::WorkspaceMismatch */
+    PFH_ERR_WORKSPACE_TOO_SMALL = 14,     /* FftError::WorkspaceTooSmall {required, actual} */
+    PFH_ERR_PREPARATION_FAILED = 15,      /* FftError::PreparationFailed */
+    PFH_ERR_ALLOCATION_FAILED = 16,       /* FftError::AllocationFailed */
+    PFH_ERR_INVALID_SPECTRUM = 17,        /* R2cError::InvalidSpectrum */
+    PFH_ERR_INCOMPATIBLE_DISTRIBUTION = 18, /* LocalTransposeError::IncompatibleDistribution */
+    PFH_ERR_SOURCE_LAYOUT_MISMATCH = 19,  /* LocalTransposeError::SourceLayoutMismatch */
+    PFH_ERR_DESTINATION_LAYOUT_MISMATCH = 20, /* LocalTransposeError::DestinationLayoutMismatch */
+    PFH_ERR_INVALID_PERMUTATION = 21,     /* AxisError::{OutOfBounds, Duplicate} (axis.rs:40-66) */
+    PFH_ERR_ZERO_EXTENT = 22,             /* PencilError::ZeroGlobalExtent */
+    PFH_ERR_INVALID_ARGUMENT = 23,        /* null pointer / unknown enum value (no Rust analogue: types forbid it) */
+    PFH_ERR_UNSUPPORTED = 24,             /* a line length the device kernels do not cover */
+    PFH_ERR_DEVICE = 25                   /* a HIP runtime call failed */
+} pfh_status;
+
+/* Payload of the last error on the calling thread (Rust error fields). */
+typedef struct pfh_error_info {
+    pfh_status status;
+    size_t required;   /* ScratchTooSmall / WorkspaceTooSmall */
+    size_t actual;
+    size_t batch;      /* InvalidSpectrumEndpoint / BatchCountMismatch source */
+    size_t index;      /* InvalidSpectrumEndpoint / BatchCountMismatch destination */
+    int hip_error;     /* hipError_t for PFH_ERR_DEVICE */
+    char message[160];
+} pfh_error_info;
+
+typedef enum pfh_precision { PFH_F32 = 1, PFH_F64 = 2 } pfh_precision;
+
+typedef enum pfh_kind {
+    /* local line plans: contiguous lines, batch = buffer length / line length */
+    PFH_LOCAL_C2C = 1,  /* LocalC2cPlan       lib.rs:367-655 */
+    PFH_LOCAL_R2C = 2,  /* LocalR2cPlan       r2c.rs:260-813 */
+    PFH_LOCAL_R2R = 3,  /* LocalR2rPlan       r2r.rs:287-778 */
+    PFH_LOCAL_DHT = 4,  /* LocalDhtPlan       dht.rs:32-303 */
+    /* distributed plans on a one-rank process grid */
+    PFH_C2C = 5,        /* C2cPlan            distributed.rs:1016-1800 */
+    PFH_R2C = 6,        /* R2cPlan            distributed/r2c.rs:255-1130 */
+    PFH_R2R = 7,        /* R2rPlan / DhtPlan  distributed/r2r.rs:71-1460 */
+    /* layout permutations */
+    PFH_LOCAL_TRANSPOSE = 8,    /* LocalTransposePlan       local_transpose.rs:46-277 */
+    PFH_ALLTOALLV_TRANSPOSE = 9 /* AllToAllvTransposePlan   alltoallv_transpose.rs:50-311 (one peer) */
+} pfh_kind;
+
+/* R2R kind codes = the reference's descriptor codes (r2r.rs:53-64, :101-106). */
+typedef enum pfh_r2r_kind {
+    PFH_IDENTITY = 0,
+    PFH_DCT1 = 1, PFH_DCT2 = 2, PFH_DCT3 = 3, PFH_DCT4 = 4,
+    PFH_DST1 = 5, PFH_DST2 = 6, PFH_DST3 = 7, PFH_DST4 = 8,
+    PFH_DHT = 9
+} pfh_r2r_kind;
+
+/* Plan operations. FORWARD / INVERSE / BACKWARD are the reference's
+ * forward, inverse (normalized) and backward (raw) methods. The adjoint
+ * operations have no reference counterpart: each is the exact transpose of
+ * the named operation under the real inner product (complex values count as
+ * two reals). */
+typedef enum pfh_op {
+    PFH_FORWARD = 0,
+    PFH_INVERSE = 1,
+    PFH_BACKWARD = 2,
+    PFH_ADJOINT_FORWARD = 3,  /* adjoint of FORWARD  (output-layout input) */
+    PFH_ADJOINT_INVERSE = 4,  /* adjoint of INVERSE  (input-layout input) */
+    PFH_ADJOINT_BACKWARD = 5  /* adjoint of BACKWARD (input-layout input) */
+} pfh_op;
+
+enum {
+    PFH_EXEC_DEFER_STATUS = 1 /* do not synchronize for device-found errors */
+};
+
+#define PFH_MAX_DIMS 4
+
+/* Plan descriptor. Fields unused by a kind are ignored. */
+typedef struct pfh_desc {
+    int kind;                       /* pfh_kind */
+    int precision;                  /* pfh_precision */
+    int ndim;                       /* N: spatial dimensions */
+    int topology_dims;              /* M: MpiTopology<M> (all extents 1); 0 = N-1 */
+    size_t shape[PFH_MAX_DIMS];     /* global logical shape (the real shape for R2C) */
+    int extra_ndim;
+    size_t extra_shape[PFH_MAX_DIMS];
+    unsigned selected;              /* AxisSelection mask (bit a = axis a); 0 = all */
+    int axis_kind[PFH_MAX_DIMS];    /* R2R: pfh_r2r_kind per axis */
+    int direction[PFH_MAX_DIMS];    /* C2C: 0 FourierDirection::Forward, 1 Backward */
+    int permute_dims;               /* DistributedLayout::permute_dims */
+    int transpose_method;           /* TransposeMethod: 0 AllToAllv, 1 PointToPoint */
+    int src_perm[PFH_MAX_DIMS];     /* transposes: source memory order */
+    int dst_perm[PFH_MAX_DIMS];     /* transposes: destination memory order */
+    size_t element_bytes;           /* transposes: bytes per element (any size) */
+    size_t line_len;                /* local plans: line length (real length for R2C) */
+    int r2r_kind;                   /* PFH_LOCAL_R2R: kind (1..8) */
+    int flags;                      /* reserved, 0 */
+} pfh_desc;
+
+/* Stage timing (TransformTiming / StageTiming, distributed.rs:290-325). */
+typedef struct pfh_transform_timing {
+    int stages;              /* number of entries below */
+    double stage_ms[12];     /* device time of each entry, in execution order */
+    int stage_axis[12];      /* transformed logical axis; -1 for an endpoint check */
+    double stage_bytes[12];  /* algorithmic HBM bytes of the entry (one read + one write) */
+    double total_ms;
+} pfh_transform_timing;
+
+const char *pfh_version(void);
+void pfh_last_error(pfh_error_info *out);
+const char *pfh_status_string(pfh_status status);
+
+/* Plan construction: LocalC2cPlan::new (lib.rs:391-438), LocalR2cPlan::new
+ * (r2c.rs:260-400), LocalR2rPlan::new (r2r.rs:471-495), LocalDhtPlan::new
+ * (dht.rs:53-68), C2cPlan::from_shape_with_layout (distributed.rs:1016-1300),
+ * R2cPlan::from_shape* (distributed/r2c.rs:255-420, construct_oriented
+ * :1204-1448), R2rPlan/DhtPlan::from_shape (distributed/r2r.rs:71-360),
+ * LocalTransposePlan::new (local_transpose.rs:59-71),
+ * AllToAllvTransposePlan::new (alltoallv_transpose.rs:50-106). Twiddle
+ * tables are built here (FP64-generated, rounded to the plan precision) and
+ * uploaded to the current device; the plan is immutable afterwards. */
+pfh_status pfh_plan_create(const pfh_desc *desc, pfh_plan *out);
+void pfh_plan_destroy(pfh_plan plan);
+
+/* Workspace bytes needed by op (TransposeWorkspaceRequirements,
+ * transpose.rs:226-254, scratch_len / embedding_len of the local plans). The
+ * workspace also holds the plan's small device status area. */
+pfh_status pfh_plan_workspace_bytes(pfh_plan plan, int op, size_t *bytes);
+
+/* Layout of the plan's input (side 0) or output (side 1) array for the
+ * forward direction: input_pencil() / output_pencil()
+ * (distributed.rs:1316-1329). perm: memory order; shape: logical shape;
+ * elements: elements per array (extras included); element_bytes: bytes per
+ * element. */
+pfh_status pfh_plan_layout(pfh_plan plan, int side, int perm[PFH_MAX_DIMS], size_t shape[PFH_MAX_DIMS],
+                           size_t *elements, size_t *element_bytes);
+
+/* Distributed-plan execution, out of place, source preserved:
+ * C2cPlan::{forward, inverse, backward} (distributed.rs:1448-1500),
+ * R2cPlan::{forward, inverse, backward} (distributed/r2c.rs:856-1000),
+ * R2rPlan/DhtPlan::{forward, inverse, backward} (distributed/r2r.rs:1271+),
+ * LocalTransposePlan::execute_views (local_transpose.rs:87-132),
+ * AllToAllvTransposePlan::execute_views (alltoallv_transpose.rs:107-160),
+ * plus the adjoint operations. src_elems / dst_elems are the element counts
+ * of the caller's buffers (checked like the Rust slice lengths). */
+pfh_status pfh_execute(pfh_plan plan, int op, const void *src, size_t src_elems, void *dst, size_t dst_elems,
+                       void *workspace, size_t workspace_bytes, pfh_stream_t stream, int flags);
+
+/* In-place execution in one buffer of capacity_bytes (C2cPlan::*_in_place,
+ * distributed.rs:3672-3774; R2cPlan in-place arrays; local *_in_place
+ * methods; LocalTransposePlan::execute_in_place, local_transpose.rs:159-243). */
+pfh_status pfh_execute_in_place(pfh_plan plan, int op, void *data, size_t capacity_bytes, void *workspace,
+                                size_t workspace_bytes, pfh_stream_t stream, int flags);
+
+/* Local line plans: LocalC2cPlan::{forward, backward, inverse}
+ * (lib.rs:498-619), LocalR2cPlan::{forward, inverse, backward}
+ * (r2c.rs:428-521), LocalR2rPlan / LocalDhtPlan::{forward, backward,
+ * inverse} (r2r.rs:530-621, dht.rs:133-200). The batch is
+ * src_elems / (input line length); BatchCountMismatch / NonIntegralBatch /
+ * BufferLengthMismatch are checked as in the reference. Local plans need no
+ * workspace except C2R (its endpoint preflight status area). */
+pfh_status pfh_local_execute(pfh_plan plan, int op, const void *src, size_t src_elems, void *dst,
+                             size_t dst_elems, void *workspace, size_t workspace_bytes, pfh_stream_t stream,
+                             int flags);
+
+/* In-place local execution: LocalC2cPlan::*_in_place (lib.rs:556-619),
+ * LocalR2cPlan::{forward,inverse,backward}_in_place on the packed array
+ * (r2c.rs:532-623), LocalR2rPlan / LocalDhtPlan::*_in_place. elems is the
+ * number of input elements (reals for an R2C forward, complex values for a
+ * C2R); capacity_bytes must hold the larger of input and output. R2C / C2R
+ * stage their input in the workspace (status area + input bytes). */
+pfh_status pfh_local_execute_in_place(pfh_plan plan, int op, void *data, size_t elems, size_t capacity_bytes,
+                                      void *workspace, size_t workspace_bytes, pfh_stream_t stream, int flags);
+
+/* *_with_timing (distributed.rs:290-325, 3796-3867): same as pfh_execute,
+ * synchronous, with per-step device times from HIP events. */
+pfh_status pfh_execute_with_timing(pfh_plan plan, int op, const void *src, size_t src_elems, void *dst,
+                                   size_t dst_elems, void *workspace, size_t workspace_bytes, pfh_stream_t stream,
+                                   pfh_transform_timing *timing);
+
+/* Reads and clears the sticky device status of a workspace after
+ * PFH_EXEC_DEFER_STATUS executions (synchronizes the stream). */
+pfh_status pfh_workspace_take_status(pfh_plan plan, void *workspace, pfh_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Batched uneven transforms (no reference counterpart; the reference runs */
+/* collections sequentially, distributed/collections.rs:229-311).          */
+/* ---------------------------------------------------------------------- */
+
+typedef struct pfh_batch_s *pfh_batch;
+
+/* Builds one persistent launch per stage over `count` independent C2C
+ * plans (forward or inverse), all of one precision, uneven shapes allowed.
+ * Tiles, radix split and lines per block come from the autotuner when
+ * autotune != 0 (measured on the current device), else from the planner's
+ * defaults. */
+pfh_status pfh_batch_create(const pfh_desc *descs, int count, int autotune, pfh_batch *out);
+void pfh_batch_destroy(pfh_batch batch);
+pfh_status pfh_batch_workspace_bytes(pfh_batch batch, size_t *bytes);
+/* srcs[i] / dsts[i]: device buffers of problem i (input / output layout of
+ * its plan). use_graph != 0 captures the launches once into a hipGraph for
+ * these pointers and replays it. */
+pfh_status pfh_batch_execute(pfh_batch batch, int op, const void *const *srcs, void *const *dsts, void *workspace,
+                             size_t workspace_bytes, pfh_stream_t stream, int use_graph);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PENCIL_FFT_HIP_H */

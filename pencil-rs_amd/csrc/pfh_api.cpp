@@ -1,0 +1,1581 @@
+// Host planner and C ABI of the gfx950 FFT path.
+//
+// A plan operation is a list of Steps; each Step is one k_line_stage launch
+// that transforms one logical axis and, in the same pass, moves the array
+// into the next layout of the route. The route itself is the reference's
+// (build_route, crates/pencil-fft/src/distributed.rs:2306-2373): stage s
+// transforms axis N-1-s; with permute_dims each stage's pencil moves its axis
+// last. Which side of a stage absorbs each transition, and which buffer
+// holds each intermediate, is chosen here to minimise workspace.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "pfh_plan.hpp"
+
+using namespace pfh;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+static thread_local pfh_error_info g_err;
+
+static pfh_status set_err(pfh_status st, const char *msg, size_t required = 0, size_t actual = 0, size_t batch = 0,
+                          size_t index = 0, int hip_error = 0)
+{
+    g_err.status = st;
+    g_err.required = required;
+    g_err.actual = actual;
+    g_err.batch = batch;
+    g_err.index = index;
+    g_err.hip_error = hip_error;
+    std::snprintf(g_err.message, sizeof(g_err.message), "%s", msg ? msg : "");
+    return st;
+}
+
+static pfh_status ok()
+{
+    g_err.status = PFH_OK;
+    g_err.message[0] = 0;
+    return PFH_OK;
+}
+
+#define PFH_HIP_CHECK(expr)                                                                     \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return set_err(PFH_ERR_DEVICE, hipGetErrorString(e_), 0, 0, 0, 0, (int)e_); \
+    } while (0)
+
+namespace pfh {
+PlanImpl::~PlanImpl()
+{
+    if (dev_tables) (void)hipFree(dev_tables);
+}
+} // namespace pfh
+
+struct pfh_plan_s {
+    PlanImpl impl;
+};
+
+// ---------------------------------------------------------------------------
+// device information
+// ---------------------------------------------------------------------------
+
+static std::mutex g_dev_mu;
+static std::vector<DeviceInfo> g_devs;
+static std::vector<int> g_dev_ready;
+
+static pfh_status device_info(DeviceInfo &out)
+{
+    int dev = 0;
+    PFH_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if ((int)g_devs.size() <= dev) {
+        g_devs.resize(dev + 1);
+        g_dev_ready.resize(dev + 1, 0);
+    }
+    if (!g_dev_ready[dev]) {
+        hipDeviceProp_t prop;
+        PFH_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+        DeviceInfo d;
+        d.device = dev;
+        d.cus = prop.multiProcessorCount;
+        size_t lim = prop.sharedMemPerBlock;
+        if (lim < 65536) lim = 65536;
+        if (lim > 65536) {
+            if (set_line_stage_lds_limit(lim) != hipSuccess) lim = 65536;
+        }
+        d.lds_limit = lim;
+        g_devs[dev] = d;
+        g_dev_ready[dev] = 1;
+    }
+    out = g_devs[dev];
+    return PFH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tables: exp(-2 pi i q / d) computed in long double and rounded
+// ---------------------------------------------------------------------------
+
+static const long double kPiL = 3.141592653589793238462643383279502884L;
+
+static void push_cexp(std::vector<double> &t, long long q, long long d, bool positive = false)
+{
+    long long r = q % d;
+    if (r < 0) r += d;
+    long double a = -2.0L * kPiL * (long double)r / (long double)d;
+    long double c = cosl(a), s = sinl(a);
+    if ((4 * r) % d == 0) { // exact quarter turns
+        const long long qt = (4 * r) / d;
+        c = (qt == 0) ? 1.0L : (qt == 2 ? -1.0L : 0.0L);
+        s = (qt == 1) ? -1.0L : (qt == 3 ? 1.0L : 0.0L);
+    }
+    if (positive) s = -s;
+    t.push_back((double)c);
+    t.push_back((double)s);
+}
+
+// Radix split of the complex FFT length m: powers of two in 16/8/4/2
+// passes, then the odd primes. Empty when m has a factor beyond 13.
+static bool radix_plan(int m, std::vector<int> &out, int max_radix)
+{
+    out.clear();
+    if (m <= 1) return true;
+    int a = 0, r = m;
+    while (r % 2 == 0) {
+        r /= 2;
+        ++a;
+    }
+    std::vector<int> p2;
+    if (max_radix < 16) { // 8 / 4 / 2 passes only
+        int left = a;
+        while (left >= 3) {
+            p2.push_back(8);
+            left -= 3;
+        }
+        if (left == 2) p2.push_back(4);
+        else if (left == 1) p2.push_back(2);
+        a = -1;
+    }
+    switch (a) {
+    case -1: break;
+    case 0: break;
+    case 1: p2 = {2}; break;
+    case 2: p2 = {4}; break;
+    case 3: p2 = {8}; break;
+    case 4: p2 = {16}; break;
+    case 5: p2 = {8, 4}; break;
+    case 6: p2 = {8, 8}; break;
+    case 7: p2 = {16, 8}; break;
+    case 8: p2 = {16, 16}; break;
+    case 9: p2 = {8, 8, 8}; break;
+    default: {
+        int left = a;
+        while (left >= 4) {
+            p2.push_back(16);
+            left -= 4;
+        }
+        if (left == 3) p2.push_back(8);
+        else if (left == 2) p2.push_back(4);
+        else if (left == 1) p2.push_back(2);
+    }
+    }
+    for (int x : p2) out.push_back(x);
+    for (int p : {3, 5, 7, 11, 13}) {
+        while (r % p == 0) {
+            out.push_back(p);
+            r /= p;
+        }
+    }
+    // remaining prime factors run through the generic O(R) pass
+    for (int p = 17; (int64_t)p * p <= r; p += 2) {
+        while (r % p == 0) {
+            out.push_back(p);
+            r /= p;
+        }
+    }
+    if (r > 1) out.push_back(r);
+    if ((int)out.size() > kMaxPasses) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// stage specifications
+// ---------------------------------------------------------------------------
+
+enum StageKind { SK_COPY, SK_C2C, SK_R2C, SK_C2R, SK_R2R };
+
+struct StageSpec {
+    int kind = SK_COPY;
+    int axis = 0;
+    bool in_c = true, out_c = true;
+    int64_t n = 1;
+    bool sign_pos = false;
+    double scale = 1.0;
+    int r2r = 0; // applied kind (1..9)
+    int flags = 0;
+    double prew0 = 1, prew1 = 1, postw0 = 1, postw1 = 1;
+    Layout in_natural, out_natural; // shape/elem on each side with the stage's natural perm
+};
+
+static int r2r_backward_kind(int k)
+{
+    switch (k) {
+    case PFH_DCT2: return PFH_DCT3;
+    case PFH_DCT3: return PFH_DCT2;
+    case PFH_DST2: return PFH_DST3;
+    case PFH_DST3: return PFH_DST2;
+    default: return k;
+    }
+}
+
+static int64_t r2r_norm(int k, int64_t n)
+{
+    switch (k) {
+    case PFH_DCT1: return 2 * (n - 1);
+    case PFH_DST1: return 2 * (n + 1);
+    case PFH_DHT: return n;
+    default: return 2 * n;
+    }
+}
+
+// Resolves the R2R stage for a plan operation (see the adjoint weights in
+// the header comment of the oracle and pencil_fft_hip.h).
+static void r2r_stage(StageSpec &s, int kind, int64_t n, int op)
+{
+    s.kind = SK_R2R;
+    s.in_c = s.out_c = false;
+    s.n = n;
+    const bool adjoint = (op == PFH_ADJOINT_FORWARD || op == PFH_ADJOINT_BACKWARD || op == PFH_ADJOINT_INVERSE);
+    const bool scaled = (op == PFH_INVERSE || op == PFH_ADJOINT_INVERSE);
+    int base = (op == PFH_FORWARD || op == PFH_ADJOINT_FORWARD) ? kind : r2r_backward_kind(kind);
+    if (scaled) s.scale = 1.0 / (double)r2r_norm(kind, n);
+    s.r2r = base;
+    if (!adjoint || kind == PFH_DHT) return;
+    switch (base) {
+    case PFH_DCT1:
+        s.prew0 = 2;
+        s.prew1 = 2;
+        s.postw0 = 0.5;
+        s.postw1 = 0.5;
+        s.flags |= F_WEIGHTS;
+        break;
+    case PFH_DCT2:
+        s.r2r = PFH_DCT3;
+        s.prew0 = 2;
+        s.flags |= F_WEIGHTS;
+        break;
+    case PFH_DCT3:
+        s.r2r = PFH_DCT2;
+        s.postw0 = 0.5;
+        s.flags |= F_WEIGHTS;
+        break;
+    case PFH_DST2:
+        s.r2r = PFH_DST3;
+        s.prew1 = 2;
+        s.flags |= F_WEIGHTS;
+        break;
+    case PFH_DST3:
+        s.r2r = PFH_DST2;
+        s.postw1 = 0.5;
+        s.flags |= F_WEIGHTS;
+        break;
+    default:
+        break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// step construction
+// ---------------------------------------------------------------------------
+
+static int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+static int pow2ceil(int v)
+{
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+static bool is_generic_radix(int R)
+{
+    return !(R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8 || R == 11 || R == 13 || R == 16);
+}
+
+static bool is_wide_radix(int R) { return !(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16); }
+
+static bool mode_needs_pre(int mode)
+{
+    return !(mode == M_COPY_C || mode == M_COPY_R || mode == M_C2C || mode == M_R2C || mode == M_DHT);
+}
+
+// Fills mode, lengths, radices and this step's tables (appended to tab).
+static pfh_status fill_transform(Step &st, const StageSpec &sp, std::vector<double> &tab, int tbytes)
+{
+    const int64_t n = sp.n;
+    st.n = (int)n;
+    st.axis = sp.axis;
+    st.scale = sp.scale;
+    st.flags = sp.flags;
+    st.prew0 = sp.prew0;
+    st.prew1 = sp.prew1;
+    st.postw0 = sp.postw0;
+    st.postw1 = sp.postw1;
+    st.in_cplx = sp.in_c;
+    st.out_cplx = sp.out_c;
+    switch (sp.kind) {
+    case SK_COPY:
+        st.mode = sp.in_c ? M_COPY_C : M_COPY_R;
+        st.m = 0;
+        st.nin = st.nout = (int)n;
+        break;
+    case SK_C2C:
+        st.mode = M_C2C;
+        st.m = (int)n;
+        st.nin = st.nout = (int)n;
+        if (sp.sign_pos) st.flags |= F_CONJ_IN | F_CONJ_OUT;
+        break;
+    case SK_R2C:
+        st.mode = (n % 2 == 0) ? M_R2C : M_R2C_ODD;
+        st.m = (n % 2 == 0) ? (int)(n / 2) : (int)n;
+        st.nin = (int)n;
+        st.nout = (int)(n / 2 + 1);
+        break;
+    case SK_C2R:
+        st.mode = (n % 2 == 0) ? M_C2R : M_C2R_ODD;
+        st.m = (n % 2 == 0) ? (int)(n / 2) : (int)n;
+        st.nin = (int)(n / 2 + 1);
+        st.nout = (int)n;
+        break;
+    case SK_R2R: {
+        st.nin = st.nout = (int)n;
+        const bool even = (n % 2 == 0);
+        switch (sp.r2r) {
+        case PFH_DCT1: st.mode = M_DCT1; st.m = (int)(n - 1); break;
+        case PFH_DST1: st.mode = M_DST1; st.m = (int)(n + 1); break;
+        case PFH_DCT2: case PFH_DST2:
+            st.mode = even ? M_DCT2 : M_DCT2_ODD;
+            st.m = even ? (int)(n / 2) : (int)n;
+            if (sp.r2r == PFH_DST2) st.flags |= F_PRE_ALT | F_POST_REV;
+            break;
+        case PFH_DCT3: case PFH_DST3:
+            st.mode = even ? M_DCT3 : M_DCT3_ODD;
+            st.m = even ? (int)(n / 2) : (int)n;
+            if (sp.r2r == PFH_DST3) st.flags |= F_PRE_REV | F_POST_ALT;
+            break;
+        case PFH_DCT4: case PFH_DST4:
+            st.mode = even ? M_DCT4 : M_DCT4_ODD;
+            st.m = even ? (int)(n / 2) : (int)(2 * n);
+            if (sp.r2r == PFH_DST4) st.flags |= F_PRE_REV | F_POST_ALT;
+            break;
+        case PFH_DHT:
+            if (n == 1) { // bit-exact identity (dht.rs:215-224)
+                st.mode = M_COPY_R;
+                st.m = 0;
+                st.scale = 1.0;
+            } else {
+                st.mode = even ? M_DHT : M_DHT_ODD;
+                st.m = even ? (int)(n / 2) : (int)n;
+            }
+            break;
+        default:
+            return set_err(PFH_ERR_INVALID_ARGUMENT, "unknown R2R kind");
+        }
+        break;
+    }
+    }
+    if (sp.kind == SK_R2C && (sp.flags & F_HERM_DOUBLE)) st.flags |= F_HERM_DOUBLE;
+    if (sp.kind == SK_C2R && (sp.flags & F_HERM_HALF)) st.flags |= F_HERM_HALF;
+    if (!radix_plan(st.m, st.radices, tbytes == 4 ? 16 : 8))
+        return set_err(PFH_ERR_UNSUPPORTED, "FFT length needs too many passes");
+
+    // tables
+    st.tw_base = (int64_t)(tab.size() / 2);
+    int off = 0;
+    auto here = [&]() { return off; };
+    int Ns = 1;
+    for (size_t p = 0; p < st.radices.size(); ++p) {
+        const int R = st.radices[p];
+        st.twoff[p] = here();
+        if (is_generic_radix(R)) { // roots exp(-2 pi i e / R) precede the pass twiddles
+            for (int e = 0; e < R; ++e) {
+                push_cexp(tab, e, R);
+                ++off;
+            }
+        }
+        if (Ns > 1) {
+            for (int k = 0; k < Ns; ++k)
+                for (int r = 1; r < R; ++r) {
+                    push_cexp(tab, (long long)r * k, (long long)Ns * R);
+                    ++off;
+                }
+        }
+        Ns *= R;
+    }
+    const int m = st.m;
+    auto split_table = [&]() { // exp(-2 pi i k / 2m), k = 0..m
+        int o = here();
+        for (int k = 0; k <= m; ++k) {
+            push_cexp(tab, k, 2LL * m);
+            ++off;
+        }
+        return o;
+    };
+    switch (st.mode) {
+    case M_R2C: case M_DCT1: case M_DST1: case M_DHT:
+        st.post0 = split_table();
+        break;
+    case M_DCT2:
+        st.post0 = split_table();
+        st.post1 = here();
+        for (int k = 0; k < n; ++k) { push_cexp(tab, k, 4 * n); ++off; }
+        break;
+    case M_DCT2_ODD:
+        st.post0 = here();
+        for (int k = 0; k < n; ++k) { push_cexp(tab, k, 4 * n); ++off; }
+        break;
+    case M_C2R:
+        st.pre0 = here();
+        for (int j = 0; j < m; ++j) { push_cexp(tab, j, n, true); ++off; }
+        break;
+    case M_DCT3:
+        st.pre0 = here();
+        for (int k = 0; k <= m; ++k) { push_cexp(tab, k, 4 * n, true); ++off; }
+        st.pre1 = here();
+        for (int j = 0; j < m; ++j) { push_cexp(tab, j, n, true); ++off; }
+        break;
+    case M_DCT3_ODD:
+        st.pre0 = here();
+        for (int k = 0; k < n; ++k) { push_cexp(tab, k, 4 * n); ++off; }
+        break;
+    case M_DCT4:
+        st.pre0 = here();
+        for (int j = 0; j < m; ++j) { push_cexp(tab, j, 2 * n); ++off; }
+        st.post0 = here();
+        for (int k = 0; k < m; ++k) { push_cexp(tab, 4LL * k + 1, 8 * n); ++off; }
+        break;
+    case M_DCT4_ODD:
+        st.pre0 = here();
+        for (int j = 0; j < n; ++j) { push_cexp(tab, j, 4 * n); ++off; }
+        st.post0 = here();
+        for (int k = 0; k < n; ++k) { push_cexp(tab, 2LL * k + 1, 8 * n); ++off; }
+        break;
+    default:
+        break;
+    }
+    st.twn = off;
+    return PFH_OK;
+}
+
+// Line grid, access patterns and launch shape of a step whose in/out
+// layouts are set.
+static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes)
+{
+    const int a = st.axis;
+    const auto sti = st.in.strides(), sto = st.out.strides();
+    st.ips = sti[a];
+    st.ops = sto[a];
+    std::vector<int> others;
+    for (int b = 0; b < st.in.N; ++b)
+        if (b != a && st.in.shape[b] > 1) others.push_back(b);
+    st.in_col = (st.ips != 1 && st.nin > 1) ? 1 : 0;
+    st.out_col = (st.ops != 1 && st.nout > 1) ? 1 : 0;
+    if (!st.in_col && st.out_col)
+        std::sort(others.begin(), others.end(), [&](int x, int y) { return sto[x] < sto[y]; });
+    else
+        std::sort(others.begin(), others.end(), [&](int x, int y) { return sti[x] < sti[y]; });
+    LineGeom g{};
+    int nd = 0;
+    int64_t nlines = 1;
+    for (int b : others) {
+        if (nd >= kMaxDigits) return set_err(PFH_ERR_UNSUPPORTED, "too many line digits");
+        g.ext[nd] = st.in.shape[b];
+        g.is[nd] = sti[b];
+        g.os[nd] = sto[b];
+        nlines *= st.in.shape[b];
+        ++nd;
+    }
+    if (st.in.extra > 1) {
+        if (nd >= kMaxDigits) return set_err(PFH_ERR_UNSUPPORTED, "too many line digits");
+        g.ext[nd] = st.in.extra;
+        g.is[nd] = st.in.len();
+        g.os[nd] = st.out.len();
+        nlines *= st.in.extra;
+        ++nd;
+    }
+    g.nd = nd;
+    st.g = g;
+    st.nlines = nlines;
+
+    const int cb = 2 * tbytes;
+    const int in_eb = st.in_cplx ? cb : tbytes, out_eb = st.out_cplx ? cb : tbytes;
+    int lt_col = 1;
+    if (st.in_col) lt_col = std::max(lt_col, 128 / in_eb);
+    if (st.out_col) lt_col = std::max(lt_col, 128 / out_eb);
+    const int emax = (tbytes == 4) ? 16 : 8;
+    int need = 1;
+    for (int R : st.radices) {
+        if (is_generic_radix(R)) {
+            need = std::max(need, ceil_div(st.m, emax));
+            continue;
+        }
+        const int MB = std::max(1, emax / R);
+        need = std::max(need, ceil_div(st.m / R, MB));
+    }
+    if (mode_needs_pre(st.mode)) need = std::max(need, ceil_div(st.m, emax));
+    int tpl = pow2ceil(need);
+    int lt = (st.in_col || st.out_col) ? lt_col : std::max(1, 256 / tpl);
+    while (lt > 1 && (int64_t)(lt / 2) >= nlines) lt /= 2;
+    while (lt * tpl < 64) tpl *= 2;
+    while (lt * tpl > kMaxThreads && lt > 1) lt /= 2;
+    if (lt * tpl > kMaxThreads) return set_err(PFH_ERR_UNSUPPORTED, "line too long for one workgroup");
+    int need_c = std::max(st.m, 1);
+    need_c = std::max(need_c, st.in_cplx ? st.nin : (st.nin + 1) / 2);
+    need_c = std::max(need_c, st.out_cplx ? st.nout : (st.nout + 1) / 2);
+    st.ls = (need_c % 2 == 0) ? need_c + 1 : need_c;
+    auto lds_of = [&](int l) { return (size_t)(st.twn + (int64_t)l * st.ls) * (size_t)cb; };
+    while (lds_of(lt) > dev.lds_limit && lt > 1) {
+        lt /= 2;
+        while (lt * tpl < 64) tpl *= 2;
+    }
+    if (lds_of(lt) > dev.lds_limit) return set_err(PFH_ERR_UNSUPPORTED, "line does not fit in LDS");
+    st.lt = lt;
+    st.tpl = tpl;
+    st.threads = lt * tpl;
+    st.lds_bytes = lds_of(lt);
+    st.tiles = (nlines + lt - 1) / lt;
+    if (st.tiles >= 64) {
+        st.flags |= F_XCD_REMAP;
+        st.xcd_per = (int)((st.tiles + 7) / 8);
+        st.grid = 8LL * st.xcd_per;
+    } else {
+        st.flags &= ~F_XCD_REMAP;
+        st.xcd_per = 0;
+        st.grid = st.tiles;
+    }
+    return PFH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// routes
+// ---------------------------------------------------------------------------
+
+static void build_route(int N, bool permute, std::vector<std::array<int, 8>> &perm)
+{
+    perm.assign(N, {});
+    for (int p = 0; p < N; ++p) perm[0][p] = p;
+    int s = 1;
+    for (int axis = N - 2; axis >= 0; --axis, ++s) {
+        if (!permute) {
+            for (int p = 0; p < N; ++p) perm[s][p] = p;
+            continue;
+        }
+        int q = 0;
+        for (int p = 0; p < N; ++p)
+            if (perm[s - 1][p] != axis) perm[s][q++] = perm[s - 1][p];
+        perm[s][N - 1] = axis;
+    }
+}
+
+static Layout make_layout(int N, const std::array<int, 8> &perm, const std::array<int64_t, 8> &shape, int64_t extra,
+                          int elem)
+{
+    Layout L;
+    L.N = N;
+    L.perm = perm;
+    L.shape = shape;
+    L.extra = extra;
+    L.elem = elem;
+    return L;
+}
+
+// Stage specs of one operation in execution order.
+static pfh_status route_specs(const PlanImpl &P, int op, std::vector<StageSpec> &specs)
+{
+    const pfh_desc &d = P.desc;
+    const int N = P.N;
+    const int tb = (P.precision == PFH_F32) ? 4 : 8;
+    std::vector<std::array<int, 8>> perm;
+    build_route(N, d.permute_dims != 0, perm);
+    std::array<int64_t, 8> orig{}, red{};
+    for (int a = 0; a < N; ++a) orig[a] = red[a] = (int64_t)d.shape[a];
+    unsigned sel = d.selected ? d.selected : ((1u << N) - 1u);
+    int boundary = -1, raxis = -1;
+    if (P.kind == PFH_R2C) {
+        for (int a = N - 1; a >= 0; --a)
+            if (sel & (1u << a)) {
+                raxis = a;
+                break;
+            }
+        boundary = N - 1 - raxis;
+        red[raxis] = orig[raxis] / 2 + 1;
+    }
+    const bool fwd = (op == PFH_FORWARD || op == PFH_ADJOINT_INVERSE || op == PFH_ADJOINT_BACKWARD);
+    std::vector<StageSpec> by_stage(N);
+    for (int s = 0; s < N; ++s) {
+        StageSpec sp;
+        const int axis = N - 1 - s;
+        sp.axis = axis;
+        // forward-orientation shapes / element kinds
+        std::array<int64_t, 8> sh_in = orig, sh_out = orig;
+        bool c_in = true, c_out = true;
+        if (P.kind == PFH_R2C) {
+            sh_in = (s <= boundary) ? orig : red;
+            sh_out = (s < boundary) ? orig : red;
+            c_in = s > boundary;
+            c_out = s >= boundary;
+        } else if (P.kind == PFH_R2R) {
+            c_in = c_out = false;
+        }
+        if (P.kind == PFH_C2C) {
+            sp.n = orig[axis];
+            if (!(sel & (1u << axis))) {
+                sp.kind = SK_COPY;
+            } else {
+                sp.kind = SK_C2C;
+                const bool neg_forward = d.direction[axis] == 0;
+                sp.sign_pos = fwd ? !neg_forward : neg_forward;
+                if (op == PFH_INVERSE || op == PFH_ADJOINT_INVERSE) sp.scale = 1.0 / (double)orig[axis];
+            }
+        } else if (P.kind == PFH_R2C) {
+            if (s < boundary) {
+                sp.kind = SK_COPY;
+                sp.n = orig[axis];
+            } else if (s > boundary) {
+                sp.n = red[axis];
+                if (!(sel & (1u << axis))) {
+                    sp.kind = SK_COPY;
+                } else {
+                    sp.kind = SK_C2C;
+                    sp.sign_pos = !fwd;
+                    if (op == PFH_INVERSE || op == PFH_ADJOINT_INVERSE) sp.scale = 1.0 / (double)red[axis];
+                }
+            } else {
+                sp.n = orig[axis];
+                if (fwd) {
+                    sp.kind = SK_R2C;
+                    if (op != PFH_FORWARD) sp.flags |= F_HERM_DOUBLE;
+                } else {
+                    sp.kind = SK_C2R;
+                    if (op == PFH_ADJOINT_FORWARD) sp.flags |= F_HERM_HALF;
+                }
+                if (op == PFH_INVERSE || op == PFH_ADJOINT_INVERSE) sp.scale = 1.0 / (double)orig[axis];
+            }
+        } else { // R2R / DHT
+            const int k = d.axis_kind[axis];
+            sp.n = orig[axis];
+            if (k == 0) sp.kind = SK_COPY;
+            else r2r_stage(sp, k, orig[axis], op);
+        }
+        // element kinds and natural layouts in execution orientation
+        if (fwd) {
+            sp.in_c = c_in;
+            sp.out_c = c_out;
+            sp.in_natural = make_layout(N, perm[s], sh_in, P.extra, (c_in ? 2 : 1) * tb);
+            sp.out_natural = make_layout(N, perm[s], sh_out, P.extra, (c_out ? 2 : 1) * tb);
+        } else {
+            sp.in_c = c_out;
+            sp.out_c = c_in;
+            sp.in_natural = make_layout(N, perm[s], sh_out, P.extra, (c_out ? 2 : 1) * tb);
+            sp.out_natural = make_layout(N, perm[s], sh_in, P.extra, (c_in ? 2 : 1) * tb);
+        }
+        if (sp.kind == SK_COPY) {
+            sp.in_c = sp.out_c = fwd ? c_in : c_out;
+        }
+        by_stage[s] = sp;
+    }
+    specs.clear();
+    if (fwd)
+        for (int s = 0; s < N; ++s) specs.push_back(by_stage[s]);
+    else
+        for (int s = N - 1; s >= 0; --s) specs.push_back(by_stage[s]);
+    (void)raxis;
+    return PFH_OK;
+}
+
+// Chooses which side of each stage absorbs each transition and assigns
+// buffers (SRC, DST, W0, W1), minimising workspace. in_place: SRC == DST.
+template <typename Fn>
+static bool assign_route(const std::vector<StageSpec> &specs, bool in_place, int64_t dst_capacity, Fn &&emit,
+                         std::vector<Layout> &X, std::vector<Layout> &Y, std::vector<int> &inb, std::vector<int> &outb,
+                         int64_t wbytes[2])
+{
+    const int K = (int)specs.size();
+    int best_w = 99;
+    std::vector<Layout> bX, bY;
+    std::vector<int> bin, bout;
+    int64_t bw[2] = {0, 0};
+    for (int combo = (1 << (std::max(K - 1, 0) + K)) - 1; combo >= 0; --combo) {
+        const int choice = combo >> K;     // which transitions are absorbed at a store
+        const int keep = combo & ((1 << K) - 1); // which in-place-capable stages run in place
+        std::vector<Layout> x(K), y(K);
+        for (int e = 0; e < K; ++e) {
+            if (e == 0) x[e] = specs[e].in_natural;
+            else {
+                x[e] = y[e - 1];
+            }
+            if (e == K - 1) {
+                y[e] = specs[e].out_natural;
+            } else {
+                const bool absorb_store = (choice >> e) & 1;
+                y[e] = specs[e].out_natural;
+                if (absorb_store) y[e].perm = specs[e + 1].in_natural.perm;
+            }
+        }
+        // the last stage must produce the natural output; the first reads the natural input
+        std::vector<int> ib(K), ob(K);
+        ob[K - 1] = BUF_DST;
+        bool okc = true;
+        int64_t wb[2] = {0, 0};
+        for (int e = K - 1; e >= 0; --e) {
+            const bool same = x[e].same_order(y[e]) && specs[e].in_c == specs[e].out_c &&
+                              x[e].bytes() == y[e].bytes() && specs[e].kind != SK_R2C &&
+                              specs[e].kind != SK_C2R && ((keep >> e) & 1);
+            if (e == 0) {
+                ib[e] = BUF_SRC;
+                if (ob[e] == BUF_DST && in_place && !same) okc = false;
+                if (ob[e] == BUF_SRC) okc = false;
+                break;
+            }
+            if (same) {
+                ib[e] = ob[e];
+            } else {
+                const int64_t need = x[e].bytes();
+                int pick = -1;
+                if (ob[e] != BUF_DST && need <= dst_capacity) pick = BUF_DST;
+                else if (ob[e] != BUF_W0) pick = BUF_W0;
+                else pick = BUF_W1;
+                ib[e] = pick;
+            }
+            if (ib[e] == BUF_W0) wb[0] = std::max(wb[0], x[e].bytes());
+            if (ib[e] == BUF_W1) wb[1] = std::max(wb[1], x[e].bytes());
+            ob[e - 1] = ib[e];
+        }
+        if (!okc) continue;
+        // in place: stage 0 reads the data buffer, which is also DST
+        if (in_place) {
+            for (int e = 0; e < K; ++e) {
+                if (ib[e] == BUF_SRC) ib[e] = BUF_DST;
+                if (ob[e] == BUF_SRC) ob[e] = BUF_DST;
+            }
+            // a non in-place stage must not read and write the same buffer
+            bool bad = false;
+            for (int e = 0; e < K; ++e) {
+                const bool same = x[e].same_order(y[e]) && specs[e].in_c == specs[e].out_c &&
+                                  x[e].bytes() == y[e].bytes() && specs[e].kind != SK_R2C &&
+                                  specs[e].kind != SK_C2R && ((keep >> e) & 1);
+                if (ib[e] == ob[e] && !same) bad = true;
+            }
+            if (bad) continue;
+        }
+        const int nw = (wb[0] > 0 ? 1 : 0) + (wb[1] > 0 ? 1 : 0);
+        if (nw < best_w) {
+            best_w = nw;
+            bX = x;
+            bY = y;
+            bin = ib;
+            bout = ob;
+            bw[0] = wb[0];
+            bw[1] = wb[1];
+        }
+    }
+    (void)emit;
+    if (best_w == 99) return false;
+    X = bX;
+    Y = bY;
+    inb = bin;
+    outb = bout;
+    wbytes[0] = bw[0];
+    wbytes[1] = bw[1];
+    return true;
+}
+
+template <typename T>
+static pfh_status upload_tables(PlanImpl &P)
+{
+    const size_t nent = P.host_tables.size() / 2;
+    P.table_entries = (int64_t)nent;
+    if (nent == 0) return PFH_OK;
+    std::vector<T> tmp(P.host_tables.size());
+    for (size_t i = 0; i < P.host_tables.size(); ++i) tmp[i] = (T)P.host_tables[i];
+    PFH_HIP_CHECK(hipMalloc(&P.dev_tables, tmp.size() * sizeof(T)));
+    PFH_HIP_CHECK(hipMemcpy(P.dev_tables, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PFH_OK;
+}
+
+static const int kStatusBytes = 256;
+
+static int64_t align256(int64_t v) { return (v + 255) / 256 * 256; }
+
+static int64_t status_area_bytes(const PlanImpl &P)
+{
+    return kStatusBytes + align256((int64_t)sizeof(EndpointPartial) * std::max<int64_t>(P.partials, 1));
+}
+
+// Builds the steps of one distributed-plan operation.
+static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
+{
+    std::vector<StageSpec> specs;
+    pfh_status st = route_specs(P, op, specs);
+    if (st) return st;
+    const int tb = (P.precision == PFH_F32) ? 4 : 8;
+    const int K = (int)specs.size();
+    int64_t dst_cap = specs[K - 1].out_natural.bytes();
+    if (in_place) dst_cap = std::max(dst_cap, specs[0].in_natural.bytes());
+    std::vector<Layout> X, Y;
+    std::vector<int> ib, ob;
+    int64_t wb[2];
+    if (!assign_route(specs, in_place, dst_cap, 0, X, Y, ib, ob, wb))
+        return set_err(PFH_ERR_PREPARATION_FAILED, "no buffer assignment");
+    out.steps.clear();
+    for (int e = 0; e < K; ++e) {
+        Step s;
+        st = fill_transform(s, specs[e], P.host_tables, tb);
+        if (st) return st;
+        s.in = X[e];
+        s.out = Y[e];
+        s.in_buf = ib[e];
+        s.out_buf = ob[e];
+        st = finish_step(s, P.dev, tb);
+        if (st) return st;
+        if (P.kind == PFH_R2C && specs[e].kind == SK_C2R && (op == PFH_INVERSE || op == PFH_BACKWARD)) {
+            // acceptance check on the C2R stage's input (distributed/r2c.rs:4709-4807)
+            const pfh_desc &d = P.desc;
+            unsigned sel = d.selected ? d.selected : ((1u << P.N) - 1u);
+            const int ra = specs[e].axis;
+            const int64_t real_len = (int64_t)d.shape[ra];
+            double depth = 1.0, transverse = 1.0;
+            for (int a = 0; a < P.N; ++a) {
+                if (a == ra || !(sel & (1u << a))) continue;
+                if (d.shape[a] > 1) {
+                    int64_t v = (int64_t)d.shape[a] - 1;
+                    int bits = 0;
+                    while (v) {
+                        ++bits;
+                        v >>= 1;
+                    }
+                    depth += bits;
+                }
+                transverse *= (double)d.shape[a];
+            }
+            const double eps = (tb == 4) ? (double)FLT_EPSILON : DBL_EPSILON;
+            const double minsub = (tb == 4) ? (double)1.40129846e-45f : 4.9406564584124654e-324;
+            EndpointArgs ep{};
+            ep.extra = P.extra;
+            ep.len = X[e].len();
+            const auto strd = X[e].strides();
+            ep.stride = strd[ra];
+            ep.clen = X[e].shape[ra];
+            ep.outer = ep.len / (ep.clen * ep.stride);
+            ep.planes = (real_len % 2 == 0) ? 2 : 1;
+            {
+                const int64_t count = ep.outer * ep.stride;
+                int64_t nb = (count + 4095) / 4096;
+                if (nb < 1) nb = 1;
+                if (nb > 256) nb = 256;
+                ep.nblk = (int)nb;
+                P.partials = std::max<int64_t>(P.partials, ep.extra * ep.planes * nb);
+            }
+            ep.relative = 128.0 * eps * depth;
+            ep.absolute = 128.0 * minsub * depth * ((op == PFH_BACKWARD) ? transverse : 1.0);
+            s.endpoint_check = true;
+            s.ep = ep;
+        }
+        out.steps.push_back(s);
+    }
+    out.in = specs[0].in_natural;
+    out.out = specs[K - 1].out_natural;
+    out.w_bytes[0] = wb[0];
+    out.w_bytes[1] = wb[1];
+    out.valid = true;
+    return PFH_OK;
+}
+
+// Local line plans: one stage on lines of a 1-D array with `batch` extras.
+static pfh_status build_local_op(PlanImpl &P, int op, OpPlan &out)
+{
+    const int tb = (P.precision == PFH_F32) ? 4 : 8;
+    StageSpec sp;
+    sp.axis = 0;
+    const int64_t n = (int64_t)P.line_len;
+    sp.n = n;
+    const bool fwd = (op == PFH_FORWARD || op == PFH_ADJOINT_INVERSE || op == PFH_ADJOINT_BACKWARD);
+    std::array<int64_t, 8> shin{}, shout{};
+    shin[0] = shout[0] = n;
+    switch (P.kind) {
+    case PFH_LOCAL_C2C:
+        sp.kind = SK_C2C;
+        sp.sign_pos = !fwd;
+        if (op == PFH_INVERSE || op == PFH_ADJOINT_INVERSE) sp.scale = 1.0 / (double)n;
+        sp.in_c = sp.out_c = true;
+        break;
+    case PFH_LOCAL_R2C:
+        if (fwd) {
+            sp.kind = SK_R2C;
+            sp.in_c = false;
+            sp.out_c = true;
+            shout[0] = n / 2 + 1;
+            if (op != PFH_FORWARD) sp.flags |= F_HERM_DOUBLE;
+        } else {
+            sp.kind = SK_C2R;
+            sp.in_c = true;
+            sp.out_c = false;
+            shin[0] = n / 2 + 1;
+            if (op == PFH_ADJOINT_FORWARD) sp.flags |= F_HERM_HALF;
+        }
+        if (op == PFH_INVERSE || op == PFH_ADJOINT_INVERSE) sp.scale = 1.0 / (double)n;
+        break;
+    case PFH_LOCAL_R2R:
+        r2r_stage(sp, P.r2r_kind, n, op);
+        break;
+    case PFH_LOCAL_DHT:
+        r2r_stage(sp, PFH_DHT, n, op);
+        break;
+    }
+    std::array<int, 8> id{};
+    Step s;
+    pfh_status st = fill_transform(s, sp, P.host_tables, tb);
+    if (st) return st;
+    s.in = make_layout(1, id, shin, 1, (sp.in_c ? 2 : 1) * tb);
+    s.out = make_layout(1, id, shout, 1, (sp.out_c ? 2 : 1) * tb);
+    s.in_buf = BUF_SRC;
+    s.out_buf = BUF_DST;
+    st = finish_step(s, P.dev, tb);
+    if (st) return st;
+    if (P.kind == PFH_LOCAL_R2C && !fwd && op != PFH_ADJOINT_FORWARD) s.local_endpoint_check = true;
+    out.steps.assign(1, s);
+    out.in = s.in;
+    out.out = s.out;
+    out.valid = true;
+    return PFH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plan creation
+// ---------------------------------------------------------------------------
+
+static bool valid_perm(int N, const int *p)
+{
+    bool seen[8] = {false};
+    for (int i = 0; i < N; ++i) {
+        if (p[i] < 0 || p[i] >= N || seen[p[i]]) return false;
+        seen[p[i]] = true;
+    }
+    return true;
+}
+
+static pfh_status create_impl(const pfh_desc *d, PlanImpl &P)
+{
+    P.desc = *d;
+    P.kind = d->kind;
+    P.precision = d->precision;
+    if (P.precision != PFH_F32 && P.precision != PFH_F64) return set_err(PFH_ERR_INVALID_ARGUMENT, "precision");
+    pfh_status st = device_info(P.dev);
+    if (st) return st;
+    P.extra = 1;
+    if (d->extra_ndim < 0 || d->extra_ndim > PFH_MAX_DIMS) return set_err(PFH_ERR_INVALID_ARGUMENT, "extra_ndim");
+    for (int i = 0; i < d->extra_ndim; ++i) P.extra *= (int64_t)d->extra_shape[i];
+    switch (P.kind) {
+    case PFH_LOCAL_C2C:
+    case PFH_LOCAL_R2C:
+    case PFH_LOCAL_R2R:
+    case PFH_LOCAL_DHT: {
+        P.N = 1;
+        P.line_len = d->line_len;
+        P.r2r_kind = d->r2r_kind;
+        if (d->line_len == 0) return set_err(PFH_ERR_INVALID_LENGTH, "line length must be non-zero");
+        if (P.kind == PFH_LOCAL_R2R) {
+            if (d->r2r_kind < PFH_DCT1 || d->r2r_kind > PFH_DST4) return set_err(PFH_ERR_INVALID_ARGUMENT, "r2r kind");
+            if (d->r2r_kind == PFH_DCT1 && d->line_len == 1)
+                return set_err(PFH_ERR_INVALID_LENGTH, "DCT-I needs length >= 2");
+        }
+        if (d->line_len > (size_t)1 << 30) return set_err(PFH_ERR_LENGTH_OVERFLOW, "line length");
+        for (int op = 0; op < 6; ++op) {
+            st = build_local_op(P, op, P.ops[op]);
+            if (st) return st;
+        }
+        break;
+    }
+    case PFH_C2C:
+    case PFH_R2C:
+    case PFH_R2R: {
+        const int N = d->ndim;
+        const int M = d->topology_dims ? d->topology_dims : N - 1;
+        if (N < 2 || N > PFH_MAX_DIMS || M < 1 || M >= N) return set_err(PFH_ERR_INVALID_DIMENSIONS, "N >= 2, 1 <= M < N");
+        P.N = N;
+        for (int a = 0; a < N; ++a)
+            if (d->shape[a] == 0) return set_err(PFH_ERR_ZERO_EXTENT, "zero global extent", 0, 0, (size_t)a);
+        if (P.kind == PFH_R2C) {
+            unsigned sel = d->selected ? d->selected : ((1u << N) - 1u);
+            if ((sel & ((1u << N) - 1u)) == 0) return set_err(PFH_ERR_INVALID_DIMENSIONS, "empty selection");
+        }
+        if (P.kind == PFH_R2R) {
+            for (int a = 0; a < N; ++a) {
+                const int k = d->axis_kind[a];
+                if (k < 0 || k > PFH_DHT) return set_err(PFH_ERR_INVALID_ARGUMENT, "axis kind");
+                if (k == PFH_DCT1 && d->shape[a] == 1) return set_err(PFH_ERR_INVALID_LENGTH, "DCT-I needs length >= 2");
+            }
+        }
+        for (int op = 0; op < 6; ++op) {
+            st = build_op(P, op, false, P.ops[op]);
+            if (st) return st;
+            st = build_op(P, op, true, P.ops_inplace[op]);
+            if (st) return st;
+        }
+        break;
+    }
+    case PFH_LOCAL_TRANSPOSE:
+    case PFH_ALLTOALLV_TRANSPOSE: {
+        const int N = d->ndim;
+        if (N < 1 || N > PFH_MAX_DIMS) return set_err(PFH_ERR_INVALID_DIMENSIONS, "ndim");
+        if (!valid_perm(N, d->src_perm) || !valid_perm(N, d->dst_perm))
+            return set_err(PFH_ERR_INVALID_PERMUTATION, "invalid axis permutation");
+        for (int a = 0; a < N; ++a)
+            if (d->shape[a] == 0) return set_err(PFH_ERR_ZERO_EXTENT, "zero global extent", 0, 0, (size_t)a);
+        if (d->element_bytes == 0) return set_err(PFH_ERR_INVALID_ARGUMENT, "element_bytes");
+        P.N = N;
+        const size_t eb = d->element_bytes;
+        int w = 1;
+        for (int c : {16, 8, 4, 2}) {
+            if (eb % (size_t)c == 0) {
+                w = c;
+                break;
+            }
+        }
+        P.word = w;
+        P.words_per_elem = (int)(eb / (size_t)w);
+        std::array<int, 8> sp{}, dp{};
+        std::array<int64_t, 8> sh{};
+        for (int a = 0; a < N; ++a) {
+            sp[a] = d->src_perm[a];
+            dp[a] = d->dst_perm[a];
+            sh[a] = (int64_t)d->shape[a];
+        }
+        P.tr_src = make_layout(N, sp, sh, P.extra, (int)eb);
+        P.tr_dst = make_layout(N, dp, sh, P.extra, (int)eb);
+        // forward (src -> dst) and backward (dst -> src) as copy steps
+        const bool tiled = (P.words_per_elem == 1 && (w == 4 || w == 8 || w == 16));
+        for (int dir = 0; dir < 2; ++dir) {
+            OpPlan &o = P.ops[dir == 0 ? PFH_FORWARD : PFH_BACKWARD];
+            const Layout &a = dir == 0 ? P.tr_src : P.tr_dst;
+            const Layout &b = dir == 0 ? P.tr_dst : P.tr_src;
+            o.in = a;
+            o.out = b;
+            o.valid = true;
+            o.steps.clear();
+            if (tiled) {
+                // word type: 4 -> f32 real, 8 -> f32 complex, 16 -> f64 complex
+                Step s;
+                StageSpec spc;
+                spc.kind = SK_COPY;
+                spc.axis = b.perm[N - 1];
+                spc.n = b.shape[spc.axis];
+                spc.in_c = spc.out_c = (w != 4);
+                fill_transform(s, spc, P.host_tables, (w == 16) ? 8 : 4);
+                s.in = a;
+                s.out = b;
+                const int tb = (w == 16) ? 8 : 4;
+                s.in.elem = s.out.elem = w;
+                s.in_buf = BUF_SRC;
+                s.out_buf = BUF_DST;
+                st = finish_step(s, P.dev, tb);
+                if (st) return st;
+                o.steps.push_back(s);
+            }
+        }
+        P.precision = (w == 16) ? PFH_F64 : PFH_F32;
+        break;
+    }
+    default:
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "unknown plan kind");
+    }
+    if (P.precision == PFH_F32) st = upload_tables<float>(P);
+    else st = upload_tables<double>(P);
+    return st;
+}
+
+// ---------------------------------------------------------------------------
+// execution
+// ---------------------------------------------------------------------------
+
+struct WsView {
+    int *status = nullptr;
+    int *gate = nullptr;
+    unsigned long long *first_code = nullptr;
+    EndpointPartial *ep = nullptr;
+    void *w[2] = {nullptr, nullptr};
+};
+
+static pfh_status view_workspace(const PlanImpl &P, const OpPlan &o, void *ws, size_t ws_bytes, WsView &v)
+{
+    const int64_t need = status_area_bytes(P) + align256(o.w_bytes[0]) + align256(o.w_bytes[1]);
+    if (ws == nullptr || (int64_t)ws_bytes < need)
+        return set_err(PFH_ERR_WORKSPACE_TOO_SMALL, "workspace too small", (size_t)need, ws ? ws_bytes : 0);
+    char *b = static_cast<char *>(ws);
+    v.status = reinterpret_cast<int *>(b);
+    v.gate = reinterpret_cast<int *>(b + 4);
+    v.first_code = reinterpret_cast<unsigned long long *>(b + 8);
+    v.ep = reinterpret_cast<EndpointPartial *>(b + kStatusBytes);
+    char *w = b + status_area_bytes(P);
+    v.w[0] = w;
+    v.w[1] = w + align256(o.w_bytes[0]);
+    return PFH_OK;
+}
+
+template <typename T>
+static pfh_status launch_step(const PlanImpl &P, const Step &s, void *const bufs[4], const WsView &wv,
+                              hipStream_t stream)
+{
+    if (s.endpoint_check) {
+        PFH_HIP_CHECK(launch_endpoint_check<T>(s.ep, bufs[s.in_buf], wv.ep, wv.gate, wv.status, stream));
+    }
+    if (s.local_endpoint_check) {
+        PFH_HIP_CHECK(launch_local_endpoint_check<T>(bufs[s.in_buf], s.nlines, s.nin, (s.n % 2 == 0) ? 1 : 0,
+                                                     wv.first_code, wv.gate, wv.status, stream));
+    }
+    StageArgs<T> a{};
+    a.src = bufs[s.in_buf];
+    a.dst = bufs[s.out_buf];
+    a.tw = reinterpret_cast<const cpx<T> *>(P.dev_tables) + s.tw_base;
+    a.gate = (s.endpoint_check || s.local_endpoint_check || s.gated) ? wv.gate : nullptr;
+    a.nlines = s.nlines;
+    a.ntiles = s.tiles;
+    a.g = s.g;
+    a.ips = s.ips;
+    a.ops = s.ops;
+    a.mode = s.mode;
+    a.flags = s.flags;
+    a.n = s.n;
+    a.nin = s.nin;
+    a.nout = s.nout;
+    a.m = s.m;
+    a.npass = (int)s.radices.size();
+    for (int p = 0; p < a.npass; ++p) {
+        a.radix[p] = s.radices[p];
+        a.twoff[p] = s.twoff[p];
+    }
+    a.pre0 = s.pre0;
+    a.pre1 = s.pre1;
+    a.post0 = s.post0;
+    a.post1 = s.post1;
+    a.twn = s.twn;
+    a.lt = s.lt;
+    a.tpl = s.tpl;
+    a.ls = s.ls;
+    a.in_col = s.in_col;
+    a.out_col = s.out_col;
+    a.xcd_per = s.xcd_per;
+    a.scale = (T)s.scale;
+    a.prew0 = (T)s.prew0;
+    a.prew1 = (T)s.prew1;
+    a.postw0 = (T)s.postw0;
+    a.postw1 = (T)s.postw1;
+    bool wide = false;
+    for (int R : s.radices) wide = wide || is_wide_radix(R);
+    PFH_HIP_CHECK(launch_line_stage<T>(a, wide, s.threads, s.lds_bytes, s.grid, stream));
+    return PFH_OK;
+}
+
+// Timing entries of one execution: an event after every endpoint check and
+// every stage kernel.
+struct TimingTrace {
+    std::vector<hipEvent_t> ev; // ev[0] recorded before the first launch
+    std::vector<int> axis;
+    std::vector<double> bytes;
+};
+
+static pfh_status run_op(const PlanImpl &P, const OpPlan &o, void *const bufs[4], const WsView &wv,
+                         hipStream_t stream, TimingTrace *tr)
+{
+    for (size_t i = 0; i < o.steps.size(); ++i) {
+        const Step &s = o.steps[i];
+        if (tr && (s.endpoint_check || s.local_endpoint_check)) {
+            // time the check on its own, then the stage with its gate
+            Step chk = s;
+            chk.gated = true;
+            hipEvent_t e;
+            PFH_HIP_CHECK(hipEventCreate(&e));
+            if (P.precision == PFH_F32) {
+                if (s.endpoint_check)
+                    PFH_HIP_CHECK(launch_endpoint_check<float>(s.ep, bufs[s.in_buf], wv.ep, wv.gate, wv.status, stream));
+            } else {
+                if (s.endpoint_check)
+                    PFH_HIP_CHECK(launch_endpoint_check<double>(s.ep, bufs[s.in_buf], wv.ep, wv.gate, wv.status, stream));
+            }
+            PFH_HIP_CHECK(hipEventRecord(e, stream));
+            tr->ev.push_back(e);
+            tr->axis.push_back(-1);
+            tr->bytes.push_back((double)(s.ep.extra * s.ep.planes * s.ep.outer * s.ep.stride) * (double)s.in.elem);
+            chk.endpoint_check = false;
+            pfh_status st = (P.precision == PFH_F32) ? launch_step<float>(P, chk, bufs, wv, stream)
+                                                     : launch_step<double>(P, chk, bufs, wv, stream);
+            if (st) return st;
+        } else {
+            pfh_status st = (P.precision == PFH_F32) ? launch_step<float>(P, s, bufs, wv, stream)
+                                                     : launch_step<double>(P, s, bufs, wv, stream);
+            if (st) return st;
+        }
+        if (tr) {
+            hipEvent_t e;
+            PFH_HIP_CHECK(hipEventCreate(&e));
+            PFH_HIP_CHECK(hipEventRecord(e, stream));
+            tr->ev.push_back(e);
+            tr->axis.push_back(s.axis);
+            tr->bytes.push_back((double)s.nlines * ((double)s.nin * s.in.elem + (double)s.nout * s.out.elem));
+        }
+    }
+    return PFH_OK;
+}
+
+static bool op_has_device_check(const OpPlan &o)
+{
+    for (const Step &s : o.steps)
+        if (s.endpoint_check || s.local_endpoint_check) return true;
+    return false;
+}
+
+// Reads (and clears) the sticky status after a synchronous execution.
+static pfh_status collect_status(const PlanImpl &P, const WsView &wv, hipStream_t stream)
+{
+    PFH_HIP_CHECK(hipStreamSynchronize(stream));
+    int status = 0;
+    unsigned long long code = ~0ull;
+    PFH_HIP_CHECK(hipMemcpy(&status, wv.status, sizeof(int), hipMemcpyDeviceToHost));
+    if (status & PFH_DEVICE_INVALID_ENDPOINT)
+        PFH_HIP_CHECK(hipMemcpy(&code, wv.first_code, sizeof(code), hipMemcpyDeviceToHost));
+    if (status) {
+        PFH_HIP_CHECK(launch_fill_i32(wv.status, 1, 0, stream));
+        PFH_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    if (status & PFH_DEVICE_INVALID_ENDPOINT) {
+        const size_t batch = (size_t)(code / 2);
+        const size_t index = (code & 1) ? (size_t)(P.line_len / 2) : 0;
+        return set_err(PFH_ERR_INVALID_SPECTRUM_ENDPOINT, "non-zero imaginary endpoint", 0, 0, batch, index);
+    }
+    if (status & PFH_DEVICE_INVALID_SPECTRUM)
+        return set_err(PFH_ERR_INVALID_SPECTRUM, "inverse spectrum has an invalid constrained boundary plane");
+    return ok();
+}
+
+static pfh_status execute_impl(PlanImpl &P, int op, const void *src, size_t src_elems, void *dst, size_t dst_elems,
+                               void *ws, size_t ws_bytes, hipStream_t stream, int flags, pfh_transform_timing *timing)
+{
+    if (op < 0 || op > 5) return set_err(PFH_ERR_INVALID_ARGUMENT, "op");
+    const bool is_tr = (P.kind == PFH_LOCAL_TRANSPOSE || P.kind == PFH_ALLTOALLV_TRANSPOSE);
+    if (is_tr && op != PFH_FORWARD && op != PFH_BACKWARD) return set_err(PFH_ERR_INVALID_ARGUMENT, "transpose op");
+    const OpPlan &o = P.ops[op];
+    if (!o.valid) return set_err(PFH_ERR_INVALID_ARGUMENT, "op not supported by this plan");
+    if (!src || !dst) return set_err(PFH_ERR_INVALID_ARGUMENT, "null buffer");
+    if ((int64_t)src_elems != o.in.total())
+        return set_err(is_tr ? PFH_ERR_SOURCE_LAYOUT_MISMATCH : PFH_ERR_INPUT_LAYOUT_MISMATCH, "source length",
+                       (size_t)o.in.total(), src_elems);
+    if ((int64_t)dst_elems != o.out.total())
+        return set_err(is_tr ? PFH_ERR_DESTINATION_LAYOUT_MISMATCH : PFH_ERR_OUTPUT_LAYOUT_MISMATCH, "destination length",
+                       (size_t)o.out.total(), dst_elems);
+    if (is_tr && o.steps.empty()) {
+        // generic word permutation
+        PermuteArgs pa{};
+        const Layout &a = o.in, &b = o.out;
+        const int N = a.N;
+        const auto sst = a.strides();
+        pa.nd = N + 1;
+        pa.len = a.len() * P.words_per_elem;
+        pa.extra = a.extra;
+        for (int p = 0; p < N; ++p) {
+            pa.dext[p] = b.shape[b.perm[p]];
+            pa.sstride[p] = sst[b.perm[p]] * P.words_per_elem;
+        }
+        pa.dext[N] = P.words_per_elem;
+        pa.sstride[N] = 1;
+        if (P.word == 1 || P.word == 2) {
+            PFH_HIP_CHECK(launch_permute_words(pa, src, dst, P.word, stream));
+        } else {
+            // wider words: permute as 2-byte words with the word split into a trailing axis
+            const int k = P.word / 2;
+            pa.len *= k;
+            for (int p = 0; p <= N; ++p) pa.sstride[p] *= k;
+            pa.nd = N + 2;
+            pa.dext[N + 1] = k;
+            pa.sstride[N + 1] = 1;
+            PFH_HIP_CHECK(launch_permute_words(pa, src, dst, 2, stream));
+        }
+        if (timing) PFH_HIP_CHECK(hipStreamSynchronize(stream));
+        return ok();
+    }
+    WsView wv;
+    const bool needs_ws = !is_tr;
+    if (needs_ws) {
+        pfh_status st = view_workspace(P, o, ws, ws_bytes, wv);
+        if (st) return st;
+    }
+    void *bufs[4] = {const_cast<void *>(src), dst, wv.w[0], wv.w[1]};
+    TimingTrace tr;
+    if (timing) {
+        hipEvent_t e0;
+        PFH_HIP_CHECK(hipEventCreate(&e0));
+        PFH_HIP_CHECK(hipEventRecord(e0, stream));
+        tr.ev.push_back(e0);
+    }
+    pfh_status st = run_op(P, o, bufs, wv, stream, timing ? &tr : nullptr);
+    if (st) return st;
+    if (timing) {
+        PFH_HIP_CHECK(hipEventSynchronize(tr.ev.back()));
+        const int ne = (int)tr.axis.size();
+        timing->stages = ne < 12 ? ne : 12;
+        timing->total_ms = 0;
+        for (int i = 0; i < timing->stages; ++i) {
+            float ms = 0;
+            PFH_HIP_CHECK(hipEventElapsedTime(&ms, tr.ev[i], tr.ev[i + 1]));
+            timing->stage_ms[i] = ms;
+            timing->stage_axis[i] = tr.axis[i];
+            timing->stage_bytes[i] = tr.bytes[i];
+            timing->total_ms += ms;
+        }
+        for (auto &e : tr.ev) (void)hipEventDestroy(e);
+    }
+    if (needs_ws && op_has_device_check(o) && !(flags & PFH_EXEC_DEFER_STATUS)) return collect_status(P, wv, stream);
+    return ok();
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char *pfh_version(void) { return "pencil-fft-hip 0.1.0 (gfx950)"; }
+
+void pfh_last_error(pfh_error_info *out)
+{
+    if (out) *out = g_err;
+}
+
+const char *pfh_status_string(pfh_status s)
+{
+    switch (s) {
+    case PFH_OK: return "ok";
+    case PFH_ERR_INVALID_LENGTH: return "FFT line length must be non-zero";
+    case PFH_ERR_LENGTH_OVERFLOW: return "FFT length or derived length cannot be represented";
+    case PFH_ERR_NON_INTEGRAL_BATCH: return "buffer length is not an integral number of FFT lines";
+    case PFH_ERR_BUFFER_LENGTH_MISMATCH: return "source and destination lengths differ";
+    case PFH_ERR_BATCH_COUNT_MISMATCH: return "source and destination batch counts differ";
+    case PFH_ERR_SCRATCH_TOO_SMALL: return "scratch is too small";
+    case PFH_ERR_INVALID_SPECTRUM_ENDPOINT: return "inverse spectrum has a non-zero imaginary endpoint";
+    case PFH_ERR_INVALID_DIMENSIONS: return "distributed FFT requires N >= 2 and 1 <= M < N";
+    case PFH_ERR_INVALID_INPUT_LAYOUT: return "input pencil must use identity permutation";
+    case PFH_ERR_INPUT_LAYOUT_MISMATCH: return "source layout does not match the plan";
+    case PFH_ERR_OUTPUT_LAYOUT_MISMATCH: return "destination layout does not match the plan";
+    case PFH_ERR_EXTRA_SHAPE_MISMATCH: return "extra shape does not match the plan";
+    case PFH_ERR_WORKSPACE_MISMATCH: return "workspace does not belong to this plan";
+    case PFH_ERR_WORKSPACE_TOO_SMALL: return "workspace is too small";
+    case PFH_ERR_PREPARATION_FAILED: return "preparation failed";
+    case PFH_ERR_ALLOCATION_FAILED: return "allocation failed";
+    case PFH_ERR_INVALID_SPECTRUM: return "inverse spectrum has an invalid constrained boundary plane";
+    case PFH_ERR_INCOMPATIBLE_DISTRIBUTION: return "source and destination distributions are incompatible";
+    case PFH_ERR_SOURCE_LAYOUT_MISMATCH: return "source view does not match the plan";
+    case PFH_ERR_DESTINATION_LAYOUT_MISMATCH: return "destination view does not match the plan";
+    case PFH_ERR_INVALID_PERMUTATION: return "invalid axis permutation";
+    case PFH_ERR_ZERO_EXTENT: return "zero global extent";
+    case PFH_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case PFH_ERR_UNSUPPORTED: return "length not supported by the device kernels";
+    case PFH_ERR_DEVICE: return "HIP runtime error";
+    }
+    return "unknown status";
+}
+
+pfh_status pfh_plan_create(const pfh_desc *desc, pfh_plan *out)
+{
+    if (!desc || !out) return set_err(PFH_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    pfh_plan p = new (std::nothrow) pfh_plan_s;
+    if (!p) return set_err(PFH_ERR_ALLOCATION_FAILED, "plan");
+    pfh_status st = create_impl(desc, p->impl);
+    if (st) {
+        delete p;
+        return st;
+    }
+    *out = p;
+    return ok();
+}
+
+void pfh_plan_destroy(pfh_plan plan) { delete plan; }
+
+pfh_status pfh_plan_workspace_bytes(pfh_plan plan, int op, size_t *bytes)
+{
+    if (!plan || !bytes || op < 0 || op > 5) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    const PlanImpl &P = plan->impl;
+    if (P.kind == PFH_LOCAL_TRANSPOSE || P.kind == PFH_ALLTOALLV_TRANSPOSE) {
+        *bytes = (size_t)align256(P.tr_src.bytes());
+        return ok();
+    }
+    const OpPlan &o = P.ops[op];
+    const OpPlan &oi = P.ops_inplace[op];
+    int64_t b = status_area_bytes(P) + align256(o.w_bytes[0]) + align256(o.w_bytes[1]);
+    if (oi.valid) b = std::max(b, status_area_bytes(P) + align256(oi.w_bytes[0]) + align256(oi.w_bytes[1]));
+    if (P.kind == PFH_LOCAL_R2C || P.kind == PFH_LOCAL_C2C || P.kind == PFH_LOCAL_R2R || P.kind == PFH_LOCAL_DHT) {
+        // in-place local execution stages the input here
+        (void)0;
+    }
+    *bytes = (size_t)b;
+    return ok();
+}
+
+pfh_status pfh_plan_layout(pfh_plan plan, int side, int perm[PFH_MAX_DIMS], size_t shape[PFH_MAX_DIMS],
+                           size_t *elements, size_t *element_bytes)
+{
+    if (!plan || (side != 0 && side != 1)) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    const PlanImpl &P = plan->impl;
+    const OpPlan &o = P.ops[PFH_FORWARD];
+    const Layout &L = side == 0 ? o.in : o.out;
+    for (int a = 0; a < PFH_MAX_DIMS; ++a) {
+        if (perm) perm[a] = a < L.N ? L.perm[a] : -1;
+        if (shape) shape[a] = a < L.N ? (size_t)L.shape[a] : 0;
+    }
+    if (elements) *elements = (size_t)L.total();
+    if (element_bytes) *element_bytes = (size_t)L.elem;
+    return ok();
+}
+
+pfh_status pfh_execute(pfh_plan plan, int op, const void *src, size_t src_elems, void *dst, size_t dst_elems,
+                       void *workspace, size_t workspace_bytes, pfh_stream_t stream, int flags)
+{
+    if (!plan) return set_err(PFH_ERR_INVALID_ARGUMENT, "null plan");
+    PlanImpl &P = plan->impl;
+    if (P.kind <= PFH_LOCAL_DHT)
+        return pfh_local_execute(plan, op, src, src_elems, dst, dst_elems, workspace, workspace_bytes, stream, flags);
+    return execute_impl(P, op, src, src_elems, dst, dst_elems, workspace, workspace_bytes,
+                        reinterpret_cast<hipStream_t>(stream), flags, nullptr);
+}
+
+pfh_status pfh_execute_with_timing(pfh_plan plan, int op, const void *src, size_t src_elems, void *dst,
+                                   size_t dst_elems, void *workspace, size_t workspace_bytes, pfh_stream_t stream,
+                                   pfh_transform_timing *timing)
+{
+    if (!plan || !timing) return set_err(PFH_ERR_INVALID_ARGUMENT, "null argument");
+    PlanImpl &P = plan->impl;
+    if (P.kind <= PFH_LOCAL_DHT) return set_err(PFH_ERR_INVALID_ARGUMENT, "timing is for distributed plans");
+    return execute_impl(P, op, src, src_elems, dst, dst_elems, workspace, workspace_bytes,
+                        reinterpret_cast<hipStream_t>(stream), 0, timing);
+}
+
+pfh_status pfh_execute_in_place(pfh_plan plan, int op, void *data, size_t capacity_bytes, void *workspace,
+                                size_t workspace_bytes, pfh_stream_t stream_, int flags)
+{
+    if (!plan || !data) return set_err(PFH_ERR_INVALID_ARGUMENT, "null argument");
+    if (op < 0 || op > 5) return set_err(PFH_ERR_INVALID_ARGUMENT, "op");
+    PlanImpl &P = plan->impl;
+    hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+    if (P.kind == PFH_LOCAL_TRANSPOSE || P.kind == PFH_ALLTOALLV_TRANSPOSE) {
+        // stage the source into the workspace, then permute back (local_transpose.rs:159-243)
+        const OpPlan &o = P.ops[op == PFH_BACKWARD ? PFH_BACKWARD : PFH_FORWARD];
+        const size_t need = (size_t)o.in.bytes();
+        if ((int64_t)capacity_bytes < o.in.bytes())
+            return set_err(PFH_ERR_SOURCE_LAYOUT_MISMATCH, "data too small", need, capacity_bytes);
+        if (!workspace || workspace_bytes < need)
+            return set_err(PFH_ERR_SCRATCH_TOO_SMALL, "scratch too small", need, workspace ? workspace_bytes : 0);
+        PFH_HIP_CHECK(hipMemcpyAsync(workspace, data, need, hipMemcpyDeviceToDevice, stream));
+        return execute_impl(P, op == PFH_BACKWARD ? PFH_BACKWARD : PFH_FORWARD, workspace, (size_t)o.in.total(), data,
+                            (size_t)o.out.total(), nullptr, 0, stream, flags, nullptr);
+    }
+    if (P.kind <= PFH_LOCAL_DHT) return set_err(PFH_ERR_INVALID_ARGUMENT, "local plans: use pfh_local_execute_in_place");
+    const OpPlan &o = P.ops_inplace[op];
+    if (!o.valid) return set_err(PFH_ERR_INVALID_ARGUMENT, "op");
+    const int64_t need = std::max(o.in.bytes(), o.out.bytes());
+    if ((int64_t)capacity_bytes < need)
+        return set_err(PFH_ERR_WORKSPACE_TOO_SMALL, "in-place storage", (size_t)need, capacity_bytes);
+    WsView wv;
+    pfh_status st = view_workspace(P, o, workspace, workspace_bytes, wv);
+    if (st) return st;
+    void *bufs[4] = {data, data, wv.w[0], wv.w[1]};
+    st = run_op(P, o, bufs, wv, stream, nullptr);
+    if (st) return st;
+    if (op_has_device_check(o) && !(flags & PFH_EXEC_DEFER_STATUS)) return collect_status(P, wv, stream);
+    return ok();
+}
+
+pfh_status pfh_local_execute(pfh_plan plan, int op, const void *src, size_t src_elems, void *dst,
+                             size_t dst_elems, void *workspace, size_t workspace_bytes, pfh_stream_t stream_,
+                             int flags)
+{
+    if (!plan) return set_err(PFH_ERR_INVALID_ARGUMENT, "null plan");
+    if (op < 0 || op > 5) return set_err(PFH_ERR_INVALID_ARGUMENT, "op");
+    PlanImpl &P = plan->impl;
+    if (P.kind > PFH_LOCAL_DHT) return set_err(PFH_ERR_INVALID_ARGUMENT, "not a local plan");
+    hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+    const OpPlan &tmpl = P.ops[op];
+    const int64_t lin = tmpl.in.shape[0], lout = tmpl.out.shape[0];
+    int64_t batch;
+    if (P.kind == PFH_LOCAL_R2C) {
+        // validate_batches (r2c.rs:571-590): source, then destination, then equality
+        if ((int64_t)src_elems % lin) return set_err(PFH_ERR_NON_INTEGRAL_BATCH, "source");
+        if ((int64_t)dst_elems % lout) return set_err(PFH_ERR_NON_INTEGRAL_BATCH, "destination");
+        const int64_t sc = (int64_t)src_elems / lin, dc = (int64_t)dst_elems / lout;
+        if (sc != dc)
+            return set_err(PFH_ERR_BATCH_COUNT_MISMATCH, "batch counts differ", 0, 0, (size_t)sc, (size_t)dc);
+        batch = sc;
+    } else {
+        if (src_elems != dst_elems) return set_err(PFH_ERR_BUFFER_LENGTH_MISMATCH, "lengths differ");
+        if ((int64_t)src_elems % lin) return set_err(PFH_ERR_NON_INTEGRAL_BATCH, "batch");
+        batch = (int64_t)src_elems / lin;
+    }
+    const bool c2r_check = tmpl.steps[0].local_endpoint_check;
+    WsView wv;
+    if (c2r_check) {
+        if (!workspace || (int64_t)workspace_bytes < status_area_bytes(P))
+            return set_err(PFH_ERR_SCRATCH_TOO_SMALL, "status area", (size_t)status_area_bytes(P),
+                           workspace ? workspace_bytes : 0);
+        char *b = static_cast<char *>(workspace);
+        wv.status = reinterpret_cast<int *>(b);
+        wv.gate = reinterpret_cast<int *>(b + 4);
+        wv.first_code = reinterpret_cast<unsigned long long *>(b + 8);
+    }
+    if (batch == 0) return ok();
+    if (!src || !dst) return set_err(PFH_ERR_INVALID_ARGUMENT, "null buffer");
+    OpPlan o = tmpl;
+    Step &s = o.steps[0];
+    s.in.extra = s.out.extra = batch;
+    s.flags &= ~F_XCD_REMAP;
+    const int tb = (P.precision == PFH_F32) ? 4 : 8;
+    pfh_status st = finish_step(s, P.dev, tb);
+    if (st) return st;
+    void *bufs[4] = {const_cast<void *>(src), dst, nullptr, nullptr};
+    st = run_op(P, o, bufs, wv, stream, nullptr);
+    if (st) return st;
+    if (c2r_check && !(flags & PFH_EXEC_DEFER_STATUS)) return collect_status(P, wv, stream);
+    return ok();
+}
+
+pfh_status pfh_local_execute_in_place(pfh_plan plan, int op, void *data, size_t elems, size_t capacity_bytes,
+                                      void *workspace, size_t workspace_bytes, pfh_stream_t stream_, int flags)
+{
+    if (!plan) return set_err(PFH_ERR_INVALID_ARGUMENT, "null plan");
+    if (op < 0 || op > 5) return set_err(PFH_ERR_INVALID_ARGUMENT, "op");
+    PlanImpl &P = plan->impl;
+    if (P.kind > PFH_LOCAL_DHT) return set_err(PFH_ERR_INVALID_ARGUMENT, "not a local plan");
+    hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+    const OpPlan &tmpl = P.ops[op];
+    const int64_t lin = tmpl.in.shape[0], lout = tmpl.out.shape[0];
+    if ((int64_t)elems % lin) return set_err(PFH_ERR_NON_INTEGRAL_BATCH, "batch");
+    const int64_t batch = (int64_t)elems / lin;
+    const int64_t in_bytes = batch * lin * tmpl.in.elem, out_bytes = batch * lout * tmpl.out.elem;
+    const int64_t need = std::max(in_bytes, out_bytes);
+    if ((int64_t)capacity_bytes < need)
+        return set_err(PFH_ERR_ALLOCATION_FAILED, "in-place storage too small", (size_t)need, capacity_bytes);
+    if (P.kind != PFH_LOCAL_R2C) {
+        // same layout on both sides: every workgroup reads its lines into LDS before writing them
+        return pfh_local_execute(plan, op, data, elems, data, elems, workspace, workspace_bytes, stream_, flags);
+    }
+    // packed real / half-complex storage: stage the input through the workspace
+    const int64_t ws_need = status_area_bytes(P) + align256(in_bytes);
+    if (!workspace || (int64_t)workspace_bytes < ws_need)
+        return set_err(PFH_ERR_SCRATCH_TOO_SMALL, "workspace too small", (size_t)ws_need,
+                       workspace ? workspace_bytes : 0);
+    if (batch == 0) return ok();
+    char *stage = static_cast<char *>(workspace) + status_area_bytes(P);
+    PFH_HIP_CHECK(hipMemcpyAsync(stage, data, (size_t)in_bytes, hipMemcpyDeviceToDevice, stream));
+    return pfh_local_execute(plan, op, stage, (size_t)(batch * lin), data, (size_t)(batch * lout), workspace,
+                             workspace_bytes, stream_, flags);
+}
+
+pfh_status pfh_workspace_take_status(pfh_plan plan, void *workspace, pfh_stream_t stream)
+{
+    if (!plan || !workspace) return set_err(PFH_ERR_INVALID_ARGUMENT, "null argument");
+    WsView wv;
+    char *b = static_cast<char *>(workspace);
+    wv.status = reinterpret_cast<int *>(b);
+    wv.gate = reinterpret_cast<int *>(b + 4);
+    wv.first_code = reinterpret_cast<unsigned long long *>(b + 8);
+    return collect_status(plan->impl, wv, reinterpret_cast<hipStream_t>(stream));
+}
+
+} // extern "C"
+
+// Batched uneven transforms: see pfh_batch.cpp.

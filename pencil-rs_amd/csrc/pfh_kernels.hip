@@ -1,0 +1,842 @@
+// gfx950 kernels of the pencil-rs single-GPU FFT path.
+//
+// k_line_stage runs one route stage: it reads a tile of `lt` lines of one
+// memory axis (row access when the axis is contiguous, column access with the
+// tile's lines adjacent in memory otherwise), stages them in LDS, applies the
+// per-mode pre-processing, runs the Stockham radix passes of the complex FFT
+// of length m in LDS, and writes the post-processed line into the output
+// layout. Writing into a different layout than the one read is how the
+// reference's transitions (pack -> exchange -> unpack,
+// crates/pencil-array/src/transpose.rs:1107-1315, alltoallv_transpose.rs:
+// 283-308) and LocalTransposePlan (local_transpose.rs:87-132) are fused into
+// the stage, so every stage is exactly one HBM read and one HBM write of the
+// working array.
+//
+// The real transforms do not embed a line into a 2x..8x longer complex FFT
+// (crates/pencil-fft/src/r2r.rs:697-743); they run pre/post twiddle kernels
+// around a complex FFT of length n/2, n-1, n+1 or n.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "pfh_kernels.hpp"
+
+namespace pfh {
+
+namespace {
+
+__device__ __forceinline__ void line_offsets(const LineGeom &g, int64_t line, int64_t &io, int64_t &oo)
+{
+    io = 0;
+    oo = 0;
+    for (int d = 0; d < g.nd; ++d) {
+        const int64_t e = g.ext[d];
+        const int64_t q = line / e;
+        const int64_t r = line - q * e;
+        io += r * g.is[d];
+        oo += r * g.os[d];
+        line = q;
+    }
+}
+
+__device__ __forceinline__ bool complex_input(int mode)
+{
+    return mode == M_COPY_C || mode == M_C2C || mode == M_C2R || mode == M_C2R_ODD;
+}
+
+__device__ __forceinline__ bool complex_output(int mode)
+{
+    return mode == M_COPY_C || mode == M_C2C || mode == M_R2C || mode == M_R2C_ODD;
+}
+
+__device__ __forceinline__ bool needs_pre(int mode)
+{
+    return !(mode == M_COPY_C || mode == M_COPY_R || mode == M_C2C || mode == M_R2C || mode == M_DHT);
+}
+
+// One input value of a real line after the reversal / alternating sign /
+// adjoint edge weights requested by the stage.
+template <typename T>
+__device__ __forceinline__ T xin(const StageArgs<T> &a, const T *lr, int i)
+{
+    const int idx = (a.flags & F_PRE_REV) ? a.n - 1 - i : i;
+    T v = lr[idx];
+    if ((a.flags & F_PRE_ALT) && (idx & 1)) v = -v;
+    if (a.flags & F_WEIGHTS) {
+        if (idx == 0) v = v * a.prew0;
+        if (idx == a.n - 1) v = v * a.prew1;
+    }
+    return v;
+}
+
+// Half-spectrum input value X_k of a C2R line (Im of DC / Nyquist dropped,
+// adjoint halving of interior bins).
+template <typename T>
+__device__ __forceinline__ cpx<T> hin(const StageArgs<T> &a, const cpx<T> *ln, int k, int last)
+{
+    cpx<T> v = ln[k];
+    if (k == 0 || k == last) v.y = (T)0;
+    if ((a.flags & F_HERM_HALF) && k > 0 && k < last) v = scal(v, (T)0.5);
+    return v;
+}
+
+template <typename T>
+__device__ cpx<T> pre_value(const StageArgs<T> &a, const cpx<T> *ln, const cpx<T> *tws, int j)
+{
+    const T *lr = reinterpret_cast<const T *>(ln);
+    const int n = a.n;
+    switch (a.mode) {
+    case M_R2C_ODD:
+    case M_DHT_ODD:
+        return mk(xin(a, lr, j), (T)0);
+    case M_C2R: {
+        const int m = a.m; // n / 2; Nyquist index m
+        cpx<T> A = hin(a, ln, j, m);
+        cpx<T> B = conjg(hin(a, ln, m - j, m));
+        cpx<T> E = A + B, D = A - B;
+        cpx<T> O = cmul(tws[a.pre0 + j], D);
+        return conjg(E + mul_pi(O));
+    }
+    case M_C2R_ODD: {
+        const int h = (n - 1) / 2;
+        cpx<T> v = (j <= h) ? hin(a, ln, j, h + 1) : conjg(hin(a, ln, n - j, h + 1));
+        if (n == 1) v.y = (T)0;
+        return conjg(v);
+    }
+    case M_DCT2: {
+        const int h = n / 2;
+        const int i0 = 2 * j, i1 = 2 * j + 1;
+        T v0 = i0 < h ? xin(a, lr, 2 * i0) : xin(a, lr, 2 * (n - 1 - i0) + 1);
+        T v1 = i1 < h ? xin(a, lr, 2 * i1) : xin(a, lr, 2 * (n - 1 - i1) + 1);
+        return mk(v0, v1);
+    }
+    case M_DCT2_ODD: {
+        const int h = (n + 1) / 2;
+        T v = j < h ? xin(a, lr, 2 * j) : xin(a, lr, 2 * (n - 1 - j) + 1);
+        return mk(v, (T)0);
+    }
+    case M_DCT3: {
+        const int m = a.m;
+        // V_k = exp(i pi k / 2n) (y_k - i y_{n-k}), y_n = 0
+        const int ka = j, kb = m - j;
+        cpx<T> Va = cmul(tws[a.pre0 + ka], mk(xin(a, lr, ka), ka == 0 ? (T)0 : -xin(a, lr, n - ka)));
+        cpx<T> Vb = cmul(tws[a.pre0 + kb], mk(xin(a, lr, kb), kb == 0 ? (T)0 : -xin(a, lr, n - kb)));
+        if (ka == 0) Va.y = (T)0;
+        if (kb == m) Vb.y = (T)0;
+        cpx<T> B = conjg(Vb);
+        cpx<T> E = Va + B, D = Va - B;
+        cpx<T> O = cmul(tws[a.pre1 + j], D);
+        return conjg(E + mul_pi(O));
+    }
+    case M_DCT3_ODD: {
+        T y = xin(a, lr, j);
+        if (j == 0) y = y * (T)0.5;
+        return scal(tws[a.pre0 + j], y);
+    }
+    case M_DCT4:
+        return cmul(mk(xin(a, lr, 2 * j), xin(a, lr, n - 1 - 2 * j)), tws[a.pre0 + j]);
+    case M_DCT4_ODD:
+        return j < n ? scal(tws[a.pre0 + j], xin(a, lr, j)) : mk((T)0, (T)0);
+    case M_DCT1: {
+        const int L = 2 * (n - 1);
+        const int i0 = 2 * j, i1 = 2 * j + 1;
+        T v0 = i0 <= n - 1 ? xin(a, lr, i0) : xin(a, lr, L - i0);
+        T v1 = i1 <= n - 1 ? xin(a, lr, i1) : xin(a, lr, L - i1);
+        return mk(v0, v1);
+    }
+    case M_DST1: {
+        const int i0 = 2 * j, i1 = 2 * j + 1;
+        auto o = [&](int i) -> T {
+            if (i == 0 || i == n + 1) return (T)0;
+            return i <= n ? xin(a, lr, i - 1) : -xin(a, lr, 2 * n + 1 - i);
+        };
+        return mk(o(i0), o(i1));
+    }
+    default:
+        return ln[j];
+    }
+}
+
+// X_k of the real sequence whose half-length FFT is Z (k = 0..m).
+template <typename T>
+__device__ __forceinline__ cpx<T> split(const cpx<T> *Z, const cpx<T> *w, int m, int k)
+{
+    const int ka = (k == m) ? 0 : k;
+    const int kb = (k == 0) ? 0 : m - k;
+    cpx<T> A = Z[ka], B = conjg(Z[kb]);
+    cpx<T> E = scal(A + B, (T)0.5), D = scal(A - B, (T)0.5);
+    return E + mul_mi(cmul(w[k], D));
+}
+
+template <typename T>
+__device__ __forceinline__ T t_real_view(const cpx<T> *Z, int i, bool conj_result)
+{
+    cpx<T> v = Z[i >> 1];
+    if (conj_result) v.y = -v.y;
+    return (i & 1) ? v.y : v.x;
+}
+
+// Real output y_k of a real-output mode.
+template <typename T>
+__device__ T post_real(const StageArgs<T> &a, const cpx<T> *Z, const cpx<T> *tws, int k)
+{
+    const int n = a.n, m = a.m;
+    const int kk = (a.flags & F_POST_REV) ? n - 1 - k : k;
+    T y;
+    switch (a.mode) {
+    case M_COPY_R:
+        y = reinterpret_cast<const T *>(Z)[kk];
+        break;
+    case M_C2R:
+        y = t_real_view(Z, kk, true);
+        break;
+    case M_C2R_ODD:
+        y = Z[kk].x;
+        break;
+    case M_DCT2: {
+        cpx<T> V = kk <= m ? split(Z, tws + a.post0, m, kk) : conjg(split(Z, tws + a.post0, m, n - kk));
+        y = (T)2 * cmul(tws[a.post1 + kk], V).x;
+        break;
+    }
+    case M_DCT2_ODD:
+        y = (T)2 * cmul(tws[a.post0 + kk], Z[kk]).x;
+        break;
+    case M_DCT3: {
+        const int i = (kk & 1) ? n - 1 - (kk >> 1) : (kk >> 1);
+        y = t_real_view(Z, i, true);
+        break;
+    }
+    case M_DCT3_ODD: {
+        const int i = (kk & 1) ? n - 1 - (kk >> 1) : (kk >> 1);
+        y = (T)2 * Z[i].x;
+        break;
+    }
+    case M_DCT4: {
+        if ((kk & 1) == 0) {
+            const int q = kk >> 1;
+            y = (T)2 * cmul(Z[q], tws[a.post0 + q]).x;
+        } else {
+            const int q = (n - 1 - kk) >> 1;
+            y = (T)-2 * cmul(Z[q], tws[a.post0 + q]).y;
+        }
+        break;
+    }
+    case M_DCT4_ODD:
+        y = (T)2 * cmul(tws[a.post0 + kk], Z[kk]).x;
+        break;
+    case M_DCT1:
+        y = split(Z, tws + a.post0, m, kk).x;
+        break;
+    case M_DST1:
+        y = -split(Z, tws + a.post0, m, kk + 1).y;
+        break;
+    case M_DHT: {
+        if (kk <= m) {
+            cpx<T> X = split(Z, tws + a.post0, m, kk);
+            y = X.x - X.y;
+        } else {
+            cpx<T> X = split(Z, tws + a.post0, m, n - kk);
+            y = X.x + X.y;
+        }
+        break;
+    }
+    case M_DHT_ODD:
+        y = Z[kk].x - Z[kk].y;
+        break;
+    default:
+        y = (T)0;
+    }
+    if ((a.flags & F_POST_ALT) && (k & 1)) y = -y;
+    if (a.flags & F_WEIGHTS) {
+        if (k == 0) y = y * a.postw0;
+        if (k == n - 1) y = y * a.postw1;
+    }
+    return y * a.scale;
+}
+
+// Complex output value of a complex-output mode.
+template <typename T>
+__device__ cpx<T> post_complex(const StageArgs<T> &a, const cpx<T> *Z, const cpx<T> *tws, int k)
+{
+    cpx<T> v;
+    switch (a.mode) {
+    case M_R2C: {
+        v = split(Z, tws + a.post0, a.m, k);
+        if (k == 0 || k == a.m) v.y = (T)0; // real-input DC / Nyquist are exactly real
+        if (a.flags & F_HERM_DOUBLE) {
+            if (k > 0 && k < a.m) v = scal(v, (T)2);
+            else v.y = (T)0;
+        }
+        break;
+    }
+    case M_R2C_ODD: {
+        v = Z[k];
+        if (k == 0) v.y = (T)0;
+        if (a.flags & F_HERM_DOUBLE) {
+            if (k > 0) v = scal(v, (T)2);
+            else v.y = (T)0;
+        }
+        break;
+    }
+    default:
+        v = Z[k];
+        if (a.flags & F_CONJ_OUT) v.y = -v.y;
+        break;
+    }
+    return scal(v, a.scale);
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void radix_pass(cpx<T> *ln, const cpx<T> *twp, int m, int Ns, int tl, int tpl)
+{
+    constexpr int MB = (Regs<T>::emax / R) > 0 ? Regs<T>::emax / R : 1;
+    const int nb = m / R;
+    cpx<T> v[MB][R];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int j = tl + tpl * b;
+        if (j < nb) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = ln[j + r * nb];
+            if (Ns > 1) {
+                const int k = j % Ns;
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], twp[k * (R - 1) + (r - 1)]);
+            }
+            Dft<T, R>::run(v[b]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int j = tl + tpl * b;
+        if (j < nb) {
+            const int k = j % Ns;
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) ln[base + r * Ns] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+} // namespace
+
+// Generic radix-R pass for any R (used for prime factors above 13): each
+// thread computes whole outputs as O(R) sums from LDS. roots = exp(-2 pi i
+// e / R), e < R, followed by the usual Ns (R - 1) pass twiddles.
+template <typename T>
+__device__ void radix_pass_generic(cpx<T> *ln, const cpx<T> *roots, int m, int R, int Ns, int tl, int tpl)
+{
+    const cpx<T> *twp = roots + R;
+    const int nb = m / R;
+    constexpr int E = Regs<T>::emax;
+    cpx<T> v[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int o = tl + tpl * i;
+        if (o < m) {
+            const int r = o / nb, j = o - r * nb;
+            const int k = j % Ns;
+            cpx<T> acc = mk((T)0, (T)0);
+            int e = 0;
+            for (int q = 0; q < R; ++q) {
+                cpx<T> x = ln[j + q * nb];
+                if (Ns > 1 && q > 0) x = cmul(x, twp[k * (R - 1) + (q - 1)]);
+                acc = acc + cmul(x, roots[e]);
+                e += r;
+                if (e >= R) e -= R;
+            }
+            v[i] = acc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int o = tl + tpl * i;
+        if (o < m) {
+            const int r = o / nb, j = o - r * nb;
+            const int k = j % Ns;
+            ln[(j - k) * R + k + r * Ns] = v[i];
+        }
+    }
+    __syncthreads();
+}
+
+// WIDE instantiations also carry the radix 7 / 11 / 13 butterflies and the
+// generic pass; the narrow one keeps register pressure down for the
+// power-of-two and 3 / 5 lengths.
+template <typename T, bool WIDE>
+__global__ __launch_bounds__(kMaxThreads) void k_line_stage(const StageArgs<T> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
+    cpx<T> *data = tws + a.twn;
+    if (a.gate != nullptr && *a.gate != 0) return;
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    int64_t tile = blockIdx.x;
+    if (a.flags & F_XCD_REMAP) {
+        tile = (int64_t)(blockIdx.x % 8u) * a.xcd_per + blockIdx.x / 8u;
+        if (tile >= a.ntiles) return;
+    }
+    for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
+
+    const int lt = a.lt, tpl = a.tpl;
+    const bool cin = complex_input(a.mode);
+    // ---- load: global -> LDS
+    {
+        int l, p0, step;
+        if (a.in_col) {
+            l = tid % lt;
+            p0 = tid / lt;
+        } else {
+            l = tid / tpl;
+            p0 = tid % tpl;
+        }
+        step = tpl;
+        const int64_t line = tile * lt + l;
+        if (line < a.nlines) {
+            int64_t io, oo;
+            line_offsets(a.g, line, io, oo);
+            cpx<T> *ln = data + l * a.ls;
+            // all loads of a chunk are issued before the first LDS write so that
+            // up to Regs<T>::unroll global loads per thread are in flight
+            if (cin) {
+                const cpx<T> *s = reinterpret_cast<const cpx<T> *>(a.src) + io;
+                const bool cj = (a.flags & F_CONJ_IN) != 0;
+                for (int base = p0; base < a.nin; base += step * Regs<T>::unroll) {
+                    cpx<T> v[Regs<T>::unroll];
+#pragma unroll
+                    for (int u = 0; u < Regs<T>::unroll; ++u) {
+                        const int p = base + u * step;
+                        if (p < a.nin) v[u] = s[(int64_t)p * a.ips];
+                    }
+#pragma unroll
+                    for (int u = 0; u < Regs<T>::unroll; ++u) {
+                        const int p = base + u * step;
+                        if (p < a.nin) {
+                            cpx<T> w = v[u];
+                            if (cj) w.y = -w.y;
+                            ln[p] = w;
+                        }
+                    }
+                }
+            } else {
+                const T *s = reinterpret_cast<const T *>(a.src) + io;
+                T *lr = reinterpret_cast<T *>(ln);
+                for (int base = p0; base < a.nin; base += step * (2 * Regs<T>::unroll)) {
+                    T v[2 * Regs<T>::unroll];
+#pragma unroll
+                    for (int u = 0; u < 2 * Regs<T>::unroll; ++u) {
+                        const int p = base + u * step;
+                        if (p < a.nin) v[u] = s[(int64_t)p * a.ips];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 2 * Regs<T>::unroll; ++u) {
+                        const int p = base + u * step;
+                        if (p < a.nin) lr[p] = v[u];
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    const int l = tid / tpl, tl = tid % tpl;
+    cpx<T> *ln = data + l * a.ls;
+    // ---- pre-processing: raw line -> FFT input (in place through registers)
+    if (needs_pre(a.mode)) {
+        constexpr int E = Regs<T>::emax;
+        cpx<T> z[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int j = tl + tpl * i;
+            if (j < a.m) z[i] = pre_value(a, ln, tws, j);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int j = tl + tpl * i;
+            if (j < a.m) ln[j] = z[i];
+        }
+        __syncthreads();
+    }
+    // ---- Stockham passes (decimation in time, natural-order output)
+    int Ns = 1;
+    for (int p = 0; p < a.npass; ++p) {
+        const int R = a.radix[p];
+        const cpx<T> *twp = tws + a.twoff[p];
+        switch (R) {
+        case 2: radix_pass<T, 2>(ln, twp, a.m, Ns, tl, tpl); break;
+        case 3: radix_pass<T, 3>(ln, twp, a.m, Ns, tl, tpl); break;
+        case 4: radix_pass<T, 4>(ln, twp, a.m, Ns, tl, tpl); break;
+        case 5: radix_pass<T, 5>(ln, twp, a.m, Ns, tl, tpl); break;
+        case 8: radix_pass<T, 8>(ln, twp, a.m, Ns, tl, tpl); break;
+        case 16:
+            if constexpr (Regs<T>::max_radix >= 16) radix_pass<T, 16>(ln, twp, a.m, Ns, tl, tpl);
+            break;
+        default:
+            if constexpr (WIDE) {
+                if (R == 7) radix_pass<T, 7>(ln, twp, a.m, Ns, tl, tpl);
+                else if (R == 11) radix_pass<T, 11>(ln, twp, a.m, Ns, tl, tpl);
+                else if (R == 13) radix_pass<T, 13>(ln, twp, a.m, Ns, tl, tpl);
+                else radix_pass_generic<T>(ln, twp, a.m, R, Ns, tl, tpl);
+            }
+            break;
+        }
+        Ns *= R;
+    }
+    // ---- post-processing + store: LDS -> global (output layout)
+    {
+        int lo, k0;
+        if (a.out_col) {
+            lo = tid % lt;
+            k0 = tid / lt;
+        } else {
+            lo = tid / tpl;
+            k0 = tid % tpl;
+        }
+        const int64_t line = tile * lt + lo;
+        if (line < a.nlines) {
+            int64_t io, oo;
+            line_offsets(a.g, line, io, oo);
+            const cpx<T> *Z = data + lo * a.ls;
+            if (a.mode == M_COPY_C) {
+                // identity stage: move the bits, no arithmetic (bit-exact permutation)
+                cpx<T> *d = reinterpret_cast<cpx<T> *>(a.dst) + oo;
+                for (int base = k0; base < a.nout; base += tpl * Regs<T>::unroll) {
+#pragma unroll
+                    for (int u = 0; u < Regs<T>::unroll; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) d[(int64_t)k * a.ops] = Z[k];
+                    }
+                }
+            } else if (a.mode == M_COPY_R) {
+                T *d = reinterpret_cast<T *>(a.dst) + oo;
+                const T *zr = reinterpret_cast<const T *>(Z);
+                for (int base = k0; base < a.nout; base += tpl * Regs<T>::unroll) {
+#pragma unroll
+                    for (int u = 0; u < Regs<T>::unroll; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) d[(int64_t)k * a.ops] = zr[k];
+                    }
+                }
+            } else if (complex_output(a.mode)) {
+                cpx<T> *d = reinterpret_cast<cpx<T> *>(a.dst) + oo;
+                for (int base = k0; base < a.nout; base += tpl * Regs<T>::unroll) {
+                    cpx<T> v[Regs<T>::unroll];
+#pragma unroll
+                    for (int u = 0; u < Regs<T>::unroll; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) v[u] = post_complex(a, Z, tws, k);
+                    }
+#pragma unroll
+                    for (int u = 0; u < Regs<T>::unroll; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) d[(int64_t)k * a.ops] = v[u];
+                    }
+                }
+            } else {
+                T *d = reinterpret_cast<T *>(a.dst) + oo;
+                for (int base = k0; base < a.nout; base += tpl * Regs<T>::unroll) {
+                    T v[Regs<T>::unroll];
+#pragma unroll
+                    for (int u = 0; u < Regs<T>::unroll; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) v[u] = post_real(a, Z, tws, k);
+                    }
+#pragma unroll
+                    for (int u = 0; u < Regs<T>::unroll; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) d[(int64_t)k * a.ops] = v[u];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// R2C boundary endpoint statistics (distributed.rs:4205-4363): per extra
+// batch and constrained plane, slot 0 = max(|re|,|im|) then sum (re/max)^2,
+// slot 1 = max |im| then sum (im/max)^2. Values are accumulated in f64.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ const cpx<T> *plane_ptr(const EndpointArgs &e, const void *data, int64_t batch, int plane)
+{
+    const int64_t k = plane == 0 ? 0 : e.clen - 1;
+    return reinterpret_cast<const cpx<T> *>(data) + batch * e.len + k * e.stride;
+}
+
+__device__ __forceinline__ double block_reduce_max(double v, double *sh)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = sh[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) r = fmax(r, sh[i]);
+    return r; // every thread gets the block result
+}
+
+__device__ __forceinline__ double block_reduce_sum(double v, double *sh)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = sh[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+    return r;
+}
+
+// One chunk of one constrained plane: its max(|re|,|im|), max|im|, and the
+// sums of (re/m)^2, (im/m)^2 normalized by the chunk maximum m, plus a
+// non-finite flag. grid: x = chunks (e.nblk), y = batch * planes + plane.
+template <typename T>
+__global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const void *data, EndpointPartial *part)
+{
+    __shared__ double sh[4];
+    const int64_t bp = blockIdx.y;
+    const int64_t batch = bp / e.planes;
+    const int plane = (int)(bp % e.planes);
+    const cpx<T> *base = plane_ptr<T>(e, data, batch, plane);
+    const int64_t count = e.outer * e.stride;
+    const int64_t chunk = (count + gridDim.x - 1) / gridDim.x;
+    const int64_t begin = (int64_t)blockIdx.x * chunk;
+    const int64_t end = begin + chunk < count ? begin + chunk : count;
+    double ma = 0.0, mi = 0.0;
+    int bad = 0;
+    for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+        const int64_t o = i / e.stride, q = i - o * e.stride;
+        const cpx<T> v = base[o * e.clen * e.stride + q];
+        const double re = (double)v.x, im = (double)v.y;
+        if (!isfinite(re) || !isfinite(im)) {
+            bad = 1;
+            continue;
+        }
+        ma = fmax(ma, fmax(fabs(re), fabs(im)));
+        mi = fmax(mi, fabs(im));
+    }
+    ma = block_reduce_max(ma, sh);
+    mi = block_reduce_max(mi, sh);
+    const double anybad = block_reduce_max((double)bad, sh);
+    double sr = 0.0, si = 0.0;
+    if (ma != 0.0) {
+        for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+            const int64_t o = i / e.stride, q = i - o * e.stride;
+            const cpx<T> v = base[o * e.clen * e.stride + q];
+            const double re = (double)v.x, im = (double)v.y;
+            if (!isfinite(re) || !isfinite(im)) continue;
+            const double nr = re / ma, ni = im / ma;
+            sr += nr * nr;
+            si += ni * ni;
+        }
+    }
+    sr = block_reduce_sum(sr, sh);
+    si = block_reduce_sum(si, sh);
+    if (threadIdx.x == 0) {
+        EndpointPartial p;
+        p.maxa = ma;
+        p.maxi = mi;
+        p.sre = sr;
+        p.sim = si;
+        p.bad = anybad != 0.0 ? 1u : 0u;
+        part[bp * e.nblk + blockIdx.x] = p;
+    }
+}
+
+// Combines the chunks of every batch in a fixed order (global max M, sums
+// rescaled by (m_chunk / M)^2, which equals the reference's sums of (x/M)^2),
+// applies the acceptance rule (distributed.rs:4330-4349) and sets *gate and
+// the sticky status.
+__global__ __launch_bounds__(256) void k_endpoint_combine(EndpointArgs e, const EndpointPartial *part, int *gate,
+                                                          int *status)
+{
+    __shared__ int any_bad;
+    if (threadIdx.x == 0) any_bad = 0;
+    __syncthreads();
+    for (int64_t b = threadIdx.x; b < e.extra; b += blockDim.x) {
+        bool invalid = !isfinite(e.relative) || !isfinite(e.absolute);
+        for (int p = 0; p < e.planes; ++p) {
+            const EndpointPartial *q = part + (b * e.planes + p) * e.nblk;
+            double M = 0.0, MI = 0.0;
+            for (int c = 0; c < e.nblk; ++c) {
+                M = fmax(M, q[c].maxa);
+                MI = fmax(MI, q[c].maxi);
+                if (q[c].bad) invalid = true;
+            }
+            double s0 = 0.0, s1 = 0.0;
+            if (M != 0.0) {
+                for (int c = 0; c < e.nblk; ++c) {
+                    const double f = q[c].maxa / M;
+                    s0 += q[c].sre * f * f;
+                    s1 += q[c].sim * f * f;
+                }
+            }
+            if (!isfinite(M) || !isfinite(MI) || !isfinite(s0) || !isfinite(s1)) invalid = true;
+            const bool abs_ok = MI <= e.absolute;
+            const bool rel_ok = sqrt(s1) <= e.relative * sqrt(s0);
+            if (!(abs_ok || rel_ok)) invalid = true;
+        }
+        if (invalid) atomicOr(&any_bad, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *gate = any_bad;
+        if (any_bad) atomicOr(status, PFH_DEVICE_INVALID_SPECTRUM);
+    }
+}
+
+// Local C2R strict endpoint preflight (r2c.rs:798-826): the first failing
+// line, DC before Nyquist, as code = batch * 2 + (Nyquist ? 1 : 0).
+template <typename T>
+__global__ __launch_bounds__(256) void k_local_endpoints(const void *data, int64_t batch, int64_t clen, int even,
+                                                         unsigned long long *first_code)
+{
+    for (int64_t b = threadIdx.x + (int64_t)blockIdx.x * blockDim.x; b < batch; b += (int64_t)gridDim.x * blockDim.x) {
+        const cpx<T> *ln = reinterpret_cast<const cpx<T> *>(data) + b * clen;
+        const bool dc = ln[0].y != (T)0;
+        const bool ny = even && ln[clen - 1].y != (T)0;
+        if (dc || ny) atomicMin(first_code, (unsigned long long)(b * 2 + (dc ? 0 : 1)));
+    }
+}
+
+__global__ void k_local_endpoint_gate(const unsigned long long *first_code, int *gate, int *status)
+{
+    const int bad = (*first_code != ~0ull) ? 1 : 0;
+    *gate = bad;
+    if (bad) atomicOr(status, PFH_DEVICE_INVALID_ENDPOINT);
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v)
+{
+    for (int64_t i = threadIdx.x + (int64_t)blockIdx.x * blockDim.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_fill_i32(int *p, int64_t n, int v)
+{
+    for (int64_t i = threadIdx.x + (int64_t)blockIdx.x * blockDim.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Generic element permutation for element sizes the line kernel does not
+// cover (1 and 2 bytes): one thread per destination element.
+template <typename W>
+__global__ __launch_bounds__(256) void k_permute_words(PermuteArgs p, const W *src, W *dst)
+{
+    const int64_t total = p.len * p.extra;
+    for (int64_t i = threadIdx.x + (int64_t)blockIdx.x * blockDim.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = i / p.len;
+        int64_t rem = i - e * p.len;
+        int64_t so = 0;
+        // i enumerates the destination memory order
+        for (int d = p.nd - 1; d >= 0; --d) {
+            const int64_t idx = rem % p.dext[d];
+            rem /= p.dext[d];
+            so += idx * p.sstride[d];
+        }
+        dst[i] = src[e * p.len + so];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch wrappers
+// ---------------------------------------------------------------------------
+
+template <typename T>
+hipError_t launch_line_stage(const StageArgs<T> &a, bool wide, int threads, size_t lds_bytes, int64_t grid,
+                             hipStream_t s)
+{
+    if (grid <= 0) return hipSuccess;
+    if (wide) hipLaunchKernelGGL((k_line_stage<T, true>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, a);
+    else hipLaunchKernelGGL((k_line_stage<T, false>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, a);
+    return hipGetLastError();
+}
+
+template hipError_t launch_line_stage<float>(const StageArgs<float> &, bool, int, size_t, int64_t, hipStream_t);
+template hipError_t launch_line_stage<double>(const StageArgs<double> &, bool, int, size_t, int64_t, hipStream_t);
+
+hipError_t set_line_stage_lds_limit(size_t bytes)
+{
+    const void *fns[4] = {reinterpret_cast<const void *>(&k_line_stage<float, false>),
+                          reinterpret_cast<const void *>(&k_line_stage<float, true>),
+                          reinterpret_cast<const void *>(&k_line_stage<double, false>),
+                          reinterpret_cast<const void *>(&k_line_stage<double, true>)};
+    for (const void *f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <typename T>
+hipError_t launch_endpoint_check(const EndpointArgs &e, const void *data, EndpointPartial *part, int *gate,
+                                 int *status, hipStream_t s)
+{
+    const int64_t by = e.extra * e.planes;
+    if (by > 0)
+        hipLaunchKernelGGL(k_endpoint_partial<T>, dim3((unsigned)e.nblk, (unsigned)by), dim3(256), 0, s, e, data,
+                           part);
+    hipLaunchKernelGGL(k_endpoint_combine, dim3(1), dim3(256), 0, s, e, part, gate, status);
+    return hipGetLastError();
+}
+
+template hipError_t launch_endpoint_check<float>(const EndpointArgs &, const void *, EndpointPartial *, int *, int *,
+                                                 hipStream_t);
+template hipError_t launch_endpoint_check<double>(const EndpointArgs &, const void *, EndpointPartial *, int *, int *,
+                                                  hipStream_t);
+
+template <typename T>
+hipError_t launch_local_endpoint_check(const void *data, int64_t batch, int64_t clen, int even,
+                                       unsigned long long *first_code, int *gate, int *status, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, first_code, (int64_t)1, ~0ull);
+    if (batch > 0) {
+        int64_t g = (batch + 255) / 256;
+        if (g > 4096) g = 4096;
+        hipLaunchKernelGGL(k_local_endpoints<T>, dim3((unsigned)g), dim3(256), 0, s, data, batch, clen, even, first_code);
+    }
+    hipLaunchKernelGGL(k_local_endpoint_gate, dim3(1), dim3(1), 0, s, first_code, gate, status);
+    return hipGetLastError();
+}
+
+template hipError_t launch_local_endpoint_check<float>(const void *, int64_t, int64_t, int, unsigned long long *, int *,
+                                                       int *, hipStream_t);
+template hipError_t launch_local_endpoint_check<double>(const void *, int64_t, int64_t, int, unsigned long long *,
+                                                        int *, int *, hipStream_t);
+
+hipError_t launch_fill_i32(int *p, int64_t n, int v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)g), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_permute_words(const PermuteArgs &p, const void *src, void *dst, int word, hipStream_t s)
+{
+    const int64_t total = p.len * p.extra;
+    if (total <= 0) return hipSuccess;
+    int64_t g = (total + 255) / 256;
+    if (g > 65536) g = 65536;
+    switch (word) {
+    case 1:
+        hipLaunchKernelGGL(k_permute_words<uint8_t>, dim3((unsigned)g), dim3(256), 0, s, p,
+                           reinterpret_cast<const uint8_t *>(src), reinterpret_cast<uint8_t *>(dst));
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_permute_words<uint16_t>, dim3((unsigned)g), dim3(256), 0, s, p,
+                           reinterpret_cast<const uint16_t *>(src), reinterpret_cast<uint16_t *>(dst));
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+} // namespace pfh

@@ -1,0 +1,62 @@
+// Host-side declarations of the gfx950 kernels' launch wrappers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pfh_stage.hpp"
+
+namespace pfh {
+
+// Sticky device status bits (kept in the caller's workspace).
+constexpr int PFH_DEVICE_INVALID_SPECTRUM = 1;
+constexpr int PFH_DEVICE_INVALID_ENDPOINT = 2;
+
+// One chunk's endpoint statistics of an R2C boundary plane.
+struct EndpointPartial {
+    double maxa; // max(|re|, |im|)
+    double maxi; // max |im|
+    double sre;  // sum (re / maxa)^2
+    double sim;  // sum (im / maxa)^2
+    unsigned bad;
+    unsigned pad_[3];
+};
+
+struct EndpointArgs {
+    int64_t extra;   // batches
+    int64_t len;     // complex elements per batch
+    int64_t outer;   // blocks before the reduction axis
+    int64_t stride;  // memory stride of the reduction axis
+    int64_t clen;    // complex length of the reduction axis
+    int planes;      // 2 (even real length) or 1
+    int nblk;        // chunks per plane
+    double relative, absolute;
+};
+
+struct PermuteArgs {
+    int nd;
+    int64_t len, extra;
+    int64_t dext[8];    // destination memory extents (slowest first)
+    int64_t sstride[8]; // source stride of the axis at each destination position
+};
+
+template <typename T>
+hipError_t launch_line_stage(const StageArgs<T> &a, bool wide, int threads, size_t lds_bytes, int64_t grid,
+                             hipStream_t s);
+
+hipError_t set_line_stage_lds_limit(size_t bytes);
+
+template <typename T>
+hipError_t launch_endpoint_check(const EndpointArgs &e, const void *data, EndpointPartial *part, int *gate,
+                                 int *status, hipStream_t s);
+
+template <typename T>
+hipError_t launch_local_endpoint_check(const void *data, int64_t batch, int64_t clen, int even,
+                                       unsigned long long *first_code, int *gate, int *status, hipStream_t s);
+
+hipError_t launch_fill_i32(int *p, int64_t n, int v, hipStream_t s);
+
+hipError_t launch_permute_words(const PermuteArgs &p, const void *src, void *dst, int word, hipStream_t s);
+
+} // namespace pfh

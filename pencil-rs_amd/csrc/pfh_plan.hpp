@@ -1,0 +1,126 @@
+// Host-side planning structures of the gfx950 FFT path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pencil_fft_hip.h"
+#include "pfh_kernels.hpp"
+
+namespace pfh {
+
+// A memory layout: logical shape, memory order (perm[pos] = logical axis at
+// memory position pos, slowest first), extra blocks and element size. This
+// is the part of a Pencil (crates/pencil-array/src/pencil.rs:69-112) that a
+// one-rank process grid needs.
+struct Layout {
+    int N = 0;
+    std::array<int, 8> perm{};
+    std::array<int64_t, 8> shape{};
+    int64_t extra = 1;
+    int elem = 0; // bytes
+
+    int64_t len() const
+    {
+        int64_t p = 1;
+        for (int a = 0; a < N; ++a) p *= shape[a];
+        return p;
+    }
+    int64_t total() const { return len() * extra; }
+    int64_t bytes() const { return total() * elem; }
+    // element stride of every logical axis
+    std::array<int64_t, 8> strides() const
+    {
+        std::array<int64_t, 8> st{};
+        int64_t run = 1;
+        for (int p = N - 1; p >= 0; --p) {
+            st[perm[p]] = run;
+            run *= shape[perm[p]];
+        }
+        return st;
+    }
+    bool same_order(const Layout &o) const
+    {
+        if (N != o.N) return false;
+        for (int p = 0; p < N; ++p)
+            if (perm[p] != o.perm[p]) return false;
+        return true;
+    }
+};
+
+enum BufId : int { BUF_SRC = 0, BUF_DST = 1, BUF_W0 = 2, BUF_W1 = 3, BUF_COUNT = 4 };
+
+// One kernel launch of a plan operation.
+struct Step {
+    int mode = M_COPY_C;
+    int flags = 0;
+    int n = 0, m = 0, nin = 0, nout = 0;
+    std::vector<int> radices;
+    double scale = 1.0, prew0 = 1.0, prew1 = 1.0, postw0 = 1.0, postw1 = 1.0;
+    int axis = -1;
+    bool in_cplx = true, out_cplx = true;
+    Layout in, out;
+    int in_buf = BUF_SRC, out_buf = BUF_DST;
+    // launch shape
+    LineGeom g{};
+    int64_t nlines = 0, ips = 1, ops = 1;
+    int in_col = 0, out_col = 0;
+    int lt = 1, tpl = 1, ls = 1, threads = 64;
+    int64_t tiles = 0, grid = 0;
+    int xcd_per = 0;
+    size_t lds_bytes = 0;
+    // tables (complex entries, offsets relative to tw_base)
+    int64_t tw_base = 0;
+    int twn = 0;
+    std::array<int, kMaxPasses> twoff{};
+    int pre0 = 0, pre1 = 0, post0 = 0, post1 = 0;
+    // R2C boundary acceptance check before this step
+    bool endpoint_check = false;
+    EndpointArgs ep{};
+    bool local_endpoint_check = false;
+    bool gated = false; // the check ran as its own launch; the stage still reads the gate
+};
+
+struct OpPlan {
+    bool valid = false;
+    std::vector<Step> steps;
+    Layout in, out;          // layouts of src and dst for this op
+    int64_t w_bytes[2] = {0, 0};
+};
+
+struct DeviceInfo {
+    int device = 0;
+    size_t lds_limit = 65536;
+    int cus = 256;
+};
+
+struct PlanImpl {
+    pfh_desc desc{};
+    int precision = PFH_F32;
+    int kind = 0;
+    int N = 0;
+    int64_t extra = 1;
+    int64_t partials = 1; // endpoint-statistics chunks kept in the workspace status area
+    DeviceInfo dev;
+    // op -> plan (out of place and in place)
+    OpPlan ops[6];
+    OpPlan ops_inplace[6];
+    // twiddle/pre/post tables of every step of every op, one device buffer
+    std::vector<double> host_tables; // complex pairs, f64
+    void *dev_tables = nullptr;
+    int64_t table_entries = 0;
+    // transposes
+    Layout tr_src, tr_dst;
+    int word = 0, words_per_elem = 1;
+    // local plans
+    size_t line_len = 0;
+    int r2r_kind = 0;
+    ~PlanImpl();
+};
+
+} // namespace pfh

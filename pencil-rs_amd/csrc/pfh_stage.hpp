@@ -1,0 +1,96 @@
+// Launch description of one line stage: a batch of 1D lines, each read from
+// one layout, transformed, and written to another layout. The layout change
+// is how a route transition (the reference's pack / exchange / unpack,
+// crates/pencil-array/src/transpose.rs:1107-1315, or a LocalTransposePlan
+// permutation, local_transpose.rs:87-132) is fused into a stage.
+#pragma once
+
+#include <cstdint>
+
+#include "pfh_math.hpp"
+
+namespace pfh {
+
+constexpr int kMaxPasses = 12;
+constexpr int kMaxDigits = 5;
+// Complex values a thread holds in registers during one radix pass: 16 for
+// f32, 8 for f64 (same register footprint).
+template <typename T>
+struct Regs {
+    static constexpr int emax = sizeof(T) == 4 ? 16 : 8;
+    static constexpr int unroll = sizeof(T) == 4 ? 16 : 8; // loads / stores in flight per chunk
+    static constexpr int max_radix = sizeof(T) == 4 ? 16 : 8;
+};
+constexpr int kEmax = 16; // upper bound over precisions
+constexpr int kMaxThreads = 1024;
+
+// Per-line transform applied by a stage (see pfh_kernels.hip for the math).
+enum StageMode : int {
+    M_COPY_C = 0,  // identity on complex values (layout change only)
+    M_COPY_R,      // identity on real values
+    M_C2C,         // complex FFT of length n
+    M_R2C,         // real n (even) -> n/2+1 complex via the n/2 complex FFT
+    M_R2C_ODD,     // real n (odd) -> (n+1)/2 complex via the n complex FFT
+    M_C2R,         // n/2+1 complex -> real n (even), positive sign
+    M_C2R_ODD,     // (n+1)/2 complex -> real n (odd)
+    M_DCT2,        // REDFT10, n even (Makhoul reorder + n/2 complex FFT)
+    M_DCT2_ODD,    // REDFT10, n odd (n complex FFT)
+    M_DCT3,        // REDFT01, n even (transpose of M_DCT2)
+    M_DCT3_ODD,    // REDFT01, n odd
+    M_DCT4,        // REDFT11, n even (n/2 complex FFT, pre/post twiddle)
+    M_DCT4_ODD,    // REDFT11, n odd (2n complex FFT of the twiddled line)
+    M_DCT1,        // REDFT00 (even extension, n-1 complex FFT)
+    M_DST1,        // RODFT00 (odd extension, n+1 complex FFT)
+    M_DHT,         // Hartley, n even (n/2 complex FFT)
+    M_DHT_ODD,     // Hartley, n odd (n complex FFT)
+    M_COUNT
+};
+
+enum StageFlags : int {
+    F_CONJ_IN = 1,      // conjugate on load (positive-sign C2C)
+    F_CONJ_OUT = 2,     // conjugate on store
+    F_PRE_REV = 4,      // read x[n-1-j] (DST-III / DST-IV from DCT-III / DCT-IV)
+    F_PRE_ALT = 8,      // read (-1)^j x[j] (DST-II from DCT-II)
+    F_POST_REV = 16,    // write y[k] = t[n-1-k]
+    F_POST_ALT = 32,    // write y[k] = (-1)^k t[k]
+    F_HERM_DOUBLE = 64, // R2C adjoint weights: x2 on interior bins, Im := 0 at DC/Nyquist
+    F_HERM_HALF = 128,  // C2R adjoint weights: x1/2 on interior bins
+    F_WEIGHTS = 256,    // apply pre/post edge weights (R2R adjoints)
+    F_XCD_REMAP = 512,  // map consecutive tiles onto one XCD
+};
+
+struct LineGeom {
+    int nd;                       // number of line digits
+    int64_t ext[kMaxDigits];      // digit extents, digit 0 fastest
+    int64_t is[kMaxDigits];       // input element stride of each digit
+    int64_t os[kMaxDigits];       // output element stride of each digit
+};
+
+template <typename T>
+struct StageArgs {
+    const void *src;
+    void *dst;
+    const cpx<T> *tw;       // this stage's twiddle / pre / post tables
+    const int *gate;        // if non-null and *gate != 0 the stage writes nothing
+    int64_t nlines;
+    int64_t ntiles;
+    LineGeom g;
+    int64_t ips, ops;       // point strides of the transform axis (elements)
+    int mode, flags;
+    int n;                  // logical line length
+    int nin, nout;          // elements per line read / written
+    int m;                  // complex FFT length (0: no FFT)
+    int npass;
+    int radix[kMaxPasses];
+    int twoff[kMaxPasses];  // pass twiddle offsets (complex units) into tw
+    int pre0, pre1, post0, post1; // table offsets into tw
+    int twn;                // complex entries of tw staged into LDS
+    int lt, tpl;            // lines per tile, threads per line
+    int ls;                 // LDS stride of one line (complex units)
+    int in_col, out_col;    // column (lines adjacent in memory) access on load / store
+    int xcd_per;            // tiles per XCD when F_XCD_REMAP
+    T scale;
+    T prew0, prew1, postw0, postw1;
+};
+
+} // namespace pfh

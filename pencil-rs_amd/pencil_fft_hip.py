@@ -1,0 +1,584 @@
+"""Python mirror of pencil-rs's FFT plan API over the gfx950 C ABI.
+
+Each class keeps the name, argument meaning and error behaviour of the Rust
+type it mirrors (cited per class). All work runs in ``libpencilfft_hip.so``
+(hand-written HIP kernels for gfx950) on device-resident torch tensors; torch
+is only used for device memory and the current stream. There is no CPU
+fallback: constructing a plan without the library or without a GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Iterable, Optional, Sequence
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpencilfft_hip.so")
+MAX_DIMS = 4
+
+# pfh_kind
+LOCAL_C2C, LOCAL_R2C, LOCAL_R2R, LOCAL_DHT, C2C, R2C, R2R, LOCAL_TRANSPOSE, ALLTOALLV_TRANSPOSE = range(1, 10)
+# pfh_op
+FORWARD, INVERSE, BACKWARD, ADJOINT_FORWARD, ADJOINT_INVERSE, ADJOINT_BACKWARD = range(6)
+EXEC_DEFER_STATUS = 1
+
+
+class R2rKind:
+    """R2rKind (crates/pencil-fft/src/r2r.rs:17-84) with the reference's
+    descriptor codes (r2r.rs:53-64); DHT is AxisR2rKind::Dht (code 9)."""
+
+    DctI, DctII, DctIII, DctIV, DstI, DstII, DstIII, DstIV = range(1, 9)
+    Dht = 9
+
+    _BACKWARD = {1: 1, 2: 3, 3: 2, 4: 4, 5: 5, 6: 7, 7: 6, 8: 8, 9: 9}
+
+    @staticmethod
+    def backward_kind(kind: int) -> int:
+        return R2rKind._BACKWARD[kind]
+
+
+STATUS_NAMES = {
+    0: "Ok", 1: "InvalidLength", 2: "LengthOverflow", 3: "NonIntegralBatch", 4: "BufferLengthMismatch",
+    5: "BatchCountMismatch", 6: "ScratchTooSmall", 7: "InvalidSpectrumEndpoint", 8: "InvalidDimensions",
+    9: "InvalidInputLayout", 10: "InputLayoutMismatch", 11: "OutputLayoutMismatch", 12: "ExtraShapeMismatch",
+    13: "WorkspaceMismatch", 14: "WorkspaceTooSmall", 15: "PreparationFailed", 16: "AllocationFailed",
+    17: "InvalidSpectrum", 18: "IncompatibleDistribution", 19: "SourceLayoutMismatch",
+    20: "DestinationLayoutMismatch", 21: "InvalidPermutation", 22: "ZeroGlobalExtent", 23: "InvalidArgument",
+    24: "Unsupported", 25: "Device",
+}
+
+
+class PfhError(Exception):
+    """A non-Ok pfh_status. ``variant`` is the Rust error variant name;
+    payload fields mirror the variant's fields."""
+
+    def __init__(self, status: int, info: "_ErrorInfo"):
+        self.status = status
+        self.variant = STATUS_NAMES.get(status, f"Status{status}")
+        self.required = info.required
+        self.actual = info.actual
+        self.batch = info.batch
+        self.index = info.index
+        self.hip_error = info.hip_error
+        msg = info.message.decode(errors="replace")
+        super().__init__(f"{self.variant}: {msg}")
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("ndim", ctypes.c_int),
+        ("topology_dims", ctypes.c_int),
+        ("shape", ctypes.c_size_t * MAX_DIMS),
+        ("extra_ndim", ctypes.c_int),
+        ("extra_shape", ctypes.c_size_t * MAX_DIMS),
+        ("selected", ctypes.c_uint),
+        ("axis_kind", ctypes.c_int * MAX_DIMS),
+        ("direction", ctypes.c_int * MAX_DIMS),
+        ("permute_dims", ctypes.c_int),
+        ("transpose_method", ctypes.c_int),
+        ("src_perm", ctypes.c_int * MAX_DIMS),
+        ("dst_perm", ctypes.c_int * MAX_DIMS),
+        ("element_bytes", ctypes.c_size_t),
+        ("line_len", ctypes.c_size_t),
+        ("r2r_kind", ctypes.c_int),
+        ("flags", ctypes.c_int),
+    ]
+
+
+class _ErrorInfo(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("required", ctypes.c_size_t),
+        ("actual", ctypes.c_size_t),
+        ("batch", ctypes.c_size_t),
+        ("index", ctypes.c_size_t),
+        ("hip_error", ctypes.c_int),
+        ("message", ctypes.c_char * 160),
+    ]
+
+
+class TransformTiming(ctypes.Structure):
+    """TransformTiming / StageTiming (distributed.rs:290-325)."""
+
+    _fields_ = [
+        ("stages", ctypes.c_int),
+        ("stage_ms", ctypes.c_double * 12),
+        ("stage_axis", ctypes.c_int * 12),
+        ("stage_bytes", ctypes.c_double * 12),
+        ("total_ms", ctypes.c_double),
+    ]
+
+    def entries(self):
+        """[(axis, ms, bytes)] in execution order; axis -1 is an endpoint check."""
+        return [(self.stage_axis[i], self.stage_ms[i], self.stage_bytes[i]) for i in range(self.stages)]
+
+
+_lib = None
+
+EXPORTED_SYMBOLS = (
+    "pfh_version", "pfh_last_error", "pfh_status_string", "pfh_plan_create", "pfh_plan_destroy",
+    "pfh_plan_workspace_bytes", "pfh_plan_layout", "pfh_execute", "pfh_execute_in_place", "pfh_local_execute",
+    "pfh_local_execute_in_place", "pfh_execute_with_timing", "pfh_workspace_take_status", "pfh_batch_create",
+    "pfh_batch_destroy", "pfh_batch_workspace_bytes", "pfh_batch_execute",
+)
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libpencilfft_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(path)
+    vp, sz, i = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+    lib.pfh_version.restype = ctypes.c_char_p
+    lib.pfh_status_string.restype = ctypes.c_char_p
+    lib.pfh_status_string.argtypes = [i]
+    lib.pfh_last_error.argtypes = [ctypes.POINTER(_ErrorInfo)]
+    lib.pfh_plan_create.argtypes = [ctypes.POINTER(_Desc), ctypes.POINTER(vp)]
+    lib.pfh_plan_destroy.argtypes = [vp]
+    lib.pfh_plan_workspace_bytes.argtypes = [vp, i, ctypes.POINTER(sz)]
+    lib.pfh_plan_layout.argtypes = [vp, i, ctypes.POINTER(ctypes.c_int * MAX_DIMS),
+                                    ctypes.POINTER(ctypes.c_size_t * MAX_DIMS), ctypes.POINTER(sz),
+                                    ctypes.POINTER(sz)]
+    lib.pfh_execute.argtypes = [vp, i, vp, sz, vp, sz, vp, sz, vp, i]
+    lib.pfh_execute_in_place.argtypes = [vp, i, vp, sz, vp, sz, vp, i]
+    lib.pfh_local_execute.argtypes = [vp, i, vp, sz, vp, sz, vp, sz, vp, i]
+    lib.pfh_local_execute_in_place.argtypes = [vp, i, vp, sz, sz, vp, sz, vp, i]
+    lib.pfh_execute_with_timing.argtypes = [vp, i, vp, sz, vp, sz, vp, sz, vp, ctypes.POINTER(TransformTiming)]
+    lib.pfh_workspace_take_status.argtypes = [vp, vp, vp]
+    lib.pfh_batch_create.argtypes = [ctypes.POINTER(_Desc), i, i, ctypes.POINTER(vp)]
+    lib.pfh_batch_destroy.argtypes = [vp]
+    lib.pfh_batch_workspace_bytes.argtypes = [vp, ctypes.POINTER(sz)]
+    lib.pfh_batch_execute.argtypes = [vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), vp, sz, vp, i]
+    _lib = lib
+    return lib
+
+
+def _check(status: int):
+    if status != 0:
+        info = _ErrorInfo()
+        _lib.pfh_last_error(ctypes.byref(info))
+        raise PfhError(status, info)
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("pencil_fft_hip needs a ROCm GPU; there is no CPU fallback")
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("pencil_fft_hip buffers must be device tensors")
+    if not t.is_contiguous():
+        raise ValueError("pencil_fft_hip buffers must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _precision(dtype: torch.dtype) -> int:
+    if dtype in (torch.float32, torch.complex64):
+        return 1
+    if dtype in (torch.float64, torch.complex128):
+        return 2
+    raise TypeError(f"unsupported dtype {dtype}")
+
+
+def _real_dtype(precision: int):
+    return torch.float32 if precision == 1 else torch.float64
+
+
+def _complex_dtype(precision: int):
+    return torch.complex64 if precision == 1 else torch.complex128
+
+
+class _Plan:
+    _kind = 0
+
+    def __init__(self, desc: _Desc):
+        _require_gpu()
+        lib = load_library()
+        handle = ctypes.c_void_p()
+        _check(lib.pfh_plan_create(ctypes.byref(desc), ctypes.byref(handle)))
+        self._h = handle
+        self._desc = desc
+        self.precision = desc.precision
+        self._ws = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.pfh_plan_destroy(h)
+            self._h = None
+
+    # -- workspace ---------------------------------------------------------
+    def workspace_bytes(self, op: int = FORWARD) -> int:
+        n = ctypes.c_size_t()
+        _check(_lib.pfh_plan_workspace_bytes(self._h, op, ctypes.byref(n)))
+        return n.value
+
+    def allocate_workspace(self) -> torch.Tensor:
+        """A workspace large enough for every operation of this plan
+        (the reference's *Workspace types, bound to one plan)."""
+        need = max(self.workspace_bytes(op) for op in range(6))
+        return torch.zeros(max(need, 256), dtype=torch.uint8, device="cuda")
+
+    def _workspace(self, ws: Optional[torch.Tensor]) -> torch.Tensor:
+        if ws is not None:
+            return ws
+        if self._ws is None:
+            self._ws = self.allocate_workspace()
+        return self._ws
+
+    def layout(self, side: int):
+        """(memory order, logical shape, elements, element bytes) of the
+        input (side 0) or output (side 1) array: input_pencil() /
+        output_pencil() (distributed.rs:1316-1329)."""
+        perm = (ctypes.c_int * MAX_DIMS)()
+        shape = (ctypes.c_size_t * MAX_DIMS)()
+        elems, eb = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(_lib.pfh_plan_layout(self._h, side, ctypes.byref(perm), ctypes.byref(shape), ctypes.byref(elems),
+                                    ctypes.byref(eb)))
+        nd = self._desc.ndim if self._desc.ndim else 1
+        return tuple(perm[:nd]), tuple(shape[:nd]), elems.value, eb.value
+
+    def take_status(self, ws: Optional[torch.Tensor] = None):
+        """Reads the deferred device status of a workspace (synchronizes)."""
+        _check(_lib.pfh_workspace_take_status(self._h, _ptr(self._workspace(ws)), _stream()))
+
+
+def _fill_common(desc: _Desc, kind: int, dtype, shape=(), extra_shape=()):
+    desc.kind = kind
+    desc.precision = _precision(dtype)
+    desc.ndim = len(shape)
+    for a, n in enumerate(shape):
+        desc.shape[a] = n
+    desc.extra_ndim = len(extra_shape)
+    for a, n in enumerate(extra_shape):
+        desc.extra_shape[a] = n
+
+
+# ------------------------------------------------------------------ local --
+
+class _LocalPlan(_Plan):
+    def _run(self, op, src, dst, ws=None, flags=0):
+        w = self._workspace(ws)
+        _check(_lib.pfh_local_execute(self._h, op, _ptr(src), src.numel(), _ptr(dst), dst.numel(), _ptr(w),
+                                      w.numel(), _stream(), flags))
+        return dst
+
+    def _run_in_place(self, op, data, elems, ws=None, flags=0):
+        w = self._workspace(ws)
+        _check(_lib.pfh_local_execute_in_place(self._h, op, _ptr(data), elems, data.numel() * data.element_size(),
+                                               _ptr(w), w.numel(), _stream(), flags))
+        return data
+
+
+class LocalC2cPlan(_LocalPlan):
+    """LocalC2cPlan (crates/pencil-fft/src/lib.rs:367-655): batched
+    unnormalized C2C lines; inverse = backward * (1/n)."""
+
+    def __init__(self, line_len: int, dtype=torch.complex64):
+        d = _Desc()
+        _fill_common(d, LOCAL_C2C, dtype)
+        d.line_len = line_len
+        self.line_len = line_len
+        super().__init__(d)
+
+    def forward(self, src, dst=None, workspace=None):
+        return self._run(FORWARD, src, torch.empty_like(src) if dst is None else dst, workspace)
+
+    def backward(self, src, dst=None, workspace=None):
+        return self._run(BACKWARD, src, torch.empty_like(src) if dst is None else dst, workspace)
+
+    def inverse(self, src, dst=None, workspace=None):
+        return self._run(INVERSE, src, torch.empty_like(src) if dst is None else dst, workspace)
+
+    def adjoint(self, op, src, dst=None, workspace=None):
+        return self._run(op, src, torch.empty_like(src) if dst is None else dst, workspace)
+
+    def forward_in_place(self, data, workspace=None):
+        return self._run_in_place(FORWARD, data, data.numel(), workspace)
+
+    def backward_in_place(self, data, workspace=None):
+        return self._run_in_place(BACKWARD, data, data.numel(), workspace)
+
+    def inverse_in_place(self, data, workspace=None):
+        return self._run_in_place(INVERSE, data, data.numel(), workspace)
+
+
+class LocalR2cPlan(_LocalPlan):
+    """LocalR2cPlan (crates/pencil-fft/src/r2c.rs:260-813): real n ->
+    n/2+1 complex; inverse/backward reject non-zero DC / Nyquist imaginary
+    parts before any write (InvalidSpectrumEndpoint {batch, index})."""
+
+    def __init__(self, real_len: int, dtype=torch.float32):
+        d = _Desc()
+        _fill_common(d, LOCAL_R2C, dtype)
+        d.line_len = real_len
+        self.real_len = real_len
+        self.complex_len = real_len // 2 + 1
+        super().__init__(d)
+
+    def _out_real(self, src):
+        batch = src.numel() // self.complex_len
+        return torch.empty(batch * self.real_len, dtype=_real_dtype(self.precision), device=src.device)
+
+    def _out_complex(self, src):
+        batch = src.numel() // self.real_len
+        return torch.empty(batch * self.complex_len, dtype=_complex_dtype(self.precision), device=src.device)
+
+    def forward(self, src, dst=None, workspace=None):
+        return self._run(FORWARD, src, self._out_complex(src) if dst is None else dst, workspace)
+
+    def inverse(self, src, dst=None, workspace=None):
+        return self._run(INVERSE, src, self._out_real(src) if dst is None else dst, workspace)
+
+    def backward(self, src, dst=None, workspace=None):
+        return self._run(BACKWARD, src, self._out_real(src) if dst is None else dst, workspace)
+
+    def adjoint_forward(self, src, dst=None, workspace=None):
+        return self._run(ADJOINT_FORWARD, src, self._out_real(src) if dst is None else dst, workspace)
+
+    def adjoint_backward(self, src, dst=None, workspace=None):
+        return self._run(ADJOINT_BACKWARD, src, self._out_complex(src) if dst is None else dst, workspace)
+
+    def adjoint_inverse(self, src, dst=None, workspace=None):
+        return self._run(ADJOINT_INVERSE, src, self._out_complex(src) if dst is None else dst, workspace)
+
+
+class LocalR2rPlan(_LocalPlan):
+    """LocalR2rPlan (crates/pencil-fft/src/r2r.rs:287-778): FFTW
+    REDFT/RODFT kinds, backward = paired kind, inverse = paired kind /
+    normalization_factor."""
+
+    def __init__(self, line_len: int, kind: int, dtype=torch.float32):
+        d = _Desc()
+        _fill_common(d, LOCAL_R2R, dtype)
+        d.line_len = line_len
+        d.r2r_kind = kind
+        self.line_len = line_len
+        self.kind = kind
+        super().__init__(d)
+
+    def normalization_factor(self) -> int:
+        n = self.line_len
+        return {1: 2 * (n - 1), 5: 2 * (n + 1)}.get(self.kind, 2 * n)
+
+    def forward(self, src, dst=None, workspace=None):
+        return self._run(FORWARD, src, torch.empty_like(src) if dst is None else dst, workspace)
+
+    def backward(self, src, dst=None, workspace=None):
+        return self._run(BACKWARD, src, torch.empty_like(src) if dst is None else dst, workspace)
+
+    def inverse(self, src, dst=None, workspace=None):
+        return self._run(INVERSE, src, torch.empty_like(src) if dst is None else dst, workspace)
+
+    def adjoint(self, op, src, dst=None, workspace=None):
+        return self._run(op, src, torch.empty_like(src) if dst is None else dst, workspace)
+
+    def forward_in_place(self, data, workspace=None):
+        return self._run_in_place(FORWARD, data, data.numel(), workspace)
+
+    def inverse_in_place(self, data, workspace=None):
+        return self._run_in_place(INVERSE, data, data.numel(), workspace)
+
+
+class LocalDhtPlan(LocalR2rPlan):
+    """LocalDhtPlan (crates/pencil-fft/src/dht.rs:32-303): self-paired
+    Hartley transform; inverse divides by n."""
+
+    def __init__(self, line_len: int, dtype=torch.float32):
+        d = _Desc()
+        _fill_common(d, LOCAL_DHT, dtype)
+        d.line_len = line_len
+        self.line_len = line_len
+        self.kind = R2rKind.Dht
+        _Plan.__init__(self, d)
+
+    def normalization_factor(self) -> int:
+        return self.line_len
+
+
+# ------------------------------------------------------------ distributed --
+
+class _RoutePlan(_Plan):
+    """Common execution of the one-rank distributed plans."""
+
+    def _in_out(self, op):
+        fwd_like = op in (FORWARD, ADJOINT_INVERSE, ADJOINT_BACKWARD)
+        _, _, in_el, in_b = self.layout(0)
+        _, _, out_el, out_b = self.layout(1)
+        return ((in_el, in_b), (out_el, out_b)) if fwd_like else ((out_el, out_b), (in_el, in_b))
+
+    def _alloc(self, elems, eb, device):
+        real = _real_dtype(self.precision)
+        cplx = _complex_dtype(self.precision)
+        dt = cplx if eb == 2 * torch.tensor([], dtype=real).element_size() else real
+        return torch.empty(elems, dtype=dt, device=device)
+
+    def execute(self, op, src, dst=None, workspace=None, flags=0):
+        (_, _), (oe, ob) = self._in_out(op)
+        if dst is None:
+            dst = self._alloc(oe, ob, src.device)
+        w = self._workspace(workspace)
+        _check(_lib.pfh_execute(self._h, op, _ptr(src), src.numel(), _ptr(dst), dst.numel(), _ptr(w), w.numel(),
+                                _stream(), flags))
+        return dst
+
+    def execute_with_timing(self, op, src, dst=None, workspace=None) -> tuple:
+        (_, _), (oe, ob) = self._in_out(op)
+        if dst is None:
+            dst = self._alloc(oe, ob, src.device)
+        w = self._workspace(workspace)
+        t = TransformTiming()
+        _check(_lib.pfh_execute_with_timing(self._h, op, _ptr(src), src.numel(), _ptr(dst), dst.numel(), _ptr(w),
+                                            w.numel(), _stream(), ctypes.byref(t)))
+        return dst, t
+
+    def execute_in_place(self, op, data, workspace=None, flags=0):
+        w = self._workspace(workspace)
+        _check(_lib.pfh_execute_in_place(self._h, op, _ptr(data), data.numel() * data.element_size(), _ptr(w),
+                                         w.numel(), _stream(), flags))
+        return data
+
+    def forward(self, src, dst=None, workspace=None, flags=0):
+        return self.execute(FORWARD, src, dst, workspace, flags)
+
+    def inverse(self, src, dst=None, workspace=None, flags=0):
+        return self.execute(INVERSE, src, dst, workspace, flags)
+
+    def backward(self, src, dst=None, workspace=None, flags=0):
+        return self.execute(BACKWARD, src, dst, workspace, flags)
+
+    def adjoint_forward(self, src, dst=None, workspace=None):
+        return self.execute(ADJOINT_FORWARD, src, dst, workspace)
+
+    def adjoint_inverse(self, src, dst=None, workspace=None):
+        return self.execute(ADJOINT_INVERSE, src, dst, workspace)
+
+    def adjoint_backward(self, src, dst=None, workspace=None):
+        return self.execute(ADJOINT_BACKWARD, src, dst, workspace)
+
+    def input_elements(self) -> int:
+        return self.layout(0)[2]
+
+    def output_elements(self) -> int:
+        return self.layout(1)[2]
+
+
+def _route_desc(kind, shape, dtype, extra_shape=(), selection=None, permute_dims=True, topology_dims=0):
+    d = _Desc()
+    _fill_common(d, kind, dtype, shape, extra_shape)
+    if selection is not None:
+        mask = 0
+        for a in selection:
+            mask |= 1 << a
+        d.selected = mask if mask else 0
+        if not mask:
+            d.selected = 0x80000000  # explicit empty selection
+    d.permute_dims = int(bool(permute_dims))
+    d.topology_dims = topology_dims
+    return d
+
+
+class C2cPlan(_RoutePlan):
+    """C2cPlan (crates/pencil-fft/src/distributed.rs:1016-1800) on a
+    one-rank process grid. Output memory order is the reversed axes when
+    permute_dims is set (build_route, distributed.rs:2306-2373)."""
+
+    def __init__(self, shape: Sequence[int], dtype=torch.complex64, extra_shape=(), selection=None,
+                 directions=None, permute_dims=True, topology_dims=0):
+        d = _route_desc(C2C, shape, dtype, extra_shape, selection, permute_dims, topology_dims)
+        if directions is not None:
+            for a, v in enumerate(directions):
+                d.direction[a] = int(v)
+        super().__init__(d)
+        self.shape = tuple(shape)
+
+
+class R2cPlan(_RoutePlan):
+    """R2cPlan (crates/pencil-fft/src/distributed/r2c.rs:255-1130): the
+    reduction axis is the last selected axis; inverse/backward run the
+    endpoint acceptance check (r2c.rs:4709-4878) and raise
+    InvalidSpectrum without writing dst."""
+
+    def __init__(self, shape: Sequence[int], dtype=torch.float32, extra_shape=(), selection=None,
+                 permute_dims=True, topology_dims=0):
+        d = _route_desc(R2C, shape, dtype, extra_shape, selection, permute_dims, topology_dims)
+        super().__init__(d)
+        self.shape = tuple(shape)
+
+
+class R2rPlan(_RoutePlan):
+    """R2rPlan (crates/pencil-fft/src/distributed/r2r.rs:71-1460): one
+    Option<AxisR2rKind> per axis (None = identity, R2rKind.Dht = Hartley)."""
+
+    def __init__(self, shape: Sequence[int], kinds: Sequence[Optional[int]], dtype=torch.float32, extra_shape=(),
+                 permute_dims=True, topology_dims=0):
+        d = _route_desc(R2R, shape, dtype, extra_shape, None, permute_dims, topology_dims)
+        for a, k in enumerate(kinds):
+            d.axis_kind[a] = int(k or 0)
+        super().__init__(d)
+        self.shape = tuple(shape)
+        self.kinds = tuple(kinds)
+
+
+class DhtPlan(R2rPlan):
+    """DhtPlan (crates/pencil-fft/src/distributed/r2r.rs:83-360): Hartley
+    transform on the selected axes."""
+
+    def __init__(self, shape: Sequence[int], dtype=torch.float32, extra_shape=(), selection=None,
+                 permute_dims=True, topology_dims=0):
+        sel = range(len(shape)) if selection is None else selection
+        kinds = [R2rKind.Dht if a in sel else None for a in range(len(shape))]
+        super().__init__(shape, kinds, dtype, extra_shape, permute_dims, topology_dims)
+
+
+class LocalTransposePlan(_Plan):
+    """LocalTransposePlan (crates/pencil-array/src/local_transpose.rs:46-277):
+    bit-exact memory-axis permutation between two layouts of one shape."""
+
+    _kind = LOCAL_TRANSPOSE
+
+    def __init__(self, shape: Sequence[int], src_perm: Sequence[int], dst_perm: Sequence[int], element_bytes: int,
+                 extra_shape=()):
+        d = _Desc()
+        _fill_common(d, self._kind, torch.float32, shape, extra_shape)
+        for a in range(len(shape)):
+            d.src_perm[a] = src_perm[a]
+            d.dst_perm[a] = dst_perm[a]
+        d.element_bytes = element_bytes
+        super().__init__(d)
+
+    def execute(self, src, dst, backward=False):
+        w = self._workspace(None)
+        _check(_lib.pfh_execute(self._h, BACKWARD if backward else FORWARD, _ptr(src),
+                                src.numel() * src.element_size() // self._desc.element_bytes, _ptr(dst),
+                                dst.numel() * dst.element_size() // self._desc.element_bytes, _ptr(w), w.numel(),
+                                _stream(), 0))
+        return dst
+
+    def execute_in_place(self, data, scratch=None):
+        w = scratch if scratch is not None else self._workspace(None)
+        _check(_lib.pfh_execute_in_place(self._h, FORWARD, _ptr(data), data.numel() * data.element_size(), _ptr(w),
+                                         w.numel(), _stream(), 0))
+        return data
+
+
+class AllToAllvTransposePlan(LocalTransposePlan):
+    """AllToAllvTransposePlan (crates/pencil-array/src/alltoallv_transpose.rs
+    :50-311) on one rank: pack, self exchange and unpack fused into one
+    device permutation."""
+
+    _kind = ALLTOALLV_TRANSPOSE

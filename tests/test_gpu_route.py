@@ -1,0 +1,271 @@
+"""GPU parity of the one-rank distributed plans and the layout permutations.
+
+Shapes follow the reference's fixture case list
+(tools/fftw-reference/README.md:271-294) and its input formulas
+(tools/fftw-reference/generate.jl:201-250): full-axis C2C / R2C, partial
+selections, extras, mixed R2R / DHT kinds. Every operation, including the
+adjoints, is compared with the oracle; permutations are bit-exact with
+integer-coded values (pencil-array/tests/local_transpose.rs:78-91).
+"""
+import itertools
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import pencil_fft_hip as pf
+from util import complex_of, real_of, rel_l2, tol, uniform_complex, uniform_real
+
+pytestmark = pytest.mark.gpu
+
+OPS = [O.FORWARD, O.INVERSE, O.BACKWARD, O.ADJ_FORWARD, O.ADJ_INVERSE, O.ADJ_BACKWARD]
+
+
+def _tdev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def complex_value(coords, seed):
+    """generate.jl:201-220 (coordinates in Rust logical order)."""
+    re = 0.173 + 0.019 * seed
+    im = -0.271 - 0.013 * seed
+    for axis, c in enumerate(coords, start=1):
+        re += (0.071 + 0.009 * axis) * (c + 1)
+        im += (0.053 + 0.011 * axis) * (c + 2)
+        re += 0.017 * np.sin((axis + 2) * (c + 1) + seed * 0.07)
+        im += 0.023 * np.cos((axis + 3) * (c + 2) - seed * 0.05)
+    for left in range(1, len(coords)):
+        for right in range(left + 1, len(coords) + 1):
+            re += 0.0067 * left * (right + 1) * (coords[left - 1] + 1) * (coords[right - 1] + 2)
+            im -= 0.0049 * (left + 1) * right * (coords[left - 1] + 2) * (coords[right - 1] + 1)
+    return re + 1j * im
+
+
+def real_value(coords, seed):
+    """generate.jl:222-234."""
+    v = 0.417 + 0.023 * seed
+    for axis, c in enumerate(coords, start=1):
+        v += (0.087 + 0.013 * axis) * (c + 1)
+        v += 0.031 * np.sin((axis + 1) * (c + 2) + seed * 0.09)
+    for left in range(1, len(coords)):
+        for right in range(left + 1, len(coords) + 1):
+            v += 0.0083 * (left + 1) * (right + 2) * (coords[left - 1] + 1) * (coords[right - 1] + 2)
+    return v
+
+
+def fixture_field(shape, extra, seed, complex_):
+    out = np.empty((extra,) + tuple(shape), dtype=np.complex128 if complex_ else np.float64)
+    for e in range(extra):
+        for idx in itertools.product(*[range(n) for n in shape]):
+            out[(e,) + idx] = (complex_value if complex_ else real_value)(list(idx), seed + e)
+    return out
+
+
+def run_all_ops(plan, kind, shape, extra, x_in, x_out, dt, **kw):
+    """Checks all six operations of a plan against the oracle. x_in / x_out
+    are inputs in the plan's input / output layouts (flat)."""
+    for op in OPS:
+        fwd_like = op in (O.FORWARD, O.ADJ_INVERSE, O.ADJ_BACKWARD)
+        src = x_in if fwd_like else x_out
+        ref = O.transform(kind, src.astype(np.complex128 if np.iscomplexobj(src) else np.float64), op,
+                          shape=shape, extra=extra, **kw)
+        got = _host(plan.execute(op, _tdev(src)))
+        err = rel_l2(got, ref)
+        assert err <= 4 * tol(dt), (shape, extra, kw, op, err)
+
+
+C2C_CASES = [
+    ((3, 4), 1, None),
+    ((3, 2, 5), 6, None),
+    ((2, 1, 3, 4), 2, None),
+    ((2, 3, 2, 3), 1, []),
+    ((2, 3, 2, 3), 1, [0, 3]),
+    ((8, 12, 10), 2, None),
+    ((16, 48, 60), 1, [0, 2]),
+]
+
+
+@pytest.mark.parametrize("permute", [True, False])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("shape,extra,sel", C2C_CASES)
+def test_c2c_plan_matches_oracle(gpu, shape, extra, sel, dt, permute):
+    cd = complex_of(dt)
+    x = fixture_field(shape, extra, 1.0, True).astype(cd)
+    plan = pf.C2cPlan(shape, dtype=torch.from_numpy(x).dtype, extra_shape=(extra,) if extra > 1 else (),
+                      selection=sel, permute_dims=permute)
+    y = O.transform(O.KIND_C2C, x.astype(np.complex128), O.FORWARD, shape=shape, extra=extra, selected=sel,
+                    permute_dims=permute).astype(cd)
+    run_all_ops(plan, O.KIND_C2C, shape, extra, x.reshape(-1), y, dt, selected=sel, permute_dims=permute)
+
+
+def test_c2c_plan_directions(gpu):
+    shape = (4, 6, 10)
+    rng = np.random.default_rng(3)
+    x = uniform_complex(rng, shape, np.complex128)
+    dirs = [False, True, False]
+    plan = pf.C2cPlan(shape, dtype=torch.complex128, directions=dirs)
+    y = O.transform(O.KIND_C2C, x, O.FORWARD, shape=shape, backward_dir=dirs)
+    run_all_ops(plan, O.KIND_C2C, shape, 1, x.reshape(-1), y, np.float64, backward_dir=dirs)
+
+
+R2C_CASES = [
+    ((3, 1), 1, None),
+    ((3, 2), 1, None),
+    ((3, 1, 4), 1, None),
+    ((3, 1, 5), 6, None),
+    ((2, 1, 3, 3), 2, None),
+    ((2, 3, 2, 3), 1, [0]),
+    ((2, 3, 2, 3), 1, [0, 2]),
+    ((2, 3, 2, 3), 1, [0, 3]),
+    ((2, 3, 4), 2, [0, 2]),
+    ((8, 12, 16), 1, None),
+    ((16, 10, 9), 3, None),
+]
+
+
+@pytest.mark.parametrize("permute", [True, False])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("shape,extra,sel", R2C_CASES)
+def test_r2c_plan_matches_oracle(gpu, shape, extra, sel, dt, permute):
+    x = fixture_field(shape, extra, 2.0, False).astype(dt)
+    plan = pf.R2cPlan(shape, dtype=torch.from_numpy(x).dtype, extra_shape=(extra,) if extra > 1 else (),
+                      selection=sel, permute_dims=permute)
+    y = O.transform(O.KIND_R2C, x.astype(np.float64), O.FORWARD, shape=shape, extra=extra, selected=sel,
+                    permute_dims=permute).astype(complex_of(dt))
+    run_all_ops(plan, O.KIND_R2C, shape, extra, x.reshape(-1), y, dt, selected=sel, permute_dims=permute)
+
+
+def test_r2c_inverse_rejects_invalid_spectrum_without_writing(gpu):
+    shape = (4, 6, 8)
+    rng = np.random.default_rng(4)
+    x = uniform_real(rng, shape, np.float64)
+    plan = pf.R2cPlan(shape, dtype=torch.float64)
+    spec = plan.forward(_tdev(x))
+    host = _host(spec).copy()
+    # find the DC plane element (kz = 0) of the output layout [kz][ky][kx] and corrupt it
+    host[3] += 0.5j
+    dst = torch.full((x.size,), -1.0, dtype=torch.float64, device="cuda")
+    with pytest.raises(pf.PfhError) as e:
+        plan.inverse(_tdev(host), dst)
+    assert e.value.variant == "InvalidSpectrum"
+    assert torch.all(dst == -1.0)
+    with pytest.raises(O.InvalidSpectrum):
+        O.transform(O.KIND_R2C, host, O.INVERSE, shape=shape)
+    # the untouched spectrum passes and round-trips
+    back = _host(plan.inverse(spec))
+    assert rel_l2(back, x.reshape(-1)) <= 1e-12
+
+
+R2R_CASES = [
+    ((3, 4), 1, [pf.R2rKind.Dht, pf.R2rKind.Dht]),
+    ((3, 2, 4), 6, [pf.R2rKind.Dht, None, pf.R2rKind.Dht]),
+    ((2, 3, 2, 3), 1, [pf.R2rKind.Dht, None, None, pf.R2rKind.Dht]),
+    ((2, 3, 2, 3), 1, [None, None, None, None]),
+    ((2, 3, 2, 3), 1, [pf.R2rKind.DctI, pf.R2rKind.DctII, pf.R2rKind.DctIII, pf.R2rKind.DctIV]),
+    ((2, 3, 2, 3), 1, [pf.R2rKind.DstI, pf.R2rKind.DstII, pf.R2rKind.DstIII, pf.R2rKind.DstIV]),
+    ((16, 24, 20), 1, [pf.R2rKind.DctII, pf.R2rKind.DstI, pf.R2rKind.DctIV]),
+    ((12, 10, 9), 2, [pf.R2rKind.DstII, pf.R2rKind.DctIII, pf.R2rKind.DstIV]),
+]
+
+
+@pytest.mark.parametrize("permute", [True, False])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("shape,extra,kinds", R2R_CASES)
+def test_r2r_plan_matches_oracle(gpu, shape, extra, kinds, dt, permute):
+    x = fixture_field(shape, extra, 3.0, False).astype(dt)
+    plan = pf.R2rPlan(shape, kinds, dtype=torch.from_numpy(x).dtype, extra_shape=(extra,) if extra > 1 else (),
+                      permute_dims=permute)
+    r2r = [k or 0 for k in kinds]
+    y = O.transform(O.KIND_R2R, x.astype(np.float64), O.FORWARD, shape=shape, extra=extra, r2r=r2r,
+                    permute_dims=permute).astype(dt)
+    run_all_ops(plan, O.KIND_R2R, shape, extra, x.reshape(-1), y, dt, r2r=r2r, permute_dims=permute)
+
+
+def _coded(shape, extra, dtype):
+    """Integer-coded values x*10000 + y*100 + z (+ extra * 1e6)."""
+    idx = np.indices(shape).reshape(len(shape), -1)
+    code = np.zeros(idx.shape[1], dtype=np.int64)
+    for a in range(len(shape)):
+        code = code * 100 + idx[a]
+    vals = np.concatenate([code + e * 1_000_000 for e in range(extra)])
+    return vals.astype(dtype)
+
+
+@pytest.mark.parametrize("plan_cls", [pf.LocalTransposePlan, pf.AllToAllvTransposePlan])
+def test_transpose_all_3d_permutations_bit_exact(gpu, plan_cls):
+    shape, extra = (5, 7, 6), 2
+    perms = list(itertools.permutations(range(3)))
+    base = _coded(shape, extra, np.uint64)
+    for sp in perms:
+        src = O.local_transpose(base, shape, extra, (0, 1, 2), sp)  # logical -> sp memory order
+        for dp in perms:
+            exp = O.local_transpose(src, shape, extra, sp, dp)
+            if plan_cls is pf.AllToAllvTransposePlan:
+                exp2, _ = O.alltoallv_1x1(src, shape, extra, sp, dp)
+                assert np.array_equal(exp, exp2)
+            plan = plan_cls(shape, sp, dp, 8, extra_shape=(extra,))
+            d = torch.empty(src.size, dtype=torch.int64, device="cuda")
+            plan.execute(_tdev(src.view(np.int64)), d)
+            assert np.array_equal(_host(d).view(np.uint64), exp), (sp, dp)
+
+
+@pytest.mark.parametrize("elem", [1, 2, 4, 8, 12, 16, 24])
+def test_transpose_element_sizes_bit_exact(gpu, elem):
+    shape, extra = (3, 4, 5, 6), 1
+    rng = np.random.default_rng(elem)
+    raw = rng.integers(0, 256, size=np.prod(shape) * elem, dtype=np.uint8)
+    for sp, dp in [((0, 1, 2, 3), (3, 2, 1, 0)), ((2, 0, 3, 1), (1, 3, 0, 2)), ((0, 1, 3, 2), (0, 1, 2, 3))]:
+        # bytes of element i are raw[i*elem:(i+1)*elem]
+        src = raw.copy()
+        exp_idx = O.local_transpose(np.arange(np.prod(shape), dtype=np.int64), shape, 1, sp, dp)
+        exp = src.reshape(-1, elem)[exp_idx].reshape(-1)
+        plan = pf.LocalTransposePlan(shape, sp, dp, elem)
+        d = torch.empty(raw.size, dtype=torch.uint8, device="cuda")
+        plan.execute(_tdev(src), d)
+        assert np.array_equal(_host(d), exp), (elem, sp, dp)
+
+
+def test_config1_64cubed_c2c_f64_roundtrip_and_transpose(gpu):
+    """64^3 complex f64, re/im uniform[-1,1) from a seed: forward, normalized
+    inverse, and the LocalTransposePlan z,y,x -> x,z,y permutation."""
+    shape = (64, 64, 64)
+    rng = np.random.default_rng(1)
+    x = uniform_complex(rng, shape, np.complex128)
+    plan = pf.C2cPlan(shape, dtype=torch.complex128)
+    y = plan.forward(_tdev(x))
+    ref = O.transform(O.KIND_C2C, x, O.FORWARD, shape=shape)
+    assert rel_l2(_host(y), ref) <= 1e-12
+    back = _host(plan.inverse(y))
+    assert rel_l2(back, x.reshape(-1)) <= 1e-12
+    tp = pf.LocalTransposePlan(shape, (2, 1, 0), (0, 2, 1), 16)
+    src = _host(y)
+    d = torch.empty_like(y)
+    tp.execute(y, d)
+    exp = O.local_transpose(src.view(np.complex128), shape, 1, (2, 1, 0), (0, 2, 1))
+    assert _host(d).tobytes() == exp.tobytes()
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_in_place_matches_out_of_place(gpu, dt):
+    shape = (8, 12, 16)
+    rng = np.random.default_rng(11)
+    x = uniform_real(rng, shape, dt)
+    plan = pf.R2cPlan(shape, dtype=torch.from_numpy(x).dtype)
+    ref = _host(plan.forward(_tdev(x)))
+    n_out = plan.output_elements()
+    data = torch.zeros(n_out, dtype=torch.from_numpy(ref).dtype, device="cuda")
+    data.view(torch.from_numpy(x).dtype)[: x.size].copy_(_tdev(x).reshape(-1))
+    plan.execute_in_place(O.FORWARD, data)
+    assert np.array_equal(_host(data), ref)
+    cplan = pf.C2cPlan(shape, dtype=torch.from_numpy(ref).dtype)
+    xc = uniform_complex(rng, shape, complex_of(dt))
+    refc = _host(cplan.forward(_tdev(xc)))
+    d = _tdev(xc.reshape(-1))
+    cplan.execute_in_place(O.FORWARD, d)
+    assert np.array_equal(_host(d), refc)
