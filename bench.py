@@ -72,6 +72,83 @@ def cpu_baseline(n: int, threads: int):
     }
 
 
+def batched_vs_plain(torch, pf, reps: int = 5):
+    """96 independent 3D C2C f32 problems, edges drawn per axis from
+    {48,60,64,84,96,112,128,160}, extra dim [3], inputs N(0,1): one
+    autotuned persistent launch per stage replayed from a hipGraph, against
+    one C2cPlan launch sequence per problem."""
+    import numpy as np
+
+    rng = np.random.default_rng(20261202)
+    edges = (48, 60, 64, 84, 96, 112, 128, 160)
+    shapes = [tuple(int(e) for e in rng.choice(edges, size=3)) for _ in range(96)]
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4)
+    srcs = [torch.randn(int(np.prod(s)) * 3, generator=g, dtype=torch.complex64, device="cuda") for s in shapes]
+    dsts = [torch.empty_like(t) for t in srcs]
+    t0 = time.perf_counter()
+    batch = pf.BatchC2cPlan(shapes, autotune=True, extra_shapes=[(3,)] * 96)
+    tune_s = time.perf_counter() - t0
+    bws = batch.allocate_workspace()
+    plans = [pf.C2cPlan(s, extra_shape=(3,)) for s in shapes]
+    pws = [p.allocate_workspace() for p in plans]
+
+    def run_batch():
+        batch.forward(srcs, dsts, bws, use_graph=True)
+
+    def run_plain():
+        for p, a, b, w in zip(plans, srcs, dsts, pws):
+            p.forward(a, b, w)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    tb = timed(run_batch)
+    tp = timed(run_plain)
+    elems = sum(int(np.prod(s)) * 3 for s in shapes)
+    return {
+        "workload": "96 x 3D C2C f32 forward, edges from {48,60,64,84,96,112,128,160}, extra [3]",
+        "batched_ms": tb * 1e3,
+        "plain_ms": tp * 1e3,
+        "speedup": tp / tb,
+        "batched_points_per_s": elems / tb,
+        "tuning": list(batch.tuning()),
+        "autotune_s": tune_s,
+    }
+
+
+def adjoint_vs_plain(torch, pf, plan, x, spec, back, ws, reps: int = 5):
+    """Adjoint round trip (adjoint of the C2R inverse, then adjoint of the R2C
+    forward) against the plain forward + inverse on the same plan."""
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    def plain():
+        plan.forward(x, spec, ws)
+        plan.inverse(spec, back, ws, flags=pf.EXEC_DEFER_STATUS)
+
+    def adjoint():
+        plan.adjoint_inverse(x, spec, ws)
+        plan.adjoint_forward(spec, back, ws)
+
+    tp = timed(plain)
+    ta = timed(adjoint)
+    return {"plain_ms": tp * 1e3, "adjoint_ms": ta * 1e3, "adjoint_over_plain": ta / tp}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +157,7 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-reps", type=int, default=5)
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint comparisons")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (this build has no "
@@ -178,6 +256,10 @@ def main():
         },
         "accuracy": {"round_trip_rel_l2": rt_err},
     }
+    if not args.no_extras:
+        out["adjoint"] = adjoint_vs_plain(torch, pf, plan, x, spec, back, ws)
+        plan.take_status(ws)
+        out["batched"] = batched_vs_plain(torch, pf)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(n, threads)

@@ -244,7 +244,8 @@ pfh_status pfh_workspace_take_status(pfh_plan plan, void *workspace, pfh_stream_
 typedef struct pfh_batch_s *pfh_batch;
 
 /* Builds one persistent launch per stage over `count` independent C2C
- * plans (forward or inverse), all of one precision, uneven shapes allowed.
+ * plans (PFH_C2C descriptors selecting every axis), all of one precision,
+ * uneven shapes allowed; ops PFH_FORWARD, PFH_INVERSE, PFH_BACKWARD.
  * Tiles, radix split and lines per block come from the autotuner when
  * autotune != 0 (measured on the current device), else from the planner's
  * defaults. */
@@ -253,9 +254,15 @@ void pfh_batch_destroy(pfh_batch batch);
 pfh_status pfh_batch_workspace_bytes(pfh_batch batch, size_t *bytes);
 /* srcs[i] / dsts[i]: device buffers of problem i (input / output layout of
  * its plan). use_graph != 0 captures the launches once into a hipGraph for
- * these pointers and replays it. */
+ * these pointers and stream and replays it (a graph asked for on the legacy
+ * null stream runs on a stream of the batch, ordered after the caller's
+ * earlier work and before its later work by events). Calls on one batch must
+ * not overlap. */
 pfh_status pfh_batch_execute(pfh_batch batch, int op, const void *const *srcs, void *const *dsts, void *workspace,
                              size_t workspace_bytes, pfh_stream_t stream, int use_graph);
+/* The tuning the batch runs with (threads per block, largest radix, lines
+ * per tile on column-access stages), as chosen by the autotuner. */
+pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines);
 
 #ifdef __cplusplus
 }

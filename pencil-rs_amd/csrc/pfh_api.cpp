@@ -13,7 +13,9 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -88,12 +90,18 @@ static pfh_status device_info(DeviceInfo &out)
         DeviceInfo d;
         d.device = dev;
         d.cus = prop.multiProcessorCount;
+        // The planner's budget leaves kStaticLds bytes per workgroup for the
+        // kernels' static LDS (the pass plan copy of k_line_stage).
+        constexpr size_t kStaticLds = 256;
         size_t lim = prop.sharedMemPerBlock;
         if (lim < 65536) lim = 65536;
         if (lim > 65536) {
-            if (set_line_stage_lds_limit(lim) != hipSuccess) lim = 65536;
+            if (set_line_stage_lds_limit(lim) != hipSuccess || set_fast_lds_limit(lim) != hipSuccess) {
+                (void)hipGetLastError(); // keep the failed attribute call out of later error checks
+                lim = 65536;
+            }
         }
-        d.lds_limit = lim;
+        d.lds_limit = lim - kStaticLds;
         g_devs[dev] = d;
         g_dev_ready[dev] = 1;
     }
@@ -135,7 +143,15 @@ static bool radix_plan(int m, std::vector<int> &out, int max_radix)
         ++a;
     }
     std::vector<int> p2;
-    if (max_radix < 16) { // 8 / 4 / 2 passes only
+    int odd = r;
+    while (odd % 3 == 0) odd /= 3;
+    while (odd % 5 == 0) odd /= 5;
+    if (max_radix >= 32 && odd == 1 && a >= 5 && a <= 10) {
+        // radix 32 halves the LDS passes of the long power-of-two lines
+        static const std::vector<int> plans[6] = {{32}, {8, 8}, {8, 16}, {16, 16}, {16, 32}, {32, 32}};
+        p2 = plans[a - 5];
+        a = -1;
+    } else if (max_radix < 16) { // 8 / 4 / 2 passes only
         int left = a;
         while (left >= 3) {
             p2.push_back(8);
@@ -287,18 +303,73 @@ static int pow2ceil(int v)
 
 static bool is_generic_radix(int R)
 {
-    return !(R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8 || R == 11 || R == 13 || R == 16);
+    return !(R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8 || R == 11 || R == 13 || R == 16 || R == 32);
 }
 
-static bool is_wide_radix(int R) { return !(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16); }
+static bool is_wide_radix(int R) { return !(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16 || R == 32); }
 
 static bool mode_needs_pre(int mode)
 {
     return !(mode == M_COPY_C || mode == M_COPY_R || mode == M_C2C || mode == M_R2C || mode == M_DHT);
 }
 
+// Picks a compile-time specialized kernel for power-of-two C2C / R2C / C2R
+// lines (k_line_fast); PFH_FAST=0 disables it, PFH_FAST512=<0|1|2> selects
+// the 512-point f32 variant (16x32 with 16 registers, 8x8x8, 16x32 with 32).
+static int env_int(const char *name, int dflt)
+{
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+}
+
+static void choose_fast(Step &st, int tbytes)
+{
+    st.fast = false;
+    static const int enabled = env_int("PFH_FAST", 1);
+    static const int v512 = env_int("PFH_FAST512", 0);
+    if (!enabled) return;
+    if (!(st.mode == M_C2C || st.mode == M_R2C || st.mode == M_C2R)) return;
+    const int m = st.m;
+    if (m < 64 || (m & (m - 1)) != 0) return;
+    int r0 = 0, r1 = 0, r2 = 0, em = 0;
+    if (tbytes == 4) {
+        switch (m) {
+        case 64: r0 = 8; r1 = 8; em = 16; break;
+        case 128: r0 = 8; r1 = 16; em = 16; break;
+        case 256: r0 = 16; r1 = 16; em = 16; break;
+        case 512:
+            if (v512 == 1) { r0 = 8; r1 = 8; r2 = 8; em = 16; }
+            else if (v512 == 2) { r0 = 16; r1 = 32; em = 32; }
+            else { r0 = 16; r1 = 32; em = 16; }
+            break;
+        case 1024: r0 = 32; r1 = 32; em = 32; break;
+        default: return;
+        }
+    } else {
+        switch (m) {
+        case 64: r0 = 8; r1 = 8; em = 8; break;
+        case 128: r0 = 4; r1 = 4; r2 = 8; em = 8; break;
+        case 256: r0 = 4; r1 = 8; r2 = 8; em = 8; break;
+        case 512: r0 = 8; r1 = 8; r2 = 8; em = 8; break;
+        default: return;
+        }
+    }
+    if (r0 == 32) return; // the direct first pass covers radix <= 16
+    if (!fast_plan_available(tbytes, r0, r1, r2, em)) return;
+    st.fast = true;
+    st.fr[0] = r0;
+    st.fr[1] = r1;
+    st.fr[2] = r2;
+    st.fem = em;
+    st.radices.clear();
+    st.radices.push_back(r0);
+    st.radices.push_back(r1);
+    if (r2) st.radices.push_back(r2);
+}
+
 // Fills mode, lengths, radices and this step's tables (appended to tab).
-static pfh_status fill_transform(Step &st, const StageSpec &sp, std::vector<double> &tab, int tbytes)
+static pfh_status fill_transform(Step &st, const StageSpec &sp, std::vector<double> &tab, int tbytes,
+                                 const Tuning &tu)
 {
     const int64_t n = sp.n;
     st.n = (int)n;
@@ -374,8 +445,9 @@ static pfh_status fill_transform(Step &st, const StageSpec &sp, std::vector<doub
     }
     if (sp.kind == SK_R2C && (sp.flags & F_HERM_DOUBLE)) st.flags |= F_HERM_DOUBLE;
     if (sp.kind == SK_C2R && (sp.flags & F_HERM_HALF)) st.flags |= F_HERM_HALF;
-    if (!radix_plan(st.m, st.radices, tbytes == 4 ? 16 : 8))
-        return set_err(PFH_ERR_UNSUPPORTED, "FFT length needs too many passes");
+    const int max_r = tu.max_radix ? tu.max_radix : (tbytes == 4 ? 16 : 8);
+    if (!radix_plan(st.m, st.radices, max_r)) return set_err(PFH_ERR_UNSUPPORTED, "FFT length needs too many passes");
+    if (tu.allow_fast) choose_fast(st, tbytes);
 
     // tables
     st.tw_base = (int64_t)(tab.size() / 2);
@@ -455,9 +527,16 @@ static pfh_status fill_transform(Step &st, const StageSpec &sp, std::vector<doub
     return PFH_OK;
 }
 
+// Tuning override read once from the environment: PFH_COLUMN_LINES sets the
+// lines per tile of stages with a column-access side.
+static int g_column_lines = []() {
+    const char *e = std::getenv("PFH_COLUMN_LINES");
+    return e ? std::atoi(e) : 0;
+}();
+
 // Line grid, access patterns and launch shape of a step whose in/out
 // layouts are set.
-static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes)
+static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const Tuning &tu)
 {
     const int a = st.axis;
     const auto sti = st.in.strides(), sto = st.out.strides();
@@ -500,7 +579,12 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes)
     int lt_col = 1;
     if (st.in_col) lt_col = std::max(lt_col, 128 / in_eb);
     if (st.out_col) lt_col = std::max(lt_col, 128 / out_eb);
-    const int emax = (tbytes == 4) ? 16 : 8;
+    if (g_column_lines > 0) lt_col = g_column_lines;
+    if (tu.col_lines > 0) lt_col = tu.col_lines;
+    bool wide_plan = false;
+    for (int R : st.radices) wide_plan = wide_plan || is_wide_radix(R);
+    const int emax = st.fast ? st.fem : kEmax;
+    const int max_threads = st.fast ? 512 : (wide_plan ? kMaxThreadsWide : kMaxThreads);
     int need = 1;
     for (int R : st.radices) {
         if (is_generic_radix(R)) {
@@ -512,15 +596,19 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes)
     }
     if (mode_needs_pre(st.mode)) need = std::max(need, ceil_div(st.m, emax));
     int tpl = pow2ceil(need);
-    int lt = (st.in_col || st.out_col) ? lt_col : std::max(1, 256 / tpl);
+    const int target = tu.block > 0 ? tu.block : 256;
+    int lt = (st.in_col || st.out_col) ? lt_col : std::max(1, target / tpl);
     while (lt > 1 && (int64_t)(lt / 2) >= nlines) lt /= 2;
     while (lt * tpl < 64) tpl *= 2;
-    while (lt * tpl > kMaxThreads && lt > 1) lt /= 2;
-    if (lt * tpl > kMaxThreads) return set_err(PFH_ERR_UNSUPPORTED, "line too long for one workgroup");
+    const int cap = tu.block_cap > 0 ? std::min(tu.block_cap, max_threads) : max_threads;
+    while (lt * tpl > cap && lt > 1) lt /= 2;
+    if (lt * tpl > cap) return set_err(PFH_ERR_UNSUPPORTED, "line too long for one workgroup");
     int need_c = std::max(st.m, 1);
     need_c = std::max(need_c, st.in_cplx ? st.nin : (st.nin + 1) / 2);
     need_c = std::max(need_c, st.out_cplx ? st.nout : (st.nout + 1) / 2);
-    st.ls = (need_c % 2 == 0) ? need_c + 1 : need_c;
+    // one pad slot per 16 values inside a line (see lp() in the kernels)
+    const int padded = need_c + (need_c >> 4) + 1;
+    st.ls = (padded % 2 == 0) ? padded + 1 : padded;
     auto lds_of = [&](int l) { return (size_t)(st.twn + (int64_t)l * st.ls) * (size_t)cb; };
     while (lds_of(lt) > dev.lds_limit && lt > 1) {
         lt /= 2;
@@ -532,7 +620,7 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes)
     st.threads = lt * tpl;
     st.lds_bytes = lds_of(lt);
     st.tiles = (nlines + lt - 1) / lt;
-    if (st.tiles >= 64) {
+    if (st.tiles >= 64 && tu.allow_xcd) {
         st.flags |= F_XCD_REMAP;
         st.xcd_per = (int)((st.tiles + 7) / 8);
         st.grid = 8LL * st.xcd_per;
@@ -818,13 +906,13 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
     out.steps.clear();
     for (int e = 0; e < K; ++e) {
         Step s;
-        st = fill_transform(s, specs[e], P.host_tables, tb);
+        st = fill_transform(s, specs[e], P.host_tables, tb, P.tuning);
         if (st) return st;
         s.in = X[e];
         s.out = Y[e];
         s.in_buf = ib[e];
         s.out_buf = ob[e];
-        st = finish_step(s, P.dev, tb);
+        st = finish_step(s, P.dev, tb, P.tuning);
         if (st) return st;
         if (P.kind == PFH_R2C && specs[e].kind == SK_C2R && (op == PFH_INVERSE || op == PFH_BACKWARD)) {
             // acceptance check on the C2R stage's input (distributed/r2c.rs:4709-4807)
@@ -922,13 +1010,13 @@ static pfh_status build_local_op(PlanImpl &P, int op, OpPlan &out)
     }
     std::array<int, 8> id{};
     Step s;
-    pfh_status st = fill_transform(s, sp, P.host_tables, tb);
+    pfh_status st = fill_transform(s, sp, P.host_tables, tb, P.tuning);
     if (st) return st;
     s.in = make_layout(1, id, shin, 1, (sp.in_c ? 2 : 1) * tb);
     s.out = make_layout(1, id, shout, 1, (sp.out_c ? 2 : 1) * tb);
     s.in_buf = BUF_SRC;
     s.out_buf = BUF_DST;
-    st = finish_step(s, P.dev, tb);
+    st = finish_step(s, P.dev, tb, P.tuning);
     if (st) return st;
     if (P.kind == PFH_LOCAL_R2C && !fwd && op != PFH_ADJOINT_FORWARD) s.local_endpoint_check = true;
     out.steps.assign(1, s);
@@ -1059,14 +1147,14 @@ static pfh_status create_impl(const pfh_desc *d, PlanImpl &P)
                 spc.axis = b.perm[N - 1];
                 spc.n = b.shape[spc.axis];
                 spc.in_c = spc.out_c = (w != 4);
-                fill_transform(s, spc, P.host_tables, (w == 16) ? 8 : 4);
+                fill_transform(s, spc, P.host_tables, (w == 16) ? 8 : 4, P.tuning);
                 s.in = a;
                 s.out = b;
                 const int tb = (w == 16) ? 8 : 4;
                 s.in.elem = s.out.elem = w;
                 s.in_buf = BUF_SRC;
                 s.out_buf = BUF_DST;
-                st = finish_step(s, P.dev, tb);
+                st = finish_step(s, P.dev, tb, P.tuning);
                 if (st) return st;
                 o.steps.push_back(s);
             }
@@ -1111,21 +1199,13 @@ static pfh_status view_workspace(const PlanImpl &P, const OpPlan &o, void *ws, s
 }
 
 template <typename T>
-static pfh_status launch_step(const PlanImpl &P, const Step &s, void *const bufs[4], const WsView &wv,
-                              hipStream_t stream)
+static StageArgs<T> fill_args(const PlanImpl &P, const Step &s, void *const bufs[4], const int *gate)
 {
-    if (s.endpoint_check) {
-        PFH_HIP_CHECK(launch_endpoint_check<T>(s.ep, bufs[s.in_buf], wv.ep, wv.gate, wv.status, stream));
-    }
-    if (s.local_endpoint_check) {
-        PFH_HIP_CHECK(launch_local_endpoint_check<T>(bufs[s.in_buf], s.nlines, s.nin, (s.n % 2 == 0) ? 1 : 0,
-                                                     wv.first_code, wv.gate, wv.status, stream));
-    }
     StageArgs<T> a{};
     a.src = bufs[s.in_buf];
     a.dst = bufs[s.out_buf];
     a.tw = reinterpret_cast<const cpx<T> *>(P.dev_tables) + s.tw_base;
-    a.gate = (s.endpoint_check || s.local_endpoint_check || s.gated) ? wv.gate : nullptr;
+    a.gate = gate;
     a.nlines = s.nlines;
     a.ntiles = s.tiles;
     a.g = s.g;
@@ -1158,6 +1238,26 @@ static pfh_status launch_step(const PlanImpl &P, const Step &s, void *const bufs
     a.prew1 = (T)s.prew1;
     a.postw0 = (T)s.postw0;
     a.postw1 = (T)s.postw1;
+    return a;
+}
+
+template <typename T>
+static pfh_status launch_step(const PlanImpl &P, const Step &s, void *const bufs[4], const WsView &wv,
+                              hipStream_t stream)
+{
+    if (s.endpoint_check) {
+        PFH_HIP_CHECK(launch_endpoint_check<T>(s.ep, bufs[s.in_buf], wv.ep, wv.gate, wv.status, stream));
+    }
+    if (s.local_endpoint_check) {
+        PFH_HIP_CHECK(launch_local_endpoint_check<T>(bufs[s.in_buf], s.nlines, s.nin, (s.n % 2 == 0) ? 1 : 0,
+                                                     wv.first_code, wv.gate, wv.status, stream));
+    }
+    const StageArgs<T> a =
+        fill_args<T>(P, s, bufs, (s.endpoint_check || s.local_endpoint_check || s.gated) ? wv.gate : nullptr);
+    if (s.fast) {
+        PFH_HIP_CHECK(launch_line_fast<T>(a, s.fr[0], s.fr[1], s.fr[2], s.fem, s.threads, s.lds_bytes, s.grid, stream));
+        return PFH_OK;
+    }
     bool wide = false;
     for (int R : s.radices) wide = wide || is_wide_radix(R);
     PFH_HIP_CHECK(launch_line_stage<T>(a, wide, s.threads, s.lds_bytes, s.grid, stream));
@@ -1524,7 +1624,7 @@ pfh_status pfh_local_execute(pfh_plan plan, int op, const void *src, size_t src_
     s.in.extra = s.out.extra = batch;
     s.flags &= ~F_XCD_REMAP;
     const int tb = (P.precision == PFH_F32) ? 4 : 8;
-    pfh_status st = finish_step(s, P.dev, tb);
+    pfh_status st = finish_step(s, P.dev, tb, P.tuning);
     if (st) return st;
     void *bufs[4] = {const_cast<void *>(src), dst, nullptr, nullptr};
     st = run_op(P, o, bufs, wv, stream, nullptr);
@@ -1578,4 +1678,338 @@ pfh_status pfh_workspace_take_status(pfh_plan plan, void *workspace, pfh_stream_
 
 } // extern "C"
 
-// Batched uneven transforms: see pfh_batch.cpp.
+// ---------------------------------------------------------------------------
+// Batched uneven-size transforms. Each problem keeps its own plan (tables,
+// route, buffer assignment). Route stage s of every problem becomes one item
+// of a work list; one persistent launch per stage walks all (item, tile)
+// pairs, so a batch is a handful of launches instead of one per stage and
+// problem. The launches can be captured once into a hipGraph and replayed.
+// ---------------------------------------------------------------------------
+
+struct BatchStage {
+    void *d_items = nullptr;     // StageArgs<T>[nitems]
+    int64_t *d_prefix = nullptr; // [nitems + 1]
+    int nitems = 0;
+    int64_t total = 0;
+    size_t lds = 0; // largest (table + lines) footprint of an item
+    int64_t grid = 0;
+};
+
+struct pfh_batch_s {
+    int count = 0;
+    int precision = PFH_F32;
+    int threads = 256;
+    std::vector<std::unique_ptr<PlanImpl>> plans;
+    Tuning tuning;
+    std::vector<int64_t> ws_off; // per problem
+    int64_t ws_bytes = 0;
+    std::vector<BatchStage> stages;
+    // pointers the uploaded work list was built for
+    int cur_op = -1;
+    std::vector<const void *> cur_src;
+    std::vector<void *> cur_dst;
+    void *cur_ws = nullptr;
+    hipGraphExec_t graph = nullptr;
+    hipStream_t graph_stream = nullptr;
+    // graphs requested on the legacy null stream run on this stream, ordered
+    // after / before the caller's work by events
+    hipStream_t own_stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    ~pfh_batch_s() { release(); }
+    void release_stages()
+    {
+        for (auto &s : stages) {
+            if (s.d_items) (void)hipFree(s.d_items);
+            if (s.d_prefix) (void)hipFree(s.d_prefix);
+        }
+        stages.clear();
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        cur_op = -1;
+    }
+    void release()
+    {
+        release_stages();
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        if (ev_out) (void)hipEventDestroy(ev_out);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
+        ev_in = ev_out = nullptr;
+        own_stream = nullptr;
+    }
+};
+
+static pfh_status batch_plan(pfh_batch_s &B, const pfh_desc *descs, int count, const Tuning &tu)
+{
+    B.plans.clear();
+    B.release_stages();
+    B.tuning = tu;
+    B.ws_off.assign(count, 0);
+    int64_t off = 0;
+    int threads = 64;
+    for (int i = 0; i < count; ++i) {
+        if (descs[i].kind != PFH_C2C) return set_err(PFH_ERR_INVALID_ARGUMENT, "batch problems must be C2C plans");
+        if (descs[i].precision != descs[0].precision) return set_err(PFH_ERR_INVALID_ARGUMENT, "mixed precision");
+        std::unique_ptr<PlanImpl> P(new (std::nothrow) PlanImpl);
+        if (!P) return set_err(PFH_ERR_ALLOCATION_FAILED, "batch plan");
+        P->tuning = tu;
+        pfh_status st = create_impl(&descs[i], *P);
+        if (st) return st;
+        int64_t need = 0;
+        for (int op : {PFH_FORWARD, PFH_INVERSE, PFH_BACKWARD}) {
+            const OpPlan &o = P->ops[op];
+            for (const Step &s : o.steps) {
+                if (s.mode != M_C2C || s.endpoint_check || s.local_endpoint_check || s.fast)
+                    return set_err(PFH_ERR_UNSUPPORTED, "batched problems must select every axis");
+                threads = std::max(threads, s.threads);
+            }
+            need = std::max(need, status_area_bytes(*P) + align256(o.w_bytes[0]) + align256(o.w_bytes[1]));
+        }
+        B.ws_off[i] = off;
+        off += align256(need);
+        B.plans.push_back(std::move(P));
+    }
+    B.ws_bytes = off;
+    B.count = count;
+    B.precision = descs[0].precision;
+    B.threads = threads;
+    return PFH_OK;
+}
+
+template <typename T>
+static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, void *const *dsts, void *ws,
+                               hipStream_t stream)
+{
+    B.release_stages();
+    int nst = 0;
+    for (auto &P : B.plans) nst = std::max(nst, (int)P->ops[op].steps.size());
+    B.stages.resize(nst);
+    for (int s = 0; s < nst; ++s) {
+        std::vector<StageArgs<T>> items;
+        std::vector<int64_t> prefix(1, 0);
+        BatchStage &bs = B.stages[s];
+        for (int i = 0; i < B.count; ++i) {
+            const PlanImpl &P = *B.plans[i];
+            const OpPlan &o = P.ops[op];
+            if (s >= (int)o.steps.size()) continue;
+            const Step &st = o.steps[s];
+            WsView wv;
+            pfh_status e = view_workspace(P, o, static_cast<char *>(ws) + B.ws_off[i], (size_t)(B.ws_bytes), wv);
+            if (e) return e;
+            void *bufs[4] = {const_cast<void *>(srcs[i]), dsts[i], wv.w[0], wv.w[1]};
+            items.push_back(fill_args<T>(P, st, bufs, nullptr));
+            prefix.push_back(prefix.back() + st.tiles);
+            bs.lds = std::max(bs.lds, st.lds_bytes);
+        }
+        bs.nitems = (int)items.size();
+        bs.total = prefix.back();
+        PFH_HIP_CHECK(hipMalloc(&bs.d_items, std::max<size_t>(1, items.size()) * sizeof(StageArgs<T>)));
+        PFH_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&bs.d_prefix), prefix.size() * sizeof(int64_t)));
+        PFH_HIP_CHECK(hipMemcpy(bs.d_items, items.data(), items.size() * sizeof(StageArgs<T>), hipMemcpyHostToDevice));
+        PFH_HIP_CHECK(hipMemcpy(bs.d_prefix, prefix.data(), prefix.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+        const int per_cu = line_batch_blocks_per_cu<T>(B.threads, bs.lds);
+        const int cus = B.plans.empty() ? 256 : B.plans[0]->dev.cus;
+        bs.grid = std::min<int64_t>(bs.total, (int64_t)per_cu * cus);
+    }
+    (void)stream;
+    B.cur_op = op;
+    B.cur_src.assign(srcs, srcs + B.count);
+    B.cur_dst.assign(dsts, dsts + B.count);
+    B.cur_ws = ws;
+    return PFH_OK;
+}
+
+template <typename T>
+static pfh_status batch_launch(pfh_batch_s &B, hipStream_t stream)
+{
+    for (const BatchStage &bs : B.stages)
+        PFH_HIP_CHECK(launch_line_batch<T>(reinterpret_cast<const StageArgs<T> *>(bs.d_items), bs.d_prefix, bs.nitems,
+                                           bs.total, B.threads, bs.lds, bs.grid, stream));
+    return PFH_OK;
+}
+
+static pfh_status batch_run(pfh_batch_s &B, int op, const void *const *srcs, void *const *dsts, void *ws,
+                            hipStream_t stream, int use_graph)
+{
+    bool same = B.cur_op == op && B.cur_ws == ws;
+    for (int i = 0; same && i < B.count; ++i) same = B.cur_src[i] == srcs[i] && B.cur_dst[i] == dsts[i];
+    if (!same) {
+        pfh_status st = B.precision == PFH_F32 ? batch_upload<float>(B, op, srcs, dsts, ws, stream)
+                                               : batch_upload<double>(B, op, srcs, dsts, ws, stream);
+        if (st) return st;
+    }
+    if (!use_graph) return B.precision == PFH_F32 ? batch_launch<float>(B, stream) : batch_launch<double>(B, stream);
+    // the legacy null stream cannot be captured: hop to a stream of our own
+    const bool hop = stream == nullptr;
+    hipStream_t caller = stream;
+    if (hop) {
+        if (!B.own_stream) {
+            PFH_HIP_CHECK(hipStreamCreateWithFlags(&B.own_stream, hipStreamNonBlocking));
+            PFH_HIP_CHECK(hipEventCreateWithFlags(&B.ev_in, hipEventDisableTiming));
+            PFH_HIP_CHECK(hipEventCreateWithFlags(&B.ev_out, hipEventDisableTiming));
+        }
+        PFH_HIP_CHECK(hipEventRecord(B.ev_in, caller));
+        PFH_HIP_CHECK(hipStreamWaitEvent(B.own_stream, B.ev_in, 0));
+        stream = B.own_stream;
+    }
+    if (!B.graph || B.graph_stream != stream) {
+        if (B.graph) (void)hipGraphExecDestroy(B.graph);
+        B.graph = nullptr;
+        hipGraph_t g = nullptr;
+        PFH_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        pfh_status st = B.precision == PFH_F32 ? batch_launch<float>(B, stream) : batch_launch<double>(B, stream);
+        hipError_t ce = hipStreamEndCapture(stream, &g);
+        if (st) return st;
+        if (ce != hipSuccess) return set_err(PFH_ERR_DEVICE, hipGetErrorString(ce), 0, 0, 0, 0, (int)ce);
+        hipError_t ie = hipGraphInstantiate(&B.graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return set_err(PFH_ERR_DEVICE, hipGetErrorString(ie), 0, 0, 0, 0, (int)ie);
+        B.graph_stream = stream;
+    }
+    PFH_HIP_CHECK(hipGraphLaunch(B.graph, stream));
+    if (hop) {
+        PFH_HIP_CHECK(hipEventRecord(B.ev_out, stream));
+        PFH_HIP_CHECK(hipStreamWaitEvent(caller, B.ev_out, 0));
+    }
+    return PFH_OK;
+}
+
+// Measured autotuning: every candidate tuning re-plans the whole batch and
+// times a forward execution (device events); the fastest is kept.
+static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int count)
+{
+    std::vector<Tuning> cands;
+    for (int block : {256, 512})
+        for (int maxr : {8, 16})
+            for (int col : {8, 16}) {
+                Tuning t;
+                t.block = block;
+                t.max_radix = maxr;
+                t.col_lines = col;
+                t.block_cap = kMaxThreadsWide;
+                t.allow_fast = false;
+                t.allow_xcd = false;
+                cands.push_back(t);
+            }
+    // scratch buffers for the measurement
+    std::vector<void *> bufs;
+    std::vector<const void *> srcs(count);
+    std::vector<void *> dsts(count);
+    auto cleanup = [&]() {
+        for (void *p : bufs) (void)hipFree(p);
+        bufs.clear();
+    };
+    int64_t maxws = 0;
+    {
+        pfh_status st = batch_plan(B, descs, count, cands[0]);
+        if (st) return st;
+    }
+    for (int i = 0; i < count; ++i) {
+        const Layout &L = B.plans[i]->ops[PFH_FORWARD].in;
+        void *a = nullptr, *b = nullptr;
+        if (hipMalloc(&a, (size_t)L.bytes()) != hipSuccess || hipMalloc(&b, (size_t)L.bytes()) != hipSuccess) {
+            cleanup();
+            return set_err(PFH_ERR_ALLOCATION_FAILED, "autotune buffers");
+        }
+        (void)hipMemset(a, 0, (size_t)L.bytes());
+        bufs.push_back(a);
+        bufs.push_back(b);
+        srcs[i] = a;
+        dsts[i] = b;
+    }
+    hipStream_t stream = nullptr;
+    if (hipStreamCreate(&stream) != hipSuccess) {
+        cleanup();
+        return set_err(PFH_ERR_DEVICE, "stream");
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    double best = 1e30;
+    Tuning best_t = cands[0];
+    void *ws = nullptr;
+    for (const Tuning &t : cands) {
+        pfh_status st = batch_plan(B, descs, count, t);
+        if (st) continue;
+        if (B.ws_bytes > maxws) {
+            if (ws) (void)hipFree(ws);
+            ws = nullptr;
+            if (hipMalloc(&ws, (size_t)B.ws_bytes) != hipSuccess) break;
+            maxws = B.ws_bytes;
+        }
+        if (batch_run(B, PFH_FORWARD, srcs.data(), dsts.data(), ws, stream, 0)) continue;
+        (void)hipEventRecord(e0, stream);
+        for (int r = 0; r < 3; ++r) (void)batch_run(B, PFH_FORWARD, srcs.data(), dsts.data(), ws, stream, 0);
+        (void)hipEventRecord(e1, stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+            best = ms;
+            best_t = t;
+        }
+    }
+    if (ws) (void)hipFree(ws);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(stream);
+    cleanup();
+    return batch_plan(B, descs, count, best_t);
+}
+
+extern "C" {
+
+pfh_status pfh_batch_create(const pfh_desc *descs, int count, int autotune, pfh_batch *out)
+{
+    if (!descs || !out || count <= 0) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    *out = nullptr;
+    pfh_batch b = new (std::nothrow) pfh_batch_s;
+    if (!b) return set_err(PFH_ERR_ALLOCATION_FAILED, "batch");
+    Tuning t;
+    t.allow_fast = false;
+    t.allow_xcd = false;
+    t.block = 256;
+    t.block_cap = kMaxThreadsWide;
+    pfh_status st = autotune ? batch_autotune(*b, descs, count) : batch_plan(*b, descs, count, t);
+    if (st) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return ok();
+}
+
+void pfh_batch_destroy(pfh_batch batch) { delete batch; }
+
+pfh_status pfh_batch_workspace_bytes(pfh_batch batch, size_t *bytes)
+{
+    if (!batch || !bytes) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    *bytes = (size_t)std::max<int64_t>(batch->ws_bytes, 256);
+    return ok();
+}
+
+pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines)
+{
+    if (!batch) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    if (block) *block = batch->tuning.block;
+    if (max_radix) *max_radix = batch->tuning.max_radix;
+    if (col_lines) *col_lines = batch->tuning.col_lines;
+    return ok();
+}
+
+pfh_status pfh_batch_execute(pfh_batch batch, int op, const void *const *srcs, void *const *dsts, void *workspace,
+                             size_t workspace_bytes, pfh_stream_t stream, int use_graph)
+{
+    if (!batch || !srcs || !dsts) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    if (op != PFH_FORWARD && op != PFH_INVERSE && op != PFH_BACKWARD)
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "batched op must be forward, inverse or backward");
+    if (!workspace || (int64_t)workspace_bytes < batch->ws_bytes)
+        return set_err(PFH_ERR_WORKSPACE_TOO_SMALL, "batch workspace", (size_t)batch->ws_bytes,
+                       workspace ? workspace_bytes : 0);
+    for (int i = 0; i < batch->count; ++i)
+        if (!srcs[i] || !dsts[i]) return set_err(PFH_ERR_INVALID_ARGUMENT, "null problem buffer");
+    pfh_status st = batch_run(*batch, op, srcs, dsts, workspace, reinterpret_cast<hipStream_t>(stream), use_graph);
+    if (st) return st;
+    return ok();
+}
+
+} // extern "C"

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "pfh_kernels.hpp"
 
@@ -25,17 +26,23 @@ namespace pfh {
 
 namespace {
 
+// Offsets of a line's first input and output element. The digit loop is
+// unrolled over the compile-time maximum so that the argument struct is never
+// indexed dynamically (which would force a private copy of it).
 __device__ __forceinline__ void line_offsets(const LineGeom &g, int64_t line, int64_t &io, int64_t &oo)
 {
     io = 0;
     oo = 0;
-    for (int d = 0; d < g.nd; ++d) {
-        const int64_t e = g.ext[d];
-        const int64_t q = line / e;
-        const int64_t r = line - q * e;
-        io += r * g.is[d];
-        oo += r * g.os[d];
-        line = q;
+#pragma unroll
+    for (int d = 0; d < kMaxDigits; ++d) {
+        if (d < g.nd) {
+            const int64_t e = g.ext[d];
+            const int64_t q = line / e;
+            const int64_t r = line - q * e;
+            io += r * g.is[d];
+            oo += r * g.os[d];
+            line = q;
+        }
     }
 }
 
@@ -54,13 +61,39 @@ __device__ __forceinline__ bool needs_pre(int mode)
     return !(mode == M_COPY_C || mode == M_COPY_R || mode == M_C2C || mode == M_R2C || mode == M_DHT);
 }
 
+// An LDS line keeps one pad slot after every 16 complex values, so that the
+// stride-R writes of a Stockham pass (R = 8, 16, 32) hit distinct banks.
+__device__ __forceinline__ int lp(int i) { return i + (i >> 4); }
+
+template <typename T>
+__device__ __forceinline__ cpx<T> L(const cpx<T> *ln, int i)
+{
+    return ln[lp(i)];
+}
+template <typename T>
+__device__ __forceinline__ void Lset(cpx<T> *ln, int i, cpx<T> v)
+{
+    ln[lp(i)] = v;
+}
+// real view of a line: real p lives in complex slot p / 2
+template <typename T>
+__device__ __forceinline__ T LR(const cpx<T> *ln, int p)
+{
+    return reinterpret_cast<const T *>(ln)[2 * lp(p >> 1) + (p & 1)];
+}
+template <typename T>
+__device__ __forceinline__ void LRset(cpx<T> *ln, int p, T v)
+{
+    reinterpret_cast<T *>(ln)[2 * lp(p >> 1) + (p & 1)] = v;
+}
+
 // One input value of a real line after the reversal / alternating sign /
 // adjoint edge weights requested by the stage.
 template <typename T>
-__device__ __forceinline__ T xin(const StageArgs<T> &a, const T *lr, int i)
+__device__ __forceinline__ T xin(const StageArgs<T> &a, const cpx<T> *ln, int i)
 {
     const int idx = (a.flags & F_PRE_REV) ? a.n - 1 - i : i;
-    T v = lr[idx];
+    T v = LR(ln, idx);
     if ((a.flags & F_PRE_ALT) && (idx & 1)) v = -v;
     if (a.flags & F_WEIGHTS) {
         if (idx == 0) v = v * a.prew0;
@@ -74,21 +107,20 @@ __device__ __forceinline__ T xin(const StageArgs<T> &a, const T *lr, int i)
 template <typename T>
 __device__ __forceinline__ cpx<T> hin(const StageArgs<T> &a, const cpx<T> *ln, int k, int last)
 {
-    cpx<T> v = ln[k];
+    cpx<T> v = L(ln, k);
     if (k == 0 || k == last) v.y = (T)0;
     if ((a.flags & F_HERM_HALF) && k > 0 && k < last) v = scal(v, (T)0.5);
     return v;
 }
 
-template <typename T>
-__device__ cpx<T> pre_value(const StageArgs<T> &a, const cpx<T> *ln, const cpx<T> *tws, int j)
+template <typename T, int MG>
+__device__ __forceinline__ cpx<T> pre_value(const StageArgs<T> &a, const cpx<T> *ln, const cpx<T> *tws, int j)
 {
-    const T *lr = reinterpret_cast<const T *>(ln);
     const int n = a.n;
+    if constexpr (MG == 1) {
     switch (a.mode) {
     case M_R2C_ODD:
-    case M_DHT_ODD:
-        return mk(xin(a, lr, j), (T)0);
+        return mk(xin(a, ln, j), (T)0);
     case M_C2R: {
         const int m = a.m; // n / 2; Nyquist index m
         cpx<T> A = hin(a, ln, j, m);
@@ -103,24 +135,31 @@ __device__ cpx<T> pre_value(const StageArgs<T> &a, const cpx<T> *ln, const cpx<T
         if (n == 1) v.y = (T)0;
         return conjg(v);
     }
+    default:
+        return L(ln, j);
+    }
+    } else {
+    switch (a.mode) {
+    case M_DHT_ODD:
+        return mk(xin(a, ln, j), (T)0);
     case M_DCT2: {
         const int h = n / 2;
         const int i0 = 2 * j, i1 = 2 * j + 1;
-        T v0 = i0 < h ? xin(a, lr, 2 * i0) : xin(a, lr, 2 * (n - 1 - i0) + 1);
-        T v1 = i1 < h ? xin(a, lr, 2 * i1) : xin(a, lr, 2 * (n - 1 - i1) + 1);
+        T v0 = i0 < h ? xin(a, ln, 2 * i0) : xin(a, ln, 2 * (n - 1 - i0) + 1);
+        T v1 = i1 < h ? xin(a, ln, 2 * i1) : xin(a, ln, 2 * (n - 1 - i1) + 1);
         return mk(v0, v1);
     }
     case M_DCT2_ODD: {
         const int h = (n + 1) / 2;
-        T v = j < h ? xin(a, lr, 2 * j) : xin(a, lr, 2 * (n - 1 - j) + 1);
+        T v = j < h ? xin(a, ln, 2 * j) : xin(a, ln, 2 * (n - 1 - j) + 1);
         return mk(v, (T)0);
     }
     case M_DCT3: {
         const int m = a.m;
         // V_k = exp(i pi k / 2n) (y_k - i y_{n-k}), y_n = 0
         const int ka = j, kb = m - j;
-        cpx<T> Va = cmul(tws[a.pre0 + ka], mk(xin(a, lr, ka), ka == 0 ? (T)0 : -xin(a, lr, n - ka)));
-        cpx<T> Vb = cmul(tws[a.pre0 + kb], mk(xin(a, lr, kb), kb == 0 ? (T)0 : -xin(a, lr, n - kb)));
+        cpx<T> Va = cmul(tws[a.pre0 + ka], mk(xin(a, ln, ka), ka == 0 ? (T)0 : -xin(a, ln, n - ka)));
+        cpx<T> Vb = cmul(tws[a.pre0 + kb], mk(xin(a, ln, kb), kb == 0 ? (T)0 : -xin(a, ln, n - kb)));
         if (ka == 0) Va.y = (T)0;
         if (kb == m) Vb.y = (T)0;
         cpx<T> B = conjg(Vb);
@@ -129,31 +168,32 @@ __device__ cpx<T> pre_value(const StageArgs<T> &a, const cpx<T> *ln, const cpx<T
         return conjg(E + mul_pi(O));
     }
     case M_DCT3_ODD: {
-        T y = xin(a, lr, j);
+        T y = xin(a, ln, j);
         if (j == 0) y = y * (T)0.5;
         return scal(tws[a.pre0 + j], y);
     }
     case M_DCT4:
-        return cmul(mk(xin(a, lr, 2 * j), xin(a, lr, n - 1 - 2 * j)), tws[a.pre0 + j]);
+        return cmul(mk(xin(a, ln, 2 * j), xin(a, ln, n - 1 - 2 * j)), tws[a.pre0 + j]);
     case M_DCT4_ODD:
-        return j < n ? scal(tws[a.pre0 + j], xin(a, lr, j)) : mk((T)0, (T)0);
+        return j < n ? scal(tws[a.pre0 + j], xin(a, ln, j)) : mk((T)0, (T)0);
     case M_DCT1: {
-        const int L = 2 * (n - 1);
+        const int Lx = 2 * (n - 1);
         const int i0 = 2 * j, i1 = 2 * j + 1;
-        T v0 = i0 <= n - 1 ? xin(a, lr, i0) : xin(a, lr, L - i0);
-        T v1 = i1 <= n - 1 ? xin(a, lr, i1) : xin(a, lr, L - i1);
+        T v0 = i0 <= n - 1 ? xin(a, ln, i0) : xin(a, ln, Lx - i0);
+        T v1 = i1 <= n - 1 ? xin(a, ln, i1) : xin(a, ln, Lx - i1);
         return mk(v0, v1);
     }
     case M_DST1: {
         const int i0 = 2 * j, i1 = 2 * j + 1;
         auto o = [&](int i) -> T {
             if (i == 0 || i == n + 1) return (T)0;
-            return i <= n ? xin(a, lr, i - 1) : -xin(a, lr, 2 * n + 1 - i);
+            return i <= n ? xin(a, ln, i - 1) : -xin(a, ln, 2 * n + 1 - i);
         };
         return mk(o(i0), o(i1));
     }
     default:
-        return ln[j];
+        return L(ln, j);
+    }
     }
 }
 
@@ -163,43 +203,39 @@ __device__ __forceinline__ cpx<T> split(const cpx<T> *Z, const cpx<T> *w, int m,
 {
     const int ka = (k == m) ? 0 : k;
     const int kb = (k == 0) ? 0 : m - k;
-    cpx<T> A = Z[ka], B = conjg(Z[kb]);
+    cpx<T> A = L(Z, ka), B = conjg(L(Z, kb));
     cpx<T> E = scal(A + B, (T)0.5), D = scal(A - B, (T)0.5);
     return E + mul_mi(cmul(w[k], D));
 }
 
+// Real value i of the (optionally conjugated) complex result viewed as reals.
 template <typename T>
 __device__ __forceinline__ T t_real_view(const cpx<T> *Z, int i, bool conj_result)
 {
-    cpx<T> v = Z[i >> 1];
+    cpx<T> v = L(Z, i >> 1);
     if (conj_result) v.y = -v.y;
     return (i & 1) ? v.y : v.x;
 }
 
 // Real output y_k of a real-output mode.
-template <typename T>
-__device__ T post_real(const StageArgs<T> &a, const cpx<T> *Z, const cpx<T> *tws, int k)
+template <typename T, int MG>
+__device__ __forceinline__ T post_real(const StageArgs<T> &a, const cpx<T> *Z, const cpx<T> *tws, int k)
 {
     const int n = a.n, m = a.m;
     const int kk = (a.flags & F_POST_REV) ? n - 1 - k : k;
-    T y;
+    T y = (T)0;
+    if constexpr (MG == 1) {
+        if (a.mode == M_C2R) y = t_real_view(Z, kk, true);
+        else y = L(Z, kk).x; // M_C2R_ODD
+    } else {
     switch (a.mode) {
-    case M_COPY_R:
-        y = reinterpret_cast<const T *>(Z)[kk];
-        break;
-    case M_C2R:
-        y = t_real_view(Z, kk, true);
-        break;
-    case M_C2R_ODD:
-        y = Z[kk].x;
-        break;
     case M_DCT2: {
         cpx<T> V = kk <= m ? split(Z, tws + a.post0, m, kk) : conjg(split(Z, tws + a.post0, m, n - kk));
         y = (T)2 * cmul(tws[a.post1 + kk], V).x;
         break;
     }
     case M_DCT2_ODD:
-        y = (T)2 * cmul(tws[a.post0 + kk], Z[kk]).x;
+        y = (T)2 * cmul(tws[a.post0 + kk], L(Z, kk)).x;
         break;
     case M_DCT3: {
         const int i = (kk & 1) ? n - 1 - (kk >> 1) : (kk >> 1);
@@ -208,21 +244,21 @@ __device__ T post_real(const StageArgs<T> &a, const cpx<T> *Z, const cpx<T> *tws
     }
     case M_DCT3_ODD: {
         const int i = (kk & 1) ? n - 1 - (kk >> 1) : (kk >> 1);
-        y = (T)2 * Z[i].x;
+        y = (T)2 * L(Z, i).x;
         break;
     }
     case M_DCT4: {
         if ((kk & 1) == 0) {
             const int q = kk >> 1;
-            y = (T)2 * cmul(Z[q], tws[a.post0 + q]).x;
+            y = (T)2 * cmul(L(Z, q), tws[a.post0 + q]).x;
         } else {
             const int q = (n - 1 - kk) >> 1;
-            y = (T)-2 * cmul(Z[q], tws[a.post0 + q]).y;
+            y = (T)-2 * cmul(L(Z, q), tws[a.post0 + q]).y;
         }
         break;
     }
     case M_DCT4_ODD:
-        y = (T)2 * cmul(tws[a.post0 + kk], Z[kk]).x;
+        y = (T)2 * cmul(tws[a.post0 + kk], L(Z, kk)).x;
         break;
     case M_DCT1:
         y = split(Z, tws + a.post0, m, kk).x;
@@ -240,11 +276,14 @@ __device__ T post_real(const StageArgs<T> &a, const cpx<T> *Z, const cpx<T> *tws
         }
         break;
     }
-    case M_DHT_ODD:
-        y = Z[kk].x - Z[kk].y;
+    case M_DHT_ODD: {
+        const cpx<T> z = L(Z, kk);
+        y = z.x - z.y;
         break;
+    }
     default:
         y = (T)0;
+    }
     }
     if ((a.flags & F_POST_ALT) && (k & 1)) y = -y;
     if (a.flags & F_WEIGHTS) {
@@ -256,7 +295,7 @@ __device__ T post_real(const StageArgs<T> &a, const cpx<T> *Z, const cpx<T> *tws
 
 // Complex output value of a complex-output mode.
 template <typename T>
-__device__ cpx<T> post_complex(const StageArgs<T> &a, const cpx<T> *Z, const cpx<T> *tws, int k)
+__device__ __forceinline__ cpx<T> post_complex(const StageArgs<T> &a, const cpx<T> *Z, const cpx<T> *tws, int k)
 {
     cpx<T> v;
     switch (a.mode) {
@@ -270,7 +309,7 @@ __device__ cpx<T> post_complex(const StageArgs<T> &a, const cpx<T> *Z, const cpx
         break;
     }
     case M_R2C_ODD: {
-        v = Z[k];
+        v = L(Z, k);
         if (k == 0) v.y = (T)0;
         if (a.flags & F_HERM_DOUBLE) {
             if (k > 0) v = scal(v, (T)2);
@@ -279,17 +318,26 @@ __device__ cpx<T> post_complex(const StageArgs<T> &a, const cpx<T> *Z, const cpx
         break;
     }
     default:
-        v = Z[k];
+        v = L(Z, k);
         if (a.flags & F_CONJ_OUT) v.y = -v.y;
         break;
     }
     return scal(v, a.scale);
 }
 
+template <typename T>
+constexpr int pass_mb(int R)
+{
+    return (Regs<T>::emax / R) > 0 ? Regs<T>::emax / R : 1;
+}
+
+// One Stockham radix-R pass in LDS: butterflies j = tl + tpl * b read
+// x[j + r m/R], twiddle by exp(-2 pi i r (j mod Ns) / (Ns R)) and write
+// y[(j - j mod Ns) R + j mod Ns + r Ns].
 template <typename T, int R>
 __device__ __forceinline__ void radix_pass(cpx<T> *ln, const cpx<T> *twp, int m, int Ns, int tl, int tpl)
 {
-    constexpr int MB = (Regs<T>::emax / R) > 0 ? Regs<T>::emax / R : 1;
+    constexpr int MB = pass_mb<T>(R);
     const int nb = m / R;
     cpx<T> v[MB][R];
 #pragma unroll
@@ -297,7 +345,7 @@ __device__ __forceinline__ void radix_pass(cpx<T> *ln, const cpx<T> *twp, int m,
         const int j = tl + tpl * b;
         if (j < nb) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = ln[j + r * nb];
+            for (int r = 0; r < R; ++r) v[b][r] = L(ln, j + r * nb);
             if (Ns > 1) {
                 const int k = j % Ns;
 #pragma unroll
@@ -314,7 +362,7 @@ __device__ __forceinline__ void radix_pass(cpx<T> *ln, const cpx<T> *twp, int m,
             const int k = j % Ns;
             const int base = (j - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) ln[base + r * Ns] = v[b][r];
+            for (int r = 0; r < R; ++r) Lset(ln, base + r * Ns, v[b][r]);
         }
     }
     __syncthreads();
@@ -326,7 +374,7 @@ __device__ __forceinline__ void radix_pass(cpx<T> *ln, const cpx<T> *twp, int m,
 // thread computes whole outputs as O(R) sums from LDS. roots = exp(-2 pi i
 // e / R), e < R, followed by the usual Ns (R - 1) pass twiddles.
 template <typename T>
-__device__ void radix_pass_generic(cpx<T> *ln, const cpx<T> *roots, int m, int R, int Ns, int tl, int tpl)
+__device__ __forceinline__ void radix_pass_generic(cpx<T> *ln, const cpx<T> *roots, int m, int R, int Ns, int tl, int tpl)
 {
     const cpx<T> *twp = roots + R;
     const int nb = m / R;
@@ -341,7 +389,7 @@ __device__ void radix_pass_generic(cpx<T> *ln, const cpx<T> *roots, int m, int R
             cpx<T> acc = mk((T)0, (T)0);
             int e = 0;
             for (int q = 0; q < R; ++q) {
-                cpx<T> x = ln[j + q * nb];
+                cpx<T> x = L(ln, j + q * nb);
                 if (Ns > 1 && q > 0) x = cmul(x, twp[k * (R - 1) + (q - 1)]);
                 acc = acc + cmul(x, roots[e]);
                 e += r;
@@ -357,17 +405,231 @@ __device__ void radix_pass_generic(cpx<T> *ln, const cpx<T> *roots, int m, int R
         if (o < m) {
             const int r = o / nb, j = o - r * nb;
             const int k = j % Ns;
-            ln[(j - k) * R + k + r * Ns] = v[i];
+            Lset(ln, (j - k) * R + k + r * Ns, v[i]);
         }
     }
     __syncthreads();
 }
 
+
+// ---- load: global -> LDS (raw line values, all loads of a chunk in flight)
+template <typename T>
+__device__ __forceinline__ void stage_load(const StageArgs<T> &a, cpx<T> *data, int64_t tile, int tid, int l, int tl)
+{
+    const int lt = a.lt, tpl = a.tpl;
+    if (tid >= lt * tpl) { // threads beyond this stage's tile shape only join the barrier
+        __syncthreads();
+        return;
+    }
+    const bool cin = complex_input(a.mode);
+    int ll, p0;
+    if (a.in_col) {
+        ll = tid % lt;
+        p0 = tid / lt;
+    } else {
+        ll = l;
+        p0 = tl;
+    }
+    const int step = tpl;
+    const int64_t line = tile * lt + ll;
+    if (line < a.nlines) {
+        int64_t io, oo;
+        line_offsets(a.g, line, io, oo);
+        cpx<T> *lln = data + ll * a.ls;
+        if (cin) {
+            const cpx<T> *s = reinterpret_cast<const cpx<T> *>(a.src) + io;
+            const bool cj = (a.flags & F_CONJ_IN) != 0;
+            for (int base = p0; base < a.nin; base += step * Regs<T>::unroll) {
+                cpx<T> v[Regs<T>::unroll];
+#pragma unroll
+                for (int u = 0; u < Regs<T>::unroll; ++u) {
+                    const int p = base + u * step;
+                    if (p < a.nin) v[u] = s[(int64_t)p * a.ips];
+                }
+#pragma unroll
+                for (int u = 0; u < Regs<T>::unroll; ++u) {
+                    const int p = base + u * step;
+                    if (p < a.nin) {
+                        cpx<T> w = v[u];
+                        if (cj) w.y = -w.y;
+                        Lset(lln, p, w);
+                    }
+                }
+            }
+        } else {
+            const T *s = reinterpret_cast<const T *>(a.src) + io;
+            for (int base = p0; base < a.nin; base += step * (2 * Regs<T>::unroll)) {
+                T v[2 * Regs<T>::unroll];
+#pragma unroll
+                for (int u = 0; u < 2 * Regs<T>::unroll; ++u) {
+                    const int p = base + u * step;
+                    if (p < a.nin) v[u] = s[(int64_t)p * a.ips];
+                }
+#pragma unroll
+                for (int u = 0; u < 2 * Regs<T>::unroll; ++u) {
+                    const int p = base + u * step;
+                    if (p < a.nin) LRset(lln, p, v[u]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// ---- post-processing + store: LDS -> global (output layout)
+template <typename T, int MG>
+__device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> *data, const cpx<T> *tws,
+                                            int64_t tile, int tid, int l, int tl)
+{
+    const int lt = a.lt, tpl = a.tpl;
+    if (tid >= lt * tpl) return;
+    {
+        int lo, k0;
+        if (a.out_col) {
+            lo = tid % lt;
+            k0 = tid / lt;
+        } else {
+            lo = l;
+            k0 = tl;
+        }
+        const int64_t line = tile * lt + lo;
+        if (line < a.nlines) {
+            int64_t io, oo;
+            line_offsets(a.g, line, io, oo);
+            const cpx<T> *Z = data + lo * a.ls;
+            constexpr int U = Regs<T>::unroll;
+            if constexpr (MG == 0) {
+            if (a.mode == M_COPY_C) {
+                // identity stage: move the bits, no arithmetic (bit-exact permutation)
+                cpx<T> *d = reinterpret_cast<cpx<T> *>(a.dst) + oo;
+                for (int base = k0; base < a.nout; base += tpl * U) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) d[(int64_t)k * a.ops] = L(Z, k);
+                    }
+                }
+            } else if (a.mode == M_COPY_R) {
+                T *d = reinterpret_cast<T *>(a.dst) + oo;
+                for (int base = k0; base < a.nout; base += tpl * U) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) d[(int64_t)k * a.ops] = LR(Z, k);
+                    }
+                }
+            }
+            } else {
+            if (MG == 1 && complex_output(a.mode)) {
+                cpx<T> *d = reinterpret_cast<cpx<T> *>(a.dst) + oo;
+                for (int base = k0; base < a.nout; base += tpl * U) {
+                    cpx<T> v[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) v[u] = post_complex(a, Z, tws, k);
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) d[(int64_t)k * a.ops] = v[u];
+                    }
+                }
+            } else {
+                T *d = reinterpret_cast<T *>(a.dst) + oo;
+                for (int base = k0; base < a.nout; base += tpl * U) {
+                    T v[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) v[u] = post_real<T, MG>(a, Z, tws, k);
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) d[(int64_t)k * a.ops] = v[u];
+                    }
+                }
+            }
+            }
+        }
+    }
+}
+
+template <typename T, bool WIDE>
+__device__ __forceinline__ void run_pass(int R, cpx<T> *ln, const cpx<T> *twp, int m, int Ns, int tl, int tpl)
+{
+    switch (R) {
+    case 2: radix_pass<T, 2>(ln, twp, m, Ns, tl, tpl); break;
+    case 3: radix_pass<T, 3>(ln, twp, m, Ns, tl, tpl); break;
+    case 4: radix_pass<T, 4>(ln, twp, m, Ns, tl, tpl); break;
+    case 5: radix_pass<T, 5>(ln, twp, m, Ns, tl, tpl); break;
+    case 8: radix_pass<T, 8>(ln, twp, m, Ns, tl, tpl); break;
+    case 16:
+        if constexpr (Regs<T>::max_radix >= 16) radix_pass<T, 16>(ln, twp, m, Ns, tl, tpl);
+        break;
+    default:
+        if constexpr (WIDE) {
+            if (R == 7) radix_pass<T, 7>(ln, twp, m, Ns, tl, tpl);
+            else if (R == 11) radix_pass<T, 11>(ln, twp, m, Ns, tl, tpl);
+            else if (R == 13) radix_pass<T, 13>(ln, twp, m, Ns, tl, tpl);
+            else radix_pass_generic<T>(ln, twp, m, R, Ns, tl, tpl);
+        }
+        break;
+    }
+}
+
 // WIDE instantiations also carry the radix 7 / 11 / 13 butterflies and the
 // generic pass; the narrow one keeps register pressure down for the
-// power-of-two and 3 / 5 lengths.
-template <typename T, bool WIDE>
-__global__ __launch_bounds__(kMaxThreads) void k_line_stage(const StageArgs<T> a)
+// power-of-two and 3 / 5 lengths (and adds radix 32 for f32).
+// One tile of a generic stage. Threads beyond lt * tpl (a batched launch
+// sizes its blocks for the largest item) only take part in the barriers.
+template <typename T, bool WIDE, int MG>
+__device__ __forceinline__ void stage_body(const StageArgs<T> &a, int64_t tile, const cpx<T> *tws, cpx<T> *data,
+                                           const int *radix, const int *twoff)
+{
+    const int tid = threadIdx.x;
+    const int lt = a.lt, tpl = a.tpl;
+    const bool act = tid < lt * tpl;
+    // pass mapping: lt lines x tpl threads; inactive threads get an index
+    // beyond every line so that each pass skips them
+    const int l = act ? tid / tpl : 0;
+    const int tl = act ? tid % tpl : (1 << 28);
+    cpx<T> *ln = data + l * a.ls;
+    stage_load(a, data, tile, tid, l, tl);
+    if constexpr (MG == 0) {
+        stage_store<T, 0>(a, data, tws, tile, tid, l, tl);
+        return;
+    }
+    // ---- pre-processing: raw line -> FFT input (in place through registers)
+    if (needs_pre(a.mode)) {
+        constexpr int E = Regs<T>::emax;
+        cpx<T> z[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int j = tl + tpl * i;
+            if (j < a.m) z[i] = pre_value<T, MG>(a, ln, tws, j);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int j = tl + tpl * i;
+            if (j < a.m) Lset(ln, j, z[i]);
+        }
+        __syncthreads();
+    }
+    // ---- Stockham passes in LDS (decimation in time, natural-order output)
+    int Ns = 1;
+    for (int p = 0; p < a.npass; ++p) {
+        const int R = radix[p];
+        run_pass<T, WIDE>(R, ln, tws + twoff[p], a.m, Ns, tl, tpl);
+        Ns *= R;
+    }
+    stage_store<T, MG>(a, data, tws, tile, tid, l, tl);
+}
+
+template <typename T, bool WIDE, int MG>
+__global__ __launch_bounds__(WIDE ? kMaxThreadsWide : kMaxThreads) void k_line_stage(const StageArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
@@ -381,180 +643,416 @@ __global__ __launch_bounds__(kMaxThreads) void k_line_stage(const StageArgs<T> a
         if (tile >= a.ntiles) return;
     }
     for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
-
-    const int lt = a.lt, tpl = a.tpl;
-    const bool cin = complex_input(a.mode);
-    // ---- load: global -> LDS
-    {
-        int l, p0, step;
-        if (a.in_col) {
-            l = tid % lt;
-            p0 = tid / lt;
-        } else {
-            l = tid / tpl;
-            p0 = tid % tpl;
+    // pass plan in LDS (read with a runtime pass index; indexing the argument
+    // struct dynamically would force a private copy of it)
+    __shared__ int s_radix[kMaxPasses], s_twoff[kMaxPasses];
+    if (tid == 0) {
+#pragma unroll
+        for (int p = 0; p < kMaxPasses; ++p) {
+            s_radix[p] = a.radix[p];
+            s_twoff[p] = a.twoff[p];
         }
-        step = tpl;
-        const int64_t line = tile * lt + l;
-        if (line < a.nlines) {
-            int64_t io, oo;
-            line_offsets(a.g, line, io, oo);
-            cpx<T> *ln = data + l * a.ls;
-            // all loads of a chunk are issued before the first LDS write so that
-            // up to Regs<T>::unroll global loads per thread are in flight
-            if (cin) {
-                const cpx<T> *s = reinterpret_cast<const cpx<T> *>(a.src) + io;
-                const bool cj = (a.flags & F_CONJ_IN) != 0;
-                for (int base = p0; base < a.nin; base += step * Regs<T>::unroll) {
-                    cpx<T> v[Regs<T>::unroll];
+    }
+    stage_body<T, WIDE, MG>(a, tile, tws, data, s_radix, s_twoff);
+}
+
+// Persistent batched stage: the blocks of one launch walk a work list of
+// (item, tile) pairs, item = one route stage of one problem (its arguments
+// live in device memory), prefix[q] = first work index of item q. Each block
+// keeps its item index and only moves forward.
+template <typename T>
+__global__ __launch_bounds__(kMaxThreadsWide) void k_line_batch(const StageArgs<T> *__restrict__ items,
+                                                                const int64_t *__restrict__ prefix, int nitems,
+                                                                int64_t total)
+{
+    // LDS per tile: the item's twn table entries, then its lt lines
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    int q = 0;
+    {
+        // first item of this block: binary search
+        const int64_t w0 = blockIdx.x;
+        int lo = 0, hi = nitems;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (prefix[mid] <= w0) lo = mid;
+            else hi = mid;
+        }
+        q = lo;
+    }
+    for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        while (q + 1 < nitems && prefix[q + 1] <= w) ++q;
+        const StageArgs<T> &a = items[q];
+        const int64_t tile = w - prefix[q];
+        __syncthreads(); // the previous tile is done with LDS
+        for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
+        stage_body<T, true, 1>(a, tile, tws, tws + a.twn, a.radix, a.twoff);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fast path: power-of-two C2C / R2C / C2R stages with a compile-time radix
+// sequence (R0, R1[, R2]) and EM complex registers per thread and pass. The
+// first pass of a C2C / R2C stage reads global memory straight into
+// registers and the last pass of a C2C / C2R stage writes global memory
+// straight from registers, so a two-pass line goes through LDS once.
+// ---------------------------------------------------------------------------
+
+template <int EM, int R>
+constexpr int fast_mb()
+{
+    return (EM / R) > 0 ? EM / R : 1;
+}
+
+// first pass (Ns = 1) from global memory into LDS
+template <typename T, int R, int EM, int M>
+__device__ __forceinline__ void fast_first(const StageArgs<T> &a, cpx<T> *ln, bool valid, int64_t io, int tl, int tpl,
+                                           bool cplx, bool cj)
+{
+    constexpr int MB = fast_mb<EM, R>();
+    constexpr int NB = M / R;
+    cpx<T> v[MB][R];
+    if (valid) {
+        if (cplx) {
+            const cpx<T> *s = reinterpret_cast<const cpx<T> *>(a.src) + io;
 #pragma unroll
-                    for (int u = 0; u < Regs<T>::unroll; ++u) {
-                        const int p = base + u * step;
-                        if (p < a.nin) v[u] = s[(int64_t)p * a.ips];
-                    }
+            for (int b = 0; b < MB; ++b) {
+                const int j = tl + tpl * b;
+                if (j < NB) {
 #pragma unroll
-                    for (int u = 0; u < Regs<T>::unroll; ++u) {
-                        const int p = base + u * step;
-                        if (p < a.nin) {
-                            cpx<T> w = v[u];
-                            if (cj) w.y = -w.y;
-                            ln[p] = w;
-                        }
+                    for (int r = 0; r < R; ++r) v[b][r] = s[(int64_t)(j + r * NB) * a.ips];
+                }
+            }
+        } else if (a.ips == 1) {
+            const cpx<T> *s = reinterpret_cast<const cpx<T> *>(reinterpret_cast<const T *>(a.src) + io);
+#pragma unroll
+            for (int b = 0; b < MB; ++b) {
+                const int j = tl + tpl * b;
+                if (j < NB) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[b][r] = s[j + r * NB];
+                }
+            }
+        } else {
+            const T *s = reinterpret_cast<const T *>(a.src) + io;
+#pragma unroll
+            for (int b = 0; b < MB; ++b) {
+                const int j = tl + tpl * b;
+                if (j < NB) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const int64_t q = 2 * (int64_t)(j + r * NB);
+                        v[b][r] = mk(s[q * a.ips], s[(q + 1) * a.ips]);
                     }
                 }
-            } else {
-                const T *s = reinterpret_cast<const T *>(a.src) + io;
-                T *lr = reinterpret_cast<T *>(ln);
-                for (int base = p0; base < a.nin; base += step * (2 * Regs<T>::unroll)) {
-                    T v[2 * Regs<T>::unroll];
+            }
+        }
 #pragma unroll
-                    for (int u = 0; u < 2 * Regs<T>::unroll; ++u) {
-                        const int p = base + u * step;
-                        if (p < a.nin) v[u] = s[(int64_t)p * a.ips];
-                    }
+        for (int b = 0; b < MB; ++b) {
+            const int j = tl + tpl * b;
+            if (j < NB) {
+                if (cj) {
 #pragma unroll
-                    for (int u = 0; u < 2 * Regs<T>::unroll; ++u) {
-                        const int p = base + u * step;
-                        if (p < a.nin) lr[p] = v[u];
-                    }
+                    for (int r = 0; r < R; ++r) v[b][r].y = -v[b][r].y;
                 }
+                Dft<T, R>::run(v[b]);
+#pragma unroll
+                for (int r = 0; r < R; ++r) Lset(ln, j * R + r, v[b][r]);
             }
         }
     }
     __syncthreads();
+}
 
+// a pass in LDS with compile-time geometry
+template <typename T, int R, int EM, int M>
+__device__ __forceinline__ void fast_mid(cpx<T> *ln, const cpx<T> *twp, int Ns, int tl, int tpl)
+{
+    constexpr int MB = fast_mb<EM, R>();
+    constexpr int NB = M / R;
+    cpx<T> v[MB][R];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int j = tl + tpl * b;
+        if (j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = L(ln, j + r * NB);
+            if (Ns > 1) {
+                const int k = j % Ns;
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], twp[k * (R - 1) + (r - 1)]);
+            }
+            Dft<T, R>::run(v[b]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int j = tl + tpl * b;
+        if (j < NB) {
+            const int k = j % Ns;
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) Lset(ln, base + r * Ns, v[b][r]);
+        }
+    }
+    __syncthreads();
+}
+
+// last pass (Ns = M / R) from LDS straight to global memory: complex output
+// (C2C, conjugation and scale applied) or the real pairs of a C2R line
+// (y_2k = Re conj Z_k, y_2k+1 = Im conj Z_k).
+template <typename T, int R, int EM, int M>
+__device__ __forceinline__ void fast_last(const StageArgs<T> &a, const cpx<T> *ln, const cpx<T> *twp, bool valid,
+                                          int64_t oo, int tl, int tpl, bool c2r)
+{
+    constexpr int MB = fast_mb<EM, R>();
+    constexpr int NB = M / R;
+    if (!valid) return;
+    cpx<T> v[MB][R];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int j = tl + tpl * b;
+        if (j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = L(ln, j + r * NB);
+            if (NB > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], twp[j * (R - 1) + (r - 1)]);
+            }
+            Dft<T, R>::run(v[b]);
+        }
+    }
+    const T sc = a.scale;
+    if (!c2r) {
+        cpx<T> *d = reinterpret_cast<cpx<T> *>(a.dst) + oo;
+        const bool cj = (a.flags & F_CONJ_OUT) != 0;
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            const int j = tl + tpl * b;
+            if (j < NB) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    cpx<T> w = v[b][r];
+                    if (cj) w.y = -w.y;
+                    d[(int64_t)(j + r * NB) * a.ops] = scal(w, sc);
+                }
+            }
+        }
+    } else if (a.ops == 1) {
+        cpx<T> *d = reinterpret_cast<cpx<T> *>(reinterpret_cast<T *>(a.dst) + oo);
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            const int j = tl + tpl * b;
+            if (j < NB) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) d[j + r * NB] = mk(v[b][r].x * sc, -v[b][r].y * sc);
+            }
+        }
+    } else {
+        T *d = reinterpret_cast<T *>(a.dst) + oo;
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            const int j = tl + tpl * b;
+            if (j < NB) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int64_t q = 2 * (int64_t)(j + r * NB);
+                    d[q * a.ops] = v[b][r].x * sc;
+                    d[(q + 1) * a.ops] = -v[b][r].y * sc;
+                }
+            }
+        }
+    }
+}
+
+// C2R merge: half spectrum X_0..X_m in LDS -> conj of z_j = E_j + i O_j in
+// LDS (z_j = x_2j + i x_2j+1 after the positive-sign transform).
+template <typename T, int EM, int M>
+__device__ __forceinline__ void fast_c2r_pre(const StageArgs<T> &a, cpx<T> *ln, const cpx<T> *tws, int tl, int tpl)
+{
+    constexpr int E = (M + 1 < EM) ? M + 1 : EM;
+    cpx<T> z[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int j = tl + tpl * i;
+        if (j < M) {
+            cpx<T> A = hin(a, ln, j, M);
+            cpx<T> B = conjg(hin(a, ln, M - j, M));
+            cpx<T> Ev = A + B, D = A - B;
+            cpx<T> O = cmul(tws[a.pre0 + j], D);
+            z[i] = conjg(Ev + mul_pi(O));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int j = tl + tpl * i;
+        if (j < M) Lset(ln, j, z[i]);
+    }
+    __syncthreads();
+}
+
+// R2C split + store of X_0..X_M from the half-length transform in LDS.
+template <typename T>
+__device__ __forceinline__ void fast_r2c_store(const StageArgs<T> &a, const cpx<T> *data, const cpx<T> *tws,
+                                               int64_t tile, int tid, int l, int tl)
+{
+    const int lt = a.lt, tpl = a.tpl;
+    int lo, k0;
+    if (a.out_col) {
+        lo = tid % lt;
+        k0 = tid / lt;
+    } else {
+        lo = l;
+        k0 = tl;
+    }
+    const int64_t line = tile * lt + lo;
+    if (line >= a.nlines) return;
+    int64_t io, oo;
+    line_offsets(a.g, line, io, oo);
+    const cpx<T> *Z = data + lo * a.ls;
+    cpx<T> *d = reinterpret_cast<cpx<T> *>(a.dst) + oo;
+    const cpx<T> *w = tws + a.post0;
+    constexpr int U = Regs<T>::unroll;
+    const bool dbl = (a.flags & F_HERM_DOUBLE) != 0;
+    for (int base = k0; base < a.nout; base += tpl * U) {
+        cpx<T> v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = base + u * tpl;
+            if (k < a.nout) {
+                cpx<T> x = split(Z, w, a.m, k);
+                if (k == 0 || k == a.m) x.y = (T)0;
+                if (dbl && k > 0 && k < a.m) x = scal(x, (T)2);
+                v[u] = scal(x, a.scale);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = base + u * tpl;
+            if (k < a.nout) d[(int64_t)k * a.ops] = v[u];
+        }
+    }
+}
+
+template <typename T, int R0, int R1, int R2, int EM>
+__global__ __launch_bounds__(512) void k_line_fast(const StageArgs<T> a)
+{
+    constexpr int M = R0 * R1 * (R2 ? R2 : 1);
+    constexpr int RL = R2 ? R2 : R1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
+    cpx<T> *data = tws + a.twn;
+    if (a.gate != nullptr && *a.gate != 0) return;
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    int64_t tile = blockIdx.x;
+    if (a.flags & F_XCD_REMAP) {
+        tile = (int64_t)(blockIdx.x % 8u) * a.xcd_per + blockIdx.x / 8u;
+        if (tile >= a.ntiles) return;
+    }
+    for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
+    const int lt = a.lt, tpl = a.tpl;
     const int l = tid / tpl, tl = tid % tpl;
     cpx<T> *ln = data + l * a.ls;
-    // ---- pre-processing: raw line -> FFT input (in place through registers)
-    if (needs_pre(a.mode)) {
-        constexpr int E = Regs<T>::emax;
-        cpx<T> z[E];
-#pragma unroll
-        for (int i = 0; i < E; ++i) {
-            const int j = tl + tpl * i;
-            if (j < a.m) z[i] = pre_value(a, ln, tws, j);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < E; ++i) {
-            const int j = tl + tpl * i;
-            if (j < a.m) ln[j] = z[i];
-        }
-        __syncthreads();
+    const int mode = a.mode;
+    if (mode == M_C2R) {
+        stage_load(a, data, tile, tid, l, tl);
+        fast_c2r_pre<T, EM, M>(a, ln, tws, tl, tpl);
+        fast_mid<T, R0, EM, M>(ln, tws, 1, tl, tpl);
+    } else {
+        // column access: lanes across the tile's lines; row access: along a line
+        const int fl = a.in_col ? tid % lt : l;
+        const int ft = a.in_col ? tid / lt : tl;
+        const int64_t line = tile * lt + fl;
+        int64_t io = 0, oo = 0;
+        const bool valid = line < a.nlines;
+        if (valid) line_offsets(a.g, line, io, oo);
+        fast_first<T, R0, EM, M>(a, data + fl * a.ls, valid, io, ft, tpl, mode == M_C2C,
+                                 (a.flags & F_CONJ_IN) != 0);
     }
-    // ---- Stockham passes (decimation in time, natural-order output)
-    int Ns = 1;
-    for (int p = 0; p < a.npass; ++p) {
-        const int R = a.radix[p];
-        const cpx<T> *twp = tws + a.twoff[p];
-        switch (R) {
-        case 2: radix_pass<T, 2>(ln, twp, a.m, Ns, tl, tpl); break;
-        case 3: radix_pass<T, 3>(ln, twp, a.m, Ns, tl, tpl); break;
-        case 4: radix_pass<T, 4>(ln, twp, a.m, Ns, tl, tpl); break;
-        case 5: radix_pass<T, 5>(ln, twp, a.m, Ns, tl, tpl); break;
-        case 8: radix_pass<T, 8>(ln, twp, a.m, Ns, tl, tpl); break;
-        case 16:
-            if constexpr (Regs<T>::max_radix >= 16) radix_pass<T, 16>(ln, twp, a.m, Ns, tl, tpl);
-            break;
-        default:
-            if constexpr (WIDE) {
-                if (R == 7) radix_pass<T, 7>(ln, twp, a.m, Ns, tl, tpl);
-                else if (R == 11) radix_pass<T, 11>(ln, twp, a.m, Ns, tl, tpl);
-                else if (R == 13) radix_pass<T, 13>(ln, twp, a.m, Ns, tl, tpl);
-                else radix_pass_generic<T>(ln, twp, a.m, R, Ns, tl, tpl);
-            }
-            break;
-        }
-        Ns *= R;
+    if constexpr (R2 != 0) fast_mid<T, R1, EM, M>(ln, tws + a.twoff[1], R0, tl, tpl);
+    if (mode == M_R2C) {
+        fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
+        fast_r2c_store(a, data, tws, tile, tid, l, tl);
+        return;
     }
-    // ---- post-processing + store: LDS -> global (output layout)
-    {
-        int lo, k0;
-        if (a.out_col) {
-            lo = tid % lt;
-            k0 = tid / lt;
-        } else {
-            lo = tid / tpl;
-            k0 = tid % tpl;
-        }
-        const int64_t line = tile * lt + lo;
-        if (line < a.nlines) {
-            int64_t io, oo;
-            line_offsets(a.g, line, io, oo);
-            const cpx<T> *Z = data + lo * a.ls;
-            if (a.mode == M_COPY_C) {
-                // identity stage: move the bits, no arithmetic (bit-exact permutation)
-                cpx<T> *d = reinterpret_cast<cpx<T> *>(a.dst) + oo;
-                for (int base = k0; base < a.nout; base += tpl * Regs<T>::unroll) {
-#pragma unroll
-                    for (int u = 0; u < Regs<T>::unroll; ++u) {
-                        const int k = base + u * tpl;
-                        if (k < a.nout) d[(int64_t)k * a.ops] = Z[k];
-                    }
-                }
-            } else if (a.mode == M_COPY_R) {
-                T *d = reinterpret_cast<T *>(a.dst) + oo;
-                const T *zr = reinterpret_cast<const T *>(Z);
-                for (int base = k0; base < a.nout; base += tpl * Regs<T>::unroll) {
-#pragma unroll
-                    for (int u = 0; u < Regs<T>::unroll; ++u) {
-                        const int k = base + u * tpl;
-                        if (k < a.nout) d[(int64_t)k * a.ops] = zr[k];
-                    }
-                }
-            } else if (complex_output(a.mode)) {
-                cpx<T> *d = reinterpret_cast<cpx<T> *>(a.dst) + oo;
-                for (int base = k0; base < a.nout; base += tpl * Regs<T>::unroll) {
-                    cpx<T> v[Regs<T>::unroll];
-#pragma unroll
-                    for (int u = 0; u < Regs<T>::unroll; ++u) {
-                        const int k = base + u * tpl;
-                        if (k < a.nout) v[u] = post_complex(a, Z, tws, k);
-                    }
-#pragma unroll
-                    for (int u = 0; u < Regs<T>::unroll; ++u) {
-                        const int k = base + u * tpl;
-                        if (k < a.nout) d[(int64_t)k * a.ops] = v[u];
-                    }
-                }
-            } else {
-                T *d = reinterpret_cast<T *>(a.dst) + oo;
-                for (int base = k0; base < a.nout; base += tpl * Regs<T>::unroll) {
-                    T v[Regs<T>::unroll];
-#pragma unroll
-                    for (int u = 0; u < Regs<T>::unroll; ++u) {
-                        const int k = base + u * tpl;
-                        if (k < a.nout) v[u] = post_real(a, Z, tws, k);
-                    }
-#pragma unroll
-                    for (int u = 0; u < Regs<T>::unroll; ++u) {
-                        const int k = base + u * tpl;
-                        if (k < a.nout) d[(int64_t)k * a.ops] = v[u];
-                    }
-                }
-            }
-        }
+    const int fl = a.out_col ? tid % lt : l;
+    const int ft = a.out_col ? tid / lt : tl;
+    const int64_t line = tile * lt + fl;
+    int64_t io = 0, oo = 0;
+    const bool valid = line < a.nlines;
+    if (valid) line_offsets(a.g, line, io, oo);
+    fast_last<T, RL, EM, M>(a, data + fl * a.ls, tws + a.twoff[R2 ? 2 : 1], valid, oo, ft, tpl, mode == M_C2R);
+}
+
+// Instantiated fast plans: {R0, R1, R2, EM}.
+#define PFH_FAST_PLANS(X)      \
+    X(float, 8, 8, 0, 16)      \
+    X(float, 8, 16, 0, 16)     \
+    X(float, 16, 16, 0, 16)    \
+    X(float, 16, 32, 0, 16)    \
+    X(float, 16, 32, 0, 32)    \
+    X(float, 8, 8, 8, 16)      \
+    X(float, 32, 32, 0, 32)    \
+    X(double, 8, 8, 0, 8)      \
+    X(double, 4, 4, 8, 8)      \
+    X(double, 4, 8, 8, 8)      \
+    X(double, 8, 8, 8, 8)
+
+template <typename T>
+hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int em, int threads, size_t lds_bytes,
+                            int64_t grid, hipStream_t s)
+{
+    if (grid <= 0) return hipSuccess;
+#define PFH_FAST_CASE(TT, A, B, C, E)                                                                           \
+    if constexpr (std::is_same<T, TT>::value) if (r0 == A && r1 == B && r2 == C && em == E) {                               \
+        hipLaunchKernelGGL((k_line_fast<T, A, B, C, E>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, a); \
+        return hipGetLastError();                                                                               \
     }
+    PFH_FAST_PLANS(PFH_FAST_CASE)
+#undef PFH_FAST_CASE
+    return hipErrorInvalidValue;
+}
+
+template hipError_t launch_line_fast<float>(const StageArgs<float> &, int, int, int, int, int, size_t, int64_t,
+                                            hipStream_t);
+template hipError_t launch_line_fast<double>(const StageArgs<double> &, int, int, int, int, int, size_t, int64_t,
+                                             hipStream_t);
+
+bool fast_plan_available(int tbytes, int r0, int r1, int r2, int em)
+{
+#define PFH_FAST_HAS(TT, A, B, C, E) \
+    if ((int)sizeof(TT) == tbytes && r0 == A && r1 == B && r2 == C && em == E) return true;
+    PFH_FAST_PLANS(PFH_FAST_HAS)
+#undef PFH_FAST_HAS
+    return false;
+}
+
+// Raises a kernel's dynamic LDS limit to `bytes` minus its static LDS.
+static hipError_t raise_lds_limit(const void *f, size_t bytes)
+{
+    hipFuncAttributes at{};
+    hipError_t e = hipFuncGetAttributes(&at, f);
+    if (e != hipSuccess) return e;
+    const int v = (int)bytes - (int)at.sharedSizeBytes;
+    if (v <= 0) return hipErrorInvalidValue;
+    return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+}
+
+hipError_t set_fast_lds_limit(size_t bytes)
+{
+#define PFH_FAST_ATTR(TT, A, B, C, E)                                                                    \
+    {                                                                                                     \
+        hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_line_fast<TT, A, B, C, E>), bytes); \
+        if (e != hipSuccess) return e;                                                                    \
+    }
+    PFH_FAST_PLANS(PFH_FAST_ATTR)
+#undef PFH_FAST_ATTR
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------
@@ -598,9 +1096,11 @@ __device__ __forceinline__ double block_reduce_sum(double v, double *sh)
 // sums of (re/m)^2, (im/m)^2 normalized by the chunk maximum m, plus a
 // non-finite flag. grid: x = chunks (e.nblk), y = batch * planes + plane.
 template <typename T>
-__global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const void *data, EndpointPartial *part)
+__global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const void *data, EndpointPartial *part,
+                                                          int *gate)
 {
     __shared__ double sh[4];
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *gate = 0;
     const int64_t bp = blockIdx.y;
     const int64_t batch = bp / e.planes;
     const int plane = (int)(bp % e.planes);
@@ -650,45 +1150,45 @@ __global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const 
     }
 }
 
-// Combines the chunks of every batch in a fixed order (global max M, sums
-// rescaled by (m_chunk / M)^2, which equals the reference's sums of (x/M)^2),
-// applies the acceptance rule (distributed.rs:4330-4349) and sets *gate and
-// the sticky status.
+// Combines the chunks of one batch (one block per batch) with a fixed-order
+// tree reduction: global max M, sums rescaled by (m_chunk / M)^2 (equal to the
+// reference's sums of (x/M)^2), then the acceptance rule
+// (distributed.rs:4330-4349). Block (0) of k_endpoint_partial cleared *gate.
 __global__ __launch_bounds__(256) void k_endpoint_combine(EndpointArgs e, const EndpointPartial *part, int *gate,
                                                           int *status)
 {
-    __shared__ int any_bad;
-    if (threadIdx.x == 0) any_bad = 0;
-    __syncthreads();
-    for (int64_t b = threadIdx.x; b < e.extra; b += blockDim.x) {
-        bool invalid = !isfinite(e.relative) || !isfinite(e.absolute);
-        for (int p = 0; p < e.planes; ++p) {
-            const EndpointPartial *q = part + (b * e.planes + p) * e.nblk;
-            double M = 0.0, MI = 0.0;
-            for (int c = 0; c < e.nblk; ++c) {
-                M = fmax(M, q[c].maxa);
-                MI = fmax(MI, q[c].maxi);
-                if (q[c].bad) invalid = true;
-            }
-            double s0 = 0.0, s1 = 0.0;
-            if (M != 0.0) {
-                for (int c = 0; c < e.nblk; ++c) {
-                    const double f = q[c].maxa / M;
-                    s0 += q[c].sre * f * f;
-                    s1 += q[c].sim * f * f;
-                }
-            }
-            if (!isfinite(M) || !isfinite(MI) || !isfinite(s0) || !isfinite(s1)) invalid = true;
-            const bool abs_ok = MI <= e.absolute;
-            const bool rel_ok = sqrt(s1) <= e.relative * sqrt(s0);
-            if (!(abs_ok || rel_ok)) invalid = true;
+    __shared__ double sh[4];
+    const int64_t b = blockIdx.x;
+    bool invalid = !isfinite(e.relative) || !isfinite(e.absolute);
+    for (int p = 0; p < e.planes; ++p) {
+        const EndpointPartial *q = part + (b * e.planes + p) * e.nblk;
+        double ma = 0.0, mi = 0.0, bad = 0.0;
+        for (int c = threadIdx.x; c < e.nblk; c += blockDim.x) {
+            ma = fmax(ma, q[c].maxa);
+            mi = fmax(mi, q[c].maxi);
+            if (q[c].bad) bad = 1.0;
         }
-        if (invalid) atomicOr(&any_bad, 1);
+        const double M = block_reduce_max(ma, sh);
+        const double MI = block_reduce_max(mi, sh);
+        if (block_reduce_max(bad, sh) != 0.0) invalid = true;
+        double s0 = 0.0, s1 = 0.0;
+        if (M != 0.0) {
+            for (int c = threadIdx.x; c < e.nblk; c += blockDim.x) {
+                const double f = q[c].maxa / M;
+                s0 += q[c].sre * f * f;
+                s1 += q[c].sim * f * f;
+            }
+        }
+        s0 = block_reduce_sum(s0, sh);
+        s1 = block_reduce_sum(s1, sh);
+        if (!isfinite(M) || !isfinite(MI) || !isfinite(s0) || !isfinite(s1)) invalid = true;
+        const bool abs_ok = MI <= e.absolute;
+        const bool rel_ok = sqrt(s1) <= e.relative * sqrt(s0);
+        if (!(abs_ok || rel_ok)) invalid = true;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        *gate = any_bad;
-        if (any_bad) atomicOr(status, PFH_DEVICE_INVALID_SPECTRUM);
+    if (threadIdx.x == 0 && invalid) {
+        atomicOr(gate, 1);
+        atomicOr(status, PFH_DEVICE_INVALID_SPECTRUM);
     }
 }
 
@@ -752,22 +1252,69 @@ hipError_t launch_line_stage(const StageArgs<T> &a, bool wide, int threads, size
                              hipStream_t s)
 {
     if (grid <= 0) return hipSuccess;
-    if (wide) hipLaunchKernelGGL((k_line_stage<T, true>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, a);
-    else hipLaunchKernelGGL((k_line_stage<T, false>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, a);
+    const int mg = (a.mode == M_COPY_C || a.mode == M_COPY_R)
+                       ? 0
+                       : ((a.mode == M_C2C || a.mode == M_R2C || a.mode == M_R2C_ODD || a.mode == M_C2R ||
+                           a.mode == M_C2R_ODD)
+                              ? 1
+                              : 2);
+#define PFH_STAGE_CASE(W, G)                                                                                     \
+    if (wide == W && mg == G) {                                                                                  \
+        hipLaunchKernelGGL((k_line_stage<T, W, G>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, a);       \
+        return hipGetLastError();                                                                               \
+    }
+    PFH_STAGE_CASE(false, 0)
+    PFH_STAGE_CASE(false, 1)
+    PFH_STAGE_CASE(false, 2)
+    PFH_STAGE_CASE(true, 1)
+    PFH_STAGE_CASE(true, 2)
+#undef PFH_STAGE_CASE
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_line_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total,
+                             int threads, size_t lds_bytes, int64_t grid, hipStream_t s)
+{
+    if (total <= 0 || grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_line_batch<T>, dim3((unsigned)grid), dim3(threads), lds_bytes, s, items, prefix, nitems, total);
     return hipGetLastError();
 }
+
+template hipError_t launch_line_batch<float>(const StageArgs<float> *, const int64_t *, int, int64_t, int, size_t, int64_t,
+                                             hipStream_t);
+template hipError_t launch_line_batch<double>(const StageArgs<double> *, const int64_t *, int, int64_t, int, size_t,
+                                              int64_t, hipStream_t);
+
+template <typename T>
+int line_batch_blocks_per_cu(int threads, size_t lds_bytes)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_line_batch<T>, threads, lds_bytes) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
+}
+template int line_batch_blocks_per_cu<float>(int, size_t);
+template int line_batch_blocks_per_cu<double>(int, size_t);
 
 template hipError_t launch_line_stage<float>(const StageArgs<float> &, bool, int, size_t, int64_t, hipStream_t);
 template hipError_t launch_line_stage<double>(const StageArgs<double> &, bool, int, size_t, int64_t, hipStream_t);
 
 hipError_t set_line_stage_lds_limit(size_t bytes)
 {
-    const void *fns[4] = {reinterpret_cast<const void *>(&k_line_stage<float, false>),
-                          reinterpret_cast<const void *>(&k_line_stage<float, true>),
-                          reinterpret_cast<const void *>(&k_line_stage<double, false>),
-                          reinterpret_cast<const void *>(&k_line_stage<double, true>)};
+    const void *fns[] = {reinterpret_cast<const void *>(&k_line_stage<float, false, 0>),
+                         reinterpret_cast<const void *>(&k_line_stage<float, false, 1>),
+                         reinterpret_cast<const void *>(&k_line_stage<float, false, 2>),
+                         reinterpret_cast<const void *>(&k_line_stage<float, true, 1>),
+                         reinterpret_cast<const void *>(&k_line_stage<float, true, 2>),
+                         reinterpret_cast<const void *>(&k_line_stage<double, false, 0>),
+                         reinterpret_cast<const void *>(&k_line_stage<double, false, 1>),
+                         reinterpret_cast<const void *>(&k_line_stage<double, false, 2>),
+                         reinterpret_cast<const void *>(&k_line_stage<double, true, 1>),
+                         reinterpret_cast<const void *>(&k_line_stage<double, true, 2>),
+                         reinterpret_cast<const void *>(&k_line_batch<float>),
+                         reinterpret_cast<const void *>(&k_line_batch<double>)};
     for (const void *f : fns) {
-        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        hipError_t e = raise_lds_limit(f, bytes);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -780,8 +1327,11 @@ hipError_t launch_endpoint_check(const EndpointArgs &e, const void *data, Endpoi
     const int64_t by = e.extra * e.planes;
     if (by > 0)
         hipLaunchKernelGGL(k_endpoint_partial<T>, dim3((unsigned)e.nblk, (unsigned)by), dim3(256), 0, s, e, data,
-                           part);
-    hipLaunchKernelGGL(k_endpoint_combine, dim3(1), dim3(256), 0, s, e, part, gate, status);
+                           part, gate);
+    else
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, gate, (int64_t)1, 0);
+    if (e.extra > 0)
+        hipLaunchKernelGGL(k_endpoint_combine, dim3((unsigned)e.extra), dim3(256), 0, s, e, part, gate, status);
     return hipGetLastError();
 }
 

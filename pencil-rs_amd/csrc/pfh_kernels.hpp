@@ -47,6 +47,20 @@ hipError_t launch_line_stage(const StageArgs<T> &a, bool wide, int threads, size
 
 hipError_t set_line_stage_lds_limit(size_t bytes);
 
+// Persistent batched stage over a device work list (see k_line_batch).
+template <typename T>
+hipError_t launch_line_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total,
+                             int threads, size_t lds_bytes, int64_t grid, hipStream_t s);
+template <typename T>
+int line_batch_blocks_per_cu(int threads, size_t lds_bytes);
+
+// Power-of-two stages with a compile-time radix sequence (see k_line_fast).
+template <typename T>
+hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int em, int threads, size_t lds_bytes,
+                            int64_t grid, hipStream_t s);
+bool fast_plan_available(int tbytes, int r0, int r1, int r2, int em);
+hipError_t set_fast_lds_limit(size_t bytes);
+
 template <typename T>
 hipError_t launch_endpoint_check(const EndpointArgs &e, const void *data, EndpointPartial *part, int *gate,
                                  int *status, hipStream_t s);

@@ -184,22 +184,22 @@ struct Dft<T, 5> {
     }
 };
 
-// Odd primes 7, 11, 13: pairwise symmetric form, O(R^2 / 2).
+// Odd primes 7, 11, 13: pairwise symmetric form, O(R^2 / 2), with the
+// symmetric sums / differences kept in the input slots (2R values live).
 template <typename T, int R>
 struct DftPrime {
     __host__ __device__ static __forceinline__ void run(cpx<T> *v)
     {
         constexpr int H = (R - 1) / 2;
-        cpx<T> a[H + 1], b[H + 1];
-#pragma unroll
-        for (int j = 1; j <= H; ++j) {
-            a[j] = v[j] + v[R - j];
-            b[j] = v[j] - v[R - j];
-        }
-        cpx<T> out[R];
         cpx<T> s0 = v[0];
 #pragma unroll
-        for (int j = 1; j <= H; ++j) s0 = s0 + a[j];
+        for (int j = 1; j <= H; ++j) {
+            const cpx<T> p = v[j] + v[R - j], q = v[j] - v[R - j];
+            v[j] = p;     // a_j
+            v[R - j] = q; // b_j
+            s0 = s0 + p;
+        }
+        cpx<T> out[R];
         out[0] = s0;
 #pragma unroll
         for (int k = 1; k <= H; ++k) {
@@ -210,11 +210,11 @@ struct DftPrime {
                 const int ee = e <= H ? e : R - e;
                 const T c = (T)PrimeTable<R>::c(ee);
                 const T s = (T)(e <= H ? PrimeTable<R>::s(ee) : -PrimeTable<R>::s(ee));
-                re = re + scal(a[j], c);
-                im = im + scal(b[j], s);
+                re = re + scal(v[j], c);
+                im = im + scal(v[R - j], s);
             }
             // exp(-i t) = cos t - i sin t
-            cpx<T> mi = mul_mi(im);
+            const cpx<T> mi = mul_mi(im);
             out[k] = re + mi;
             out[R - k] = re - mi;
         }
@@ -280,6 +280,88 @@ struct Dft<T, 16> {
             Dft<T, 4>::run(c);
 #pragma unroll
             for (int k1 = 0; k1 < 4; ++k1) v[k2 + 4 * k1] = c[k1];
+        }
+    }
+};
+
+// exp(-2 pi i e / 32) = (W32c(e), -W32s(e)), e = 0..21
+__host__ __device__ inline double W32c(int e)
+{
+    constexpr double t[22] = {1.0,
+                                 0.9807852804032304491262,
+                                 0.9238795325112867561282,
+                                 0.8314696123025452370788,
+                                 0.7071067811865475244008,
+                                 0.5555702330196022247428,
+                                 0.3826834323650897717285,
+                                 0.1950903220161282678483,
+                                 8.478427660368899643959e-32,
+                                 -0.1950903220161282678483,
+                                 -0.3826834323650897717285,
+                                 -0.5555702330196022247428,
+                                 -0.7071067811865475244008,
+                                 -0.8314696123025452370788,
+                                 -0.9238795325112867561282,
+                                 -0.9807852804032304491262,
+                                 -1.0,
+                                 -0.9807852804032304491262,
+                                 -0.9238795325112867561282,
+                                 -0.8314696123025452370788,
+                                 -0.7071067811865475244008,
+                                 -0.5555702330196022247428};
+    return t[e];
+}
+__host__ __device__ inline double W32s(int e)
+{
+    constexpr double t[22] = {0.0,
+                                 0.1950903220161282678483,
+                                 0.3826834323650897717285,
+                                 0.5555702330196022247428,
+                                 0.7071067811865475244008,
+                                 0.8314696123025452370788,
+                                 0.9238795325112867561282,
+                                 0.9807852804032304491262,
+                                 1.0,
+                                 0.9807852804032304491262,
+                                 0.9238795325112867561282,
+                                 0.8314696123025452370788,
+                                 0.7071067811865475244008,
+                                 0.5555702330196022247428,
+                                 0.3826834323650897717285,
+                                 0.1950903220161282678483,
+                                 1.695685532073779928792e-31,
+                                 -0.1950903220161282678483,
+                                 -0.3826834323650897717285,
+                                 -0.5555702330196022247428,
+                                 -0.7071067811865475244008,
+                                 -0.8314696123025452370788};
+    return t[e];
+}
+
+template <typename T>
+struct Dft<T, 32> {
+    __host__ __device__ static __forceinline__ void run(cpx<T> *v)
+    {
+        // j = r1 + 4 r2, k = k2 + 8 k1
+        cpx<T> t[4][8];
+#pragma unroll
+        for (int r1 = 0; r1 < 4; ++r1) {
+            cpx<T> c[8];
+#pragma unroll
+            for (int r2 = 0; r2 < 8; ++r2) c[r2] = v[r1 + 4 * r2];
+            Dft<T, 8>::run(c);
+#pragma unroll
+            for (int k2 = 0; k2 < 8; ++k2) {
+                const int e = r1 * k2;
+                t[r1][k2] = (e == 0) ? c[k2] : cmul(c[k2], mk((T)W32c(e), (T)-W32s(e)));
+            }
+        }
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) {
+            cpx<T> c[4] = {t[0][k2], t[1][k2], t[2][k2], t[3][k2]};
+            Dft<T, 4>::run(c);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) v[k2 + 8 * k1] = c[k1];
         }
     }
 };

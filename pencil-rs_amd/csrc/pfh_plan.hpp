@@ -84,6 +84,10 @@ struct Step {
     EndpointArgs ep{};
     bool local_endpoint_check = false;
     bool gated = false; // the check ran as its own launch; the stage still reads the gate
+    // compile-time specialized kernel (k_line_fast) when fast is set
+    bool fast = false;
+    int fr[3] = {0, 0, 0};
+    int fem = 16;
 };
 
 struct OpPlan {
@@ -99,6 +103,16 @@ struct DeviceInfo {
     int cus = 256;
 };
 
+// Planner knobs (defaults, or chosen by the batch autotuner).
+struct Tuning {
+    int col_lines = 0;   // lines per tile on column-access stages (0: 128 bytes per row)
+    int max_radix = 0;   // largest radix of the generic kernel (0: 16 for f32, 8 for f64)
+    int block = 0;       // target threads per block (0: 256)
+    int block_cap = 0;   // upper bound on threads per block (0: the kernel's launch bound)
+    bool allow_fast = true;
+    bool allow_xcd = true;
+};
+
 struct PlanImpl {
     pfh_desc desc{};
     int precision = PFH_F32;
@@ -106,6 +120,7 @@ struct PlanImpl {
     int N = 0;
     int64_t extra = 1;
     int64_t partials = 1; // endpoint-statistics chunks kept in the workspace status area
+    Tuning tuning;
     DeviceInfo dev;
     // op -> plan (out of place and in place)
     OpPlan ops[6];

@@ -17,11 +17,13 @@ constexpr int kMaxDigits = 5;
 // f32, 8 for f64 (same register footprint).
 template <typename T>
 struct Regs {
-    static constexpr int emax = sizeof(T) == 4 ? 16 : 8;
+    // generic kernel: 8 values per thread and pass (f32 and f64)
+    static constexpr int emax = 8;
     static constexpr int unroll = sizeof(T) == 4 ? 16 : 8; // loads / stores in flight per chunk
-    static constexpr int max_radix = sizeof(T) == 4 ? 16 : 8;
+    static constexpr int max_radix = sizeof(T) == 4 ? 32 : 8;
 };
-constexpr int kEmax = 16; // upper bound over precisions
+constexpr int kEmax = 8;         // generic kernel values per thread and pass
+constexpr int kMaxThreadsWide = 512; // generic kernel with the odd-prime passes
 constexpr int kMaxThreads = 1024;
 
 // Per-line transform applied by a stage (see pfh_kernels.hip for the math).
@@ -57,6 +59,8 @@ enum StageFlags : int {
     F_HERM_HALF = 128,  // C2R adjoint weights: x1/2 on interior bins
     F_WEIGHTS = 256,    // apply pre/post edge weights (R2R adjoints)
     F_XCD_REMAP = 512,  // map consecutive tiles onto one XCD
+    F_DIRECT_FIRST = 1024, // first radix pass reads global memory into registers
+    F_DIRECT_LAST = 2048,  // last radix pass writes global memory from registers (C2C)
 };
 
 struct LineGeom {

@@ -123,7 +123,7 @@ EXPORTED_SYMBOLS = (
     "pfh_version", "pfh_last_error", "pfh_status_string", "pfh_plan_create", "pfh_plan_destroy",
     "pfh_plan_workspace_bytes", "pfh_plan_layout", "pfh_execute", "pfh_execute_in_place", "pfh_local_execute",
     "pfh_local_execute_in_place", "pfh_execute_with_timing", "pfh_workspace_take_status", "pfh_batch_create",
-    "pfh_batch_destroy", "pfh_batch_workspace_bytes", "pfh_batch_execute",
+    "pfh_batch_destroy", "pfh_batch_workspace_bytes", "pfh_batch_execute", "pfh_batch_tuning",
 )
 
 
@@ -156,6 +156,8 @@ def load_library(path: str = LIB_PATH):
     lib.pfh_batch_destroy.argtypes = [vp]
     lib.pfh_batch_workspace_bytes.argtypes = [vp, ctypes.POINTER(sz)]
     lib.pfh_batch_execute.argtypes = [vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), vp, sz, vp, i]
+    ip = ctypes.POINTER(i)
+    lib.pfh_batch_tuning.argtypes = [vp, ip, ip, ip]
     _lib = lib
     return lib
 
@@ -582,3 +584,72 @@ class AllToAllvTransposePlan(LocalTransposePlan):
     device permutation."""
 
     _kind = ALLTOALLV_TRANSPOSE
+
+
+class BatchC2cPlan:
+    """A batch of independent C2C problems of uneven shapes (one C2cPlan
+    each, every axis selected, one precision), executed as one persistent
+    launch per route stage, optionally replayed from a captured hipGraph.
+    The reference runs such collections one plan after another
+    (crates/pencil-fft/src/distributed/collections.rs:229-311); the results
+    are those of the per-problem C2cPlan.execute calls."""
+
+    def __init__(self, shapes, dtype=torch.complex64, autotune: bool = True, extra_shapes=None):
+        _require_gpu()
+        lib = load_library()
+        n = len(shapes)
+        if n == 0:
+            raise ValueError("empty batch")
+        descs = (_Desc * n)()
+        for k, shp in enumerate(shapes):
+            extra = extra_shapes[k] if extra_shapes is not None else ()
+            descs[k] = _route_desc(C2C, shp, dtype, extra)
+        handle = ctypes.c_void_p()
+        _check(lib.pfh_batch_create(descs, n, int(bool(autotune)), ctypes.byref(handle)))
+        self._h = handle
+        self.shapes = [tuple(s) for s in shapes]
+        self.precision = _precision(dtype)
+        self._ws = None
+        nb = ctypes.c_size_t()
+        _check(lib.pfh_batch_workspace_bytes(self._h, ctypes.byref(nb)))
+        self.workspace_bytes = nb.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.pfh_batch_destroy(h)
+            self._h = None
+
+    def tuning(self):
+        """(threads per block target, largest radix, lines per column tile)."""
+        b, r, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(_lib.pfh_batch_tuning(self._h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(c)))
+        return b.value, r.value, c.value
+
+    def allocate_workspace(self) -> torch.Tensor:
+        return torch.zeros(max(self.workspace_bytes, 256), dtype=torch.uint8, device="cuda")
+
+    def execute(self, op, srcs, dsts=None, workspace=None, use_graph=True):
+        if len(srcs) != len(self.shapes):
+            raise ValueError("one source per problem")
+        if dsts is None:
+            dsts = [torch.empty_like(s) for s in srcs]
+        if workspace is None:
+            if self._ws is None:
+                self._ws = self.allocate_workspace()
+            workspace = self._ws
+        n = len(srcs)
+        sp = (ctypes.c_void_p * n)(*[_ptr(t).value for t in srcs])
+        dp = (ctypes.c_void_p * n)(*[_ptr(t).value for t in dsts])
+        _check(_lib.pfh_batch_execute(self._h, op, sp, dp, _ptr(workspace), workspace.numel(), _stream(),
+                                      int(bool(use_graph))))
+        return dsts
+
+    def forward(self, srcs, dsts=None, workspace=None, use_graph=True):
+        return self.execute(FORWARD, srcs, dsts, workspace, use_graph)
+
+    def inverse(self, srcs, dsts=None, workspace=None, use_graph=True):
+        return self.execute(INVERSE, srcs, dsts, workspace, use_graph)
+
+    def backward(self, srcs, dsts=None, workspace=None, use_graph=True):
+        return self.execute(BACKWARD, srcs, dsts, workspace, use_graph)

@@ -29,6 +29,15 @@ def _host(t):
     return t.cpu().numpy()
 
 
+def test_first_plan_leaves_no_pending_device_error(gpu):
+    """Device set-up (LDS limits of every kernel) happens on the first plan;
+    a failed attribute call must not surface in the caller's next HIP call."""
+    pf.LocalC2cPlan(64, dtype=torch.complex64)
+    t = torch.ones(16, device="cuda") * 2
+    torch.cuda.synchronize()
+    assert float(t.sum().item()) == 32.0
+
+
 @pytest.mark.parametrize("dt", DTYPES)
 @pytest.mark.parametrize("n", LENGTHS)
 def test_local_c2c_matches_oracle(gpu, n, dt):
