@@ -6,10 +6,13 @@ One step = one 3D R2C forward of a 512^3 real N(0,1) field (seed 2) followed
 by the normalized C2R inverse back to real space, inputs resident in HBM,
 through the C ABI (pencil_fft_hip). Prints ONE JSON line.
 
-roofline: the dominant kernel is k_line_stage<float> (every stage of both
-directions). achieved = algorithmic bytes per launch (one read + one write of
-the stage's input and output arrays) / average launch duration, both measured
-here with HIP events on the stream the kernels run on.
+roofline: the dominant kernels are the line-stage launches (k_line_fast
+<float, 16, 16|32, 0, 16> at 512^3: every stage of both directions, 6 per
+round trip). achieved = algorithmic bytes per launch (one read of the stage's
+input array + one write of its output array) / average launch duration, both
+measured here with HIP events on the stream the kernels run on. traffic =
+HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+summary (profiles/line_stages_512cube_f32_pmc.md).
 cpu_baseline: the oracle's f32 restatement of the reference route (per-line
 FFT with copy-in/out, pack / self-exchange / unpack transitions) timed on the
 host for one full round trip ("port").
@@ -37,16 +40,31 @@ def route_model_bytes(n: int, real_bytes: int) -> int:
 
 
 def traffic_from_profiles():
-    """HBM bytes per k_line_stage<float> launch from the committed rocprofv3
-    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "k_line_stage_512cube_f32_pmc.json")
+    """HBM bytes per line-stage launch of the round trip from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration): its line "hbm_bytes_per_launch = N", or None."""
+    path = os.path.join(ROOT, "profiles", "line_stages_512cube_f32_pmc.md")
     try:
         with open(path) as f:
-            d = json.load(f)
-        return float(d["hbm_bytes_per_launch"])
-    except Exception:
-        return None
+            for line in f:
+                if line.strip().startswith("hbm_bytes_per_launch"):
+                    return float(line.split("=")[1].strip().split()[0])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(torch, reps: int = 5):
+    """A plain device copy of a known size (1 GiB read + 1 GiB write per
+    launch): the reference point that scales FETCH_SIZE / WRITE_SIZE."""
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                      "launches": reps}))
 
 
 def cpu_baseline(n: int, threads: int):
@@ -158,6 +176,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-reps", type=int, default=5)
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint comparisons")
+    ap.add_argument("--copy-calibration", action="store_true", help="only run the 1 GiB copy kernel")
+    ap.add_argument("--batched-only", action="store_true", help="only run the batched comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (this build has no "
@@ -170,6 +190,12 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
+    if args.batched_only:
+        print(json.dumps(batched_vs_plain(torch, pf)))
+        return
     n = args.size
     shape = (n, n, n)
     g = torch.Generator(device="cuda")
@@ -244,7 +270,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / PEAK_GBPS,
             "traffic": traffic,
-            "kernel": "k_line_stage<float>",
+            "kernel": "line stages (k_line_fast<float,16,16|32,0,16>)",
             "avg_launch_ms": avg_ms,
             "algorithmic_bytes_per_launch": avg_bytes,
         },
