@@ -170,8 +170,8 @@ def adjoint_vs_plain(torch, pf, plan, x, spec, back, ws, reps: int = 5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-reps", type=int, default=5)
@@ -226,20 +226,23 @@ def main():
     rt_err = float(torch.linalg.norm((back - x).double()) / torch.linalg.norm(x.double()))
 
     # per-kernel HIP-event timing on the same stream
-    stage_ms, stage_bytes, check_ms = [], [], []
+    # (a stage run slab by slab is one entry with several launches)
+    stage_ms, stage_bytes, check_ms, launches = [], [], [], 0
     for _ in range(args.timing_reps):
         _, tf = plan.execute_with_timing(pf.FORWARD, x, spec, ws)
         _, ti = plan.execute_with_timing(pf.INVERSE, spec, back, ws)
         for t in (tf, ti):
-            for axis, t_ms, b in t.entries():
+            for (axis, t_ms, b), nl in zip(t.entries(), t.launches()):
                 if axis < 0:
                     check_ms.append(t_ms)
                 else:
                     stage_ms.append(t_ms)
                     stage_bytes.append(b)
-    avg_ms = sum(stage_ms) / len(stage_ms)
-    avg_bytes = sum(stage_bytes) / len(stage_bytes)
+                    launches += nl
+    avg_ms = sum(stage_ms) / launches
+    avg_bytes = sum(stage_bytes) / launches
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    stage_avg_ms = sum(stage_ms) / len(stage_ms)
     traffic = traffic_from_profiles()
     rm = route_model_bytes(n, 4)
     route_gbps = rm / (ms * 1e-3) / 1e9
@@ -273,6 +276,8 @@ def main():
             "kernel": "line stages (k_line_fast<float,16,16|32,0,16>)",
             "avg_launch_ms": avg_ms,
             "algorithmic_bytes_per_launch": avg_bytes,
+            "launches_per_round_trip": launches / args.timing_reps,
+            "avg_stage_ms": stage_avg_ms,
         },
         "route_model": {
             "bytes_per_round_trip": rm,

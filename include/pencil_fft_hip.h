@@ -154,9 +154,11 @@ typedef struct pfh_desc {
 /* Stage timing (TransformTiming / StageTiming, distributed.rs:290-325). */
 typedef struct pfh_transform_timing {
     int stages;              /* number of entries below */
-    double stage_ms[12];     /* device time of each entry, in execution order */
+    double stage_ms[12];     /* device time of each entry (sum over its launches) */
     int stage_axis[12];      /* transformed logical axis; -1 for an endpoint check */
-    double stage_bytes[12];  /* algorithmic HBM bytes of the entry (one read + one write) */
+    double stage_bytes[12];  /* algorithmic bytes of the entry (one read of its input array + one write of
+                                its output array) */
+    int stage_launches[12];  /* kernel launches of the entry (a stage run slab by slab has several) */
     double total_ms;
 } pfh_transform_timing;
 
@@ -178,7 +180,11 @@ void pfh_plan_destroy(pfh_plan plan);
 
 /* Workspace bytes needed by op (TransposeWorkspaceRequirements,
  * transpose.rs:226-254, scratch_len / embedding_len of the local plans). The
- * workspace also holds the plan's small device status area. */
+ * workspace also holds the plan's small device status area: fill a new
+ * workspace with zeros once before its first use (the library keeps the
+ * status area consistent afterwards). Besides the route's intermediate
+ * arrays it holds the slab intermediate of slab-pipelined stage pairs, which
+ * are off unless PFH_SLAB_MB in the environment sets a slab size in MiB. */
 pfh_status pfh_plan_workspace_bytes(pfh_plan plan, int op, size_t *bytes);
 
 /* Layout of the plan's input (side 0) or output (side 1) array for the

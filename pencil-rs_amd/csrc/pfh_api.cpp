@@ -534,6 +534,22 @@ static int g_column_lines = []() {
     return e ? std::atoi(e) : 0;
 }();
 
+// Tiles and grid of a step whose lines and lines per tile are set.
+static void set_tiles(Step &st, const Tuning &tu)
+{
+    st.tiles = (st.nlines + st.lt - 1) / st.lt;
+    static const int xcd_env = env_int("PFH_XCD", 0);
+    if (st.tiles >= 64 && tu.allow_xcd && xcd_env) {
+        st.flags |= F_XCD_REMAP;
+        st.xcd_per = (int)((st.tiles + 7) / 8);
+        st.grid = 8LL * st.xcd_per;
+    } else {
+        st.flags &= ~F_XCD_REMAP;
+        st.xcd_per = 0;
+        st.grid = st.tiles;
+    }
+}
+
 // Line grid, access patterns and launch shape of a step whose in/out
 // layouts are set.
 static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const Tuning &tu)
@@ -554,11 +570,13 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
     LineGeom g{};
     int nd = 0;
     int64_t nlines = 1;
+    for (int d = 0; d < kMaxDigits; ++d) st.digit_axis[d] = -1;
     for (int b : others) {
         if (nd >= kMaxDigits) return set_err(PFH_ERR_UNSUPPORTED, "too many line digits");
         g.ext[nd] = st.in.shape[b];
         g.is[nd] = sti[b];
         g.os[nd] = sto[b];
+        st.digit_axis[nd] = b;
         nlines *= st.in.shape[b];
         ++nd;
     }
@@ -576,9 +594,11 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
 
     const int cb = 2 * tbytes;
     const int in_eb = st.in_cplx ? cb : tbytes, out_eb = st.out_cplx ? cb : tbytes;
+    // column access: enough lines per tile that each point of the tile is a
+    // 256-byte run (measured faster than 128-byte runs at 512^3 f32)
     int lt_col = 1;
-    if (st.in_col) lt_col = std::max(lt_col, 128 / in_eb);
-    if (st.out_col) lt_col = std::max(lt_col, 128 / out_eb);
+    if (st.in_col) lt_col = std::max(lt_col, 256 / in_eb);
+    if (st.out_col) lt_col = std::max(lt_col, 256 / out_eb);
     if (g_column_lines > 0) lt_col = g_column_lines;
     if (tu.col_lines > 0) lt_col = tu.col_lines;
     bool wide_plan = false;
@@ -596,7 +616,8 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
     }
     if (mode_needs_pre(st.mode)) need = std::max(need, ceil_div(st.m, emax));
     int tpl = pow2ceil(need);
-    const int target = tu.block > 0 ? tu.block : 256;
+    static const int block_env = env_int("PFH_BLOCK", 256);
+    const int target = tu.block > 0 ? tu.block : block_env;
     int lt = (st.in_col || st.out_col) ? lt_col : std::max(1, target / tpl);
     while (lt > 1 && (int64_t)(lt / 2) >= nlines) lt /= 2;
     while (lt * tpl < 64) tpl *= 2;
@@ -619,16 +640,7 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
     st.tpl = tpl;
     st.threads = lt * tpl;
     st.lds_bytes = lds_of(lt);
-    st.tiles = (nlines + lt - 1) / lt;
-    if (st.tiles >= 64 && tu.allow_xcd) {
-        st.flags |= F_XCD_REMAP;
-        st.xcd_per = (int)((st.tiles + 7) / 8);
-        st.grid = 8LL * st.xcd_per;
-    } else {
-        st.flags &= ~F_XCD_REMAP;
-        st.xcd_per = 0;
-        st.grid = st.tiles;
-    }
+    set_tiles(st, tu);
     return PFH_OK;
 }
 
@@ -771,13 +783,17 @@ static pfh_status route_specs(const PlanImpl &P, int op, std::vector<StageSpec> 
 
 // Chooses which side of each stage absorbs each transition and assigns
 // buffers (SRC, DST, W0, W1), minimising workspace. in_place: SRC == DST.
+// Chooses, for every route transition, whether the stage before it
+// absorbs it at its store or the stage after it at its load, and assigns
+// every stage's input and output to SRC / DST / W0 / W1. Preference: the
+// most slab-pipelined pairs (pair_score), then the fewest workspace arrays.
 template <typename Fn>
-static bool assign_route(const std::vector<StageSpec> &specs, bool in_place, int64_t dst_capacity, Fn &&emit,
+static bool assign_route(const std::vector<StageSpec> &specs, bool in_place, int64_t dst_capacity, Fn &&pair_score,
                          std::vector<Layout> &X, std::vector<Layout> &Y, std::vector<int> &inb, std::vector<int> &outb,
                          int64_t wbytes[2])
 {
     const int K = (int)specs.size();
-    int best_w = 99;
+    int best_w = 99, best_pairs = -1;
     std::vector<Layout> bX, bY;
     std::vector<int> bin, bout;
     int64_t bw[2] = {0, 0};
@@ -845,7 +861,9 @@ static bool assign_route(const std::vector<StageSpec> &specs, bool in_place, int
             if (bad) continue;
         }
         const int nw = (wb[0] > 0 ? 1 : 0) + (wb[1] > 0 ? 1 : 0);
-        if (nw < best_w) {
+        const int pairs = pair_score(x, y);
+        if (pairs > best_pairs || (pairs == best_pairs && nw < best_w)) {
+            best_pairs = pairs;
             best_w = nw;
             bX = x;
             bY = y;
@@ -855,7 +873,6 @@ static bool assign_route(const std::vector<StageSpec> &specs, bool in_place, int
             bw[1] = wb[1];
         }
     }
-    (void)emit;
     if (best_w == 99) return false;
     X = bX;
     Y = bY;
@@ -889,6 +906,18 @@ static int64_t status_area_bytes(const PlanImpl &P)
 }
 
 // Builds the steps of one distributed-plan operation.
+// Slab intermediate size: PFH_SLAB_MB (MiB), 0 (off) by default. Slabs of
+// 16-128 MiB keep the intermediate in the 256 MB last-level cache, but on
+// MI355X the slab launches ran 1.6-1.9x slower per byte than one launch per
+// stage at 512^3 (390-438 against 713 round trips/s), so the pipelining is
+// opt-in.
+static int64_t slab_target_bytes(const Tuning &tu)
+{
+    static const int env_mb = env_int("PFH_SLAB_MB", 0);
+    const int mb = tu.slab_mb >= 0 ? tu.slab_mb : env_mb;
+    return (int64_t)mb << 20;
+}
+
 static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
 {
     std::vector<StageSpec> specs;
@@ -901,7 +930,34 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
     std::vector<Layout> X, Y;
     std::vector<int> ib, ob;
     int64_t wb[2];
-    if (!assign_route(specs, in_place, dst_cap, 0, X, Y, ib, ob, wb))
+    // slab pipelining (out of place only): the pair (e, e+1) runs slab by
+    // slab over an axis A that neither stage transforms and that is the
+    // slowest memory axis of stage e's input and of the intermediate array
+    const int64_t target = in_place ? 0 : slab_target_bytes(P.tuning);
+    std::vector<char> checked(K, 0);
+    for (int e = 0; e < K; ++e)
+        checked[e] = (P.kind == PFH_R2C && specs[e].kind == SK_C2R && (op == PFH_INVERSE || op == PFH_BACKWARD));
+    auto slab_axis_of = [&](int e, const Layout &xin, const Layout &mid) -> int {
+        if (target <= 0 || e + 1 >= K || checked[e] || checked[e + 1]) return -1;
+        const int A = xin.perm[0];
+        if (mid.perm[0] != A || A == specs[e].axis || A == specs[e + 1].axis) return -1;
+        const int64_t nA = xin.shape[A];
+        if (nA < 2 || mid.bytes() < 2 * target) return -1;
+        return A;
+    };
+    auto pair_score = [&](const std::vector<Layout> &x, const std::vector<Layout> &y) {
+        int n = 0;
+        for (int e = 0; e + 1 < K;) {
+            if (slab_axis_of(e, x[e], y[e]) >= 0) {
+                ++n;
+                e += 2;
+            } else {
+                ++e;
+            }
+        }
+        return n;
+    };
+    if (!assign_route(specs, in_place, dst_cap, pair_score, X, Y, ib, ob, wb))
         return set_err(PFH_ERR_PREPARATION_FAILED, "no buffer assignment");
     out.steps.clear();
     for (int e = 0; e < K; ++e) {
@@ -958,6 +1014,71 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
             s.ep = ep;
         }
         out.steps.push_back(s);
+    }
+    out.pairs.clear();
+    out.t_bytes = 0;
+    // slab-shaped copies of steps e (input X[e] full, output T) and e + 1
+    // (input T, output Y[e + 1] full) for a slab of extent S along A
+    auto slab_steps = [&](int e, int A, int64_t S, Step &f, Step &g2) -> pfh_status {
+        const int64_t nA = X[e].shape[A];
+        Layout T = Y[e];
+        T.shape[A] = S;
+        f = out.steps[e];
+        f.out = T;
+        pfh_status r = finish_step(f, P.dev, tb, P.tuning);
+        if (r) return r;
+        bool found = false;
+        for (int d = 0; d < f.g.nd; ++d)
+            if (f.digit_axis[d] == A) {
+                f.g.ext[d] = S;
+                found = true;
+            }
+        if (!found) return PFH_ERR_UNSUPPORTED;
+        f.nlines = f.nlines / nA * S;
+        set_tiles(f, P.tuning);
+        g2 = out.steps[e + 1];
+        g2.in = T;
+        return finish_step(g2, P.dev, tb, P.tuning);
+    };
+    for (int e = 0; e + 1 < K;) {
+        const int A = slab_axis_of(e, X[e], Y[e]);
+        const Step &a = out.steps[e], &b = out.steps[e + 1];
+        if (A < 0 || a.in_buf == b.out_buf) {
+            ++e;
+            continue;
+        }
+        // slabs of about `target` intermediate bytes; when the slab axis
+        // groups the lines of a column-access tile, the slab extent is a
+        // multiple of the tile's lines so that no tile straddles two slabs
+        const int64_t nA = X[e].shape[A];
+        const int64_t slice = Y[e].bytes() / nA;
+        const int64_t want = std::max<int64_t>(2, (nA * slice + target - 1) / target);
+        int64_t q = 1;
+        if (a.out_col && a.digit_axis[0] == A) q = std::max<int64_t>(q, a.lt);
+        if ((b.in_col || b.out_col) && b.digit_axis[0] == A) q = std::max<int64_t>(q, b.lt);
+        const int64_t St = (nA + want - 1) / want;
+        const int64_t S = std::max<int64_t>(q, (St + q / 2) / q * q);
+        const int64_t nfull = nA / S, tail = nA - nfull * S;
+        if (nfull + (tail ? 1 : 0) < 2) {
+            ++e;
+            continue;
+        }
+        SlabPair pr;
+        pr.e = e;
+        pr.axis = A;
+        pr.nslab = (int)nfull;
+        pr.extent = S;
+        pr.tail = tail;
+        if (slab_steps(e, A, S, pr.first, pr.second) != PFH_OK ||
+            (tail && slab_steps(e, A, tail, pr.first_tail, pr.second_tail) != PFH_OK)) {
+            ++e;
+            continue;
+        }
+        pr.in_plane_bytes = X[e].strides()[A] * X[e].elem;
+        pr.out_plane_bytes = Y[e + 1].strides()[A] * Y[e + 1].elem;
+        out.t_bytes = std::max(out.t_bytes, slice * S);
+        out.pairs.push_back(pr);
+        e += 2;
     }
     out.in = specs[0].in_natural;
     out.out = specs[K - 1].out_natural;
@@ -1180,11 +1301,18 @@ struct WsView {
     unsigned long long *first_code = nullptr;
     EndpointPartial *ep = nullptr;
     void *w[2] = {nullptr, nullptr};
+    void *t = nullptr; // slab intermediate
 };
+
+// Workspace of one operation: status area, W0, W1, slab intermediate T.
+static int64_t op_ws_bytes(const PlanImpl &P, const OpPlan &o)
+{
+    return status_area_bytes(P) + align256(o.w_bytes[0]) + align256(o.w_bytes[1]) + align256(o.t_bytes);
+}
 
 static pfh_status view_workspace(const PlanImpl &P, const OpPlan &o, void *ws, size_t ws_bytes, WsView &v)
 {
-    const int64_t need = status_area_bytes(P) + align256(o.w_bytes[0]) + align256(o.w_bytes[1]);
+    const int64_t need = op_ws_bytes(P, o);
     if (ws == nullptr || (int64_t)ws_bytes < need)
         return set_err(PFH_ERR_WORKSPACE_TOO_SMALL, "workspace too small", (size_t)need, ws ? ws_bytes : 0);
     char *b = static_cast<char *>(ws);
@@ -1195,6 +1323,7 @@ static pfh_status view_workspace(const PlanImpl &P, const OpPlan &o, void *ws, s
     char *w = b + status_area_bytes(P);
     v.w[0] = w;
     v.w[1] = w + align256(o.w_bytes[0]);
+    v.t = w + align256(o.w_bytes[0]) + align256(o.w_bytes[1]);
     return PFH_OK;
 }
 
@@ -1242,18 +1371,22 @@ static StageArgs<T> fill_args(const PlanImpl &P, const Step &s, void *const bufs
 }
 
 template <typename T>
-static pfh_status launch_step(const PlanImpl &P, const Step &s, void *const bufs[4], const WsView &wv,
-                              hipStream_t stream)
+static pfh_status launch_step_ptrs(const PlanImpl &P, const Step &s, const void *in, void *out, const WsView &wv,
+                                   hipStream_t stream)
 {
     if (s.endpoint_check) {
-        PFH_HIP_CHECK(launch_endpoint_check<T>(s.ep, bufs[s.in_buf], wv.ep, wv.gate, wv.status, stream));
+        PFH_HIP_CHECK(launch_endpoint_check<T>(s.ep, in, wv.ep, wv.gate, wv.status, stream));
     }
     if (s.local_endpoint_check) {
-        PFH_HIP_CHECK(launch_local_endpoint_check<T>(bufs[s.in_buf], s.nlines, s.nin, (s.n % 2 == 0) ? 1 : 0,
-                                                     wv.first_code, wv.gate, wv.status, stream));
+        PFH_HIP_CHECK(launch_local_endpoint_check<T>(in, s.nlines, s.nin, (s.n % 2 == 0) ? 1 : 0, wv.first_code,
+                                                     wv.gate, wv.status, stream));
     }
+    void *io[4] = {const_cast<void *>(in), out, nullptr, nullptr};
+    Step sp = s;
+    sp.in_buf = 0;
+    sp.out_buf = 1;
     const StageArgs<T> a =
-        fill_args<T>(P, s, bufs, (s.endpoint_check || s.local_endpoint_check || s.gated) ? wv.gate : nullptr);
+        fill_args<T>(P, sp, io, (s.endpoint_check || s.local_endpoint_check || s.gated) ? wv.gate : nullptr);
     if (s.fast) {
         PFH_HIP_CHECK(launch_line_fast<T>(a, s.fr[0], s.fr[1], s.fr[2], s.fem, s.threads, s.lds_bytes, s.grid, stream));
         return PFH_OK;
@@ -1264,25 +1397,93 @@ static pfh_status launch_step(const PlanImpl &P, const Step &s, void *const bufs
     return PFH_OK;
 }
 
-// Timing entries of one execution: an event after every endpoint check and
-// every stage kernel.
+template <typename T>
+static pfh_status launch_step(const PlanImpl &P, const Step &s, void *const bufs[4], const WsView &wv,
+                              hipStream_t stream)
+{
+    return launch_step_ptrs<T>(P, s, bufs[s.in_buf], bufs[s.out_buf], wv, stream);
+}
+
+static pfh_status launch_any(const PlanImpl &P, const Step &s, const void *in, void *out, const WsView &wv,
+                             hipStream_t stream)
+{
+    return (P.precision == PFH_F32) ? launch_step_ptrs<float>(P, s, in, out, wv, stream)
+                                    : launch_step_ptrs<double>(P, s, in, out, wv, stream);
+}
+
+// Device timing of an execution: events recorded between launches; each
+// entry (one stage, or one endpoint check) sums its launch intervals.
 struct TimingTrace {
+    struct Entry {
+        int axis = -1;
+        double bytes = 0;
+        std::vector<int> iv; // interval i spans ev[i - 1] .. ev[i]
+    };
     std::vector<hipEvent_t> ev; // ev[0] recorded before the first launch
-    std::vector<int> axis;
-    std::vector<double> bytes;
+    std::vector<Entry> entries;
+    pfh_status mark(hipStream_t stream, int &idx)
+    {
+        hipEvent_t e;
+        PFH_HIP_CHECK(hipEventCreate(&e));
+        ev.push_back(e);
+        PFH_HIP_CHECK(hipEventRecord(e, stream));
+        idx = (int)ev.size() - 1;
+        return PFH_OK;
+    }
 };
+
+static double step_bytes(const Step &s)
+{
+    return (double)s.nlines * ((double)s.nin * s.in.elem + (double)s.nout * s.out.elem);
+}
 
 static pfh_status run_op(const PlanImpl &P, const OpPlan &o, void *const bufs[4], const WsView &wv,
                          hipStream_t stream, TimingTrace *tr)
 {
+    size_t pi = 0;
     for (size_t i = 0; i < o.steps.size(); ++i) {
         const Step &s = o.steps[i];
+        int idx = 0;
+        if (pi < o.pairs.size() && o.pairs[pi].e == (int)i) {
+            // slab-pipelined pair: steps i and i + 1 through the slab region
+            const SlabPair &pr = o.pairs[pi++];
+            const char *in0 = static_cast<const char *>(bufs[pr.first.in_buf]);
+            char *out0 = static_cast<char *>(bufs[pr.second.out_buf]);
+            TimingTrace::Entry ea, eb;
+            ea.axis = pr.first.axis;
+            eb.axis = pr.second.axis;
+            const int ns = pr.nslab + (pr.tail ? 1 : 0);
+            for (int k = 0; k < ns; ++k) {
+                const int64_t a0 = (int64_t)k * pr.extent; // first plane of the slab
+                const bool part = k == pr.nslab;
+                const Step &f = part ? pr.first_tail : pr.first;
+                const Step &g2 = part ? pr.second_tail : pr.second;
+                pfh_status st = launch_any(P, f, in0 + a0 * pr.in_plane_bytes, wv.t, wv, stream);
+                if (st) return st;
+                if (tr) {
+                    if (tr->mark(stream, idx)) return PFH_ERR_DEVICE;
+                    ea.iv.push_back(idx);
+                    ea.bytes += step_bytes(f);
+                }
+                st = launch_any(P, g2, wv.t, out0 + a0 * pr.out_plane_bytes, wv, stream);
+                if (st) return st;
+                if (tr) {
+                    if (tr->mark(stream, idx)) return PFH_ERR_DEVICE;
+                    eb.iv.push_back(idx);
+                    eb.bytes += step_bytes(g2);
+                }
+            }
+            if (tr) {
+                tr->entries.push_back(ea);
+                tr->entries.push_back(eb);
+            }
+            ++i;
+            continue;
+        }
         if (tr && (s.endpoint_check || s.local_endpoint_check)) {
             // time the check on its own, then the stage with its gate
             Step chk = s;
             chk.gated = true;
-            hipEvent_t e;
-            PFH_HIP_CHECK(hipEventCreate(&e));
             if (P.precision == PFH_F32) {
                 if (s.endpoint_check)
                     PFH_HIP_CHECK(launch_endpoint_check<float>(s.ep, bufs[s.in_buf], wv.ep, wv.gate, wv.status, stream));
@@ -1290,26 +1491,26 @@ static pfh_status run_op(const PlanImpl &P, const OpPlan &o, void *const bufs[4]
                 if (s.endpoint_check)
                     PFH_HIP_CHECK(launch_endpoint_check<double>(s.ep, bufs[s.in_buf], wv.ep, wv.gate, wv.status, stream));
             }
-            PFH_HIP_CHECK(hipEventRecord(e, stream));
-            tr->ev.push_back(e);
-            tr->axis.push_back(-1);
-            tr->bytes.push_back((double)(s.ep.extra * s.ep.planes * s.ep.outer * s.ep.stride) * (double)s.in.elem);
+            if (tr->mark(stream, idx)) return PFH_ERR_DEVICE;
+            TimingTrace::Entry ec;
+            ec.axis = -1;
+            ec.bytes = (double)(s.ep.extra * s.ep.planes * s.ep.outer * s.ep.stride) * (double)s.in.elem;
+            ec.iv.push_back(idx);
+            tr->entries.push_back(ec);
             chk.endpoint_check = false;
-            pfh_status st = (P.precision == PFH_F32) ? launch_step<float>(P, chk, bufs, wv, stream)
-                                                     : launch_step<double>(P, chk, bufs, wv, stream);
+            pfh_status st = launch_any(P, chk, bufs[s.in_buf], bufs[s.out_buf], wv, stream);
             if (st) return st;
         } else {
-            pfh_status st = (P.precision == PFH_F32) ? launch_step<float>(P, s, bufs, wv, stream)
-                                                     : launch_step<double>(P, s, bufs, wv, stream);
+            pfh_status st = launch_any(P, s, bufs[s.in_buf], bufs[s.out_buf], wv, stream);
             if (st) return st;
         }
         if (tr) {
-            hipEvent_t e;
-            PFH_HIP_CHECK(hipEventCreate(&e));
-            PFH_HIP_CHECK(hipEventRecord(e, stream));
-            tr->ev.push_back(e);
-            tr->axis.push_back(s.axis);
-            tr->bytes.push_back((double)s.nlines * ((double)s.nin * s.in.elem + (double)s.nout * s.out.elem));
+            if (tr->mark(stream, idx)) return PFH_ERR_DEVICE;
+            TimingTrace::Entry es;
+            es.axis = s.axis;
+            es.bytes = step_bytes(s);
+            es.iv.push_back(idx);
+            tr->entries.push_back(es);
         }
     }
     return PFH_OK;
@@ -1399,24 +1600,32 @@ static pfh_status execute_impl(PlanImpl &P, int op, const void *src, size_t src_
     void *bufs[4] = {const_cast<void *>(src), dst, wv.w[0], wv.w[1]};
     TimingTrace tr;
     if (timing) {
-        hipEvent_t e0;
-        PFH_HIP_CHECK(hipEventCreate(&e0));
-        PFH_HIP_CHECK(hipEventRecord(e0, stream));
-        tr.ev.push_back(e0);
+        int idx = 0;
+        pfh_status e0 = tr.mark(stream, idx);
+        if (e0) return e0;
     }
     pfh_status st = run_op(P, o, bufs, wv, stream, timing ? &tr : nullptr);
-    if (st) return st;
+    if (st) {
+        for (auto &e : tr.ev) (void)hipEventDestroy(e);
+        return st;
+    }
     if (timing) {
         PFH_HIP_CHECK(hipEventSynchronize(tr.ev.back()));
-        const int ne = (int)tr.axis.size();
+        const int ne = (int)tr.entries.size();
         timing->stages = ne < 12 ? ne : 12;
         timing->total_ms = 0;
         for (int i = 0; i < timing->stages; ++i) {
-            float ms = 0;
-            PFH_HIP_CHECK(hipEventElapsedTime(&ms, tr.ev[i], tr.ev[i + 1]));
+            const TimingTrace::Entry &en = tr.entries[i];
+            double ms = 0;
+            for (int j : en.iv) {
+                float f = 0;
+                PFH_HIP_CHECK(hipEventElapsedTime(&f, tr.ev[j - 1], tr.ev[j]));
+                ms += f;
+            }
             timing->stage_ms[i] = ms;
-            timing->stage_axis[i] = tr.axis[i];
-            timing->stage_bytes[i] = tr.bytes[i];
+            timing->stage_axis[i] = en.axis;
+            timing->stage_bytes[i] = en.bytes;
+            timing->stage_launches[i] = (int)en.iv.size();
             timing->total_ms += ms;
         }
         for (auto &e : tr.ev) (void)hipEventDestroy(e);
@@ -1498,8 +1707,8 @@ pfh_status pfh_plan_workspace_bytes(pfh_plan plan, int op, size_t *bytes)
     }
     const OpPlan &o = P.ops[op];
     const OpPlan &oi = P.ops_inplace[op];
-    int64_t b = status_area_bytes(P) + align256(o.w_bytes[0]) + align256(o.w_bytes[1]);
-    if (oi.valid) b = std::max(b, status_area_bytes(P) + align256(oi.w_bytes[0]) + align256(oi.w_bytes[1]));
+    int64_t b = op_ws_bytes(P, o);
+    if (oi.valid) b = std::max(b, op_ws_bytes(P, oi));
     if (P.kind == PFH_LOCAL_R2C || P.kind == PFH_LOCAL_C2C || P.kind == PFH_LOCAL_R2R || P.kind == PFH_LOCAL_DHT) {
         // in-place local execution stages the input here
         (void)0;
@@ -1762,7 +1971,7 @@ static pfh_status batch_plan(pfh_batch_s &B, const pfh_desc *descs, int count, c
                     return set_err(PFH_ERR_UNSUPPORTED, "batched problems must select every axis");
                 threads = std::max(threads, s.threads);
             }
-            need = std::max(need, status_area_bytes(*P) + align256(o.w_bytes[0]) + align256(o.w_bytes[1]));
+            need = std::max(need, op_ws_bytes(*P, o));
         }
         B.ws_off[i] = off;
         off += align256(need);

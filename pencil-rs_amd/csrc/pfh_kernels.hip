@@ -989,20 +989,6 @@ __global__ __launch_bounds__(512) void k_line_fast(const StageArgs<T> a)
     fast_last<T, RL, EM, M>(a, data + fl * a.ls, tws + a.twoff[R2 ? 2 : 1], valid, oo, ft, tpl, mode == M_C2R);
 }
 
-// Instantiated fast plans: {R0, R1, R2, EM}.
-#define PFH_FAST_PLANS(X)      \
-    X(float, 8, 8, 0, 16)      \
-    X(float, 8, 16, 0, 16)     \
-    X(float, 16, 16, 0, 16)    \
-    X(float, 16, 32, 0, 16)    \
-    X(float, 16, 32, 0, 32)    \
-    X(float, 8, 8, 8, 16)      \
-    X(float, 32, 32, 0, 32)    \
-    X(double, 8, 8, 0, 8)      \
-    X(double, 4, 4, 8, 8)      \
-    X(double, 4, 8, 8, 8)      \
-    X(double, 8, 8, 8, 8)
-
 template <typename T>
 hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int em, int threads, size_t lds_bytes,
                             int64_t grid, hipStream_t s)

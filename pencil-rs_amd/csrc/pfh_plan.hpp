@@ -88,6 +88,24 @@ struct Step {
     bool fast = false;
     int fr[3] = {0, 0, 0};
     int fem = 16;
+    int digit_axis[kMaxDigits] = {-1, -1, -1, -1, -1}; // logical axis of each line digit (-1: extras)
+};
+
+// Two consecutive steps run slab by slab over an axis that neither
+// transforms and that is the slowest memory axis of the first step's input
+// and of the intermediate array: the intermediate of one slab lives in a
+// small workspace region (T) that stays in the device's last-level cache
+// between the two launches, instead of a full array in HBM.
+struct SlabPair {
+    int e = -1;                   // first step index (the pair is steps e, e+1)
+    int axis = -1;                // slab axis
+    int nslab = 0;                // number of full slabs
+    int64_t extent = 0;           // planes of a full slab
+    int64_t tail = 0;             // planes of the last, partial slab (0: none)
+    Step first, second;           // slab-shaped steps of a full slab (first writes T, second reads T)
+    Step first_tail, second_tail; // the same for the partial slab
+    int64_t in_plane_bytes = 0;   // source advance per plane of the slab axis (first step's input)
+    int64_t out_plane_bytes = 0;  // destination advance per plane (second step's output)
 };
 
 struct OpPlan {
@@ -95,6 +113,8 @@ struct OpPlan {
     std::vector<Step> steps;
     Layout in, out;          // layouts of src and dst for this op
     int64_t w_bytes[2] = {0, 0};
+    std::vector<SlabPair> pairs; // slab-pipelined step pairs (disjoint, ascending e)
+    int64_t t_bytes = 0;         // slab intermediate region of the workspace
 };
 
 struct DeviceInfo {
@@ -111,6 +131,7 @@ struct Tuning {
     int block_cap = 0;   // upper bound on threads per block (0: the kernel's launch bound)
     bool allow_fast = true;
     bool allow_xcd = true;
+    int slab_mb = -1;    // slab intermediate target in MiB (0: no slab pipelining, -1: default)
 };
 
 struct PlanImpl {

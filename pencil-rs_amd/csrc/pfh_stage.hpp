@@ -97,4 +97,18 @@ struct StageArgs {
     T prew0, prew1, postw0, postw1;
 };
 
+// Instantiated fast plans: {R0, R1, R2, EM}.
+#define PFH_FAST_PLANS(X)      \
+    X(float, 8, 8, 0, 16)      \
+    X(float, 8, 16, 0, 16)     \
+    X(float, 16, 16, 0, 16)    \
+    X(float, 16, 32, 0, 16)    \
+    X(float, 16, 32, 0, 32)    \
+    X(float, 8, 8, 8, 16)      \
+    X(float, 32, 32, 0, 32)    \
+    X(double, 8, 8, 0, 8)      \
+    X(double, 4, 4, 8, 8)      \
+    X(double, 4, 8, 8, 8)      \
+    X(double, 8, 8, 8, 8)
+
 } // namespace pfh

@@ -109,12 +109,17 @@ class TransformTiming(ctypes.Structure):
         ("stage_ms", ctypes.c_double * 12),
         ("stage_axis", ctypes.c_int * 12),
         ("stage_bytes", ctypes.c_double * 12),
+        ("stage_launches", ctypes.c_int * 12),
         ("total_ms", ctypes.c_double),
     ]
 
     def entries(self):
         """[(axis, ms, bytes)] in execution order; axis -1 is an endpoint check."""
         return [(self.stage_axis[i], self.stage_ms[i], self.stage_bytes[i]) for i in range(self.stages)]
+
+    def launches(self):
+        """Kernel launches of each entry (a stage run slab by slab has several)."""
+        return [self.stage_launches[i] for i in range(self.stages)]
 
 
 _lib = None

@@ -1,0 +1,736 @@
+// Host emulation of the HIP runtime and of the library's kernel launchers,
+// linked with pencil-rs_amd/csrc/pfh_api.cpp (the planner, compiled by g++)
+// to check the planner on a machine without a GPU:
+//  - every launch's shape against the kernel it targets (threads, LDS,
+//    grid, tiles, threads per line against each radix pass);
+//  - every global address a launch would read or write lies inside one
+//    allocation, and no element is written twice by one launch;
+//  - the layout semantics of a whole operation: with each line transform
+//    replaced by the identity on its points (R2C keeps the first n/2+1
+//    points, C2R zero-fills), the output array must hold the input value at
+//    the same logical coordinates, so routes, fused transitions and slab
+//    pipelining are checked end to end.
+// The FFT arithmetic itself is not emulated (the GPU tests check it).
+#include <hip/hip_runtime_api.h>
+
+#include <cinttypes>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pencil_fft_hip.h"
+#include "../../pencil-rs_amd/csrc/pfh_kernels.hpp"
+
+namespace {
+
+int g_fail = 0;
+
+void fail(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    std::fprintf(stdout, "FAIL: ");
+    std::vfprintf(stdout, fmt, ap);
+    std::fprintf(stdout, "\n");
+    va_end(ap);
+    ++g_fail;
+    if (g_fail > 20) std::exit(1);
+}
+
+struct Alloc {
+    size_t bytes;
+    std::vector<uint8_t> writes; // writes per byte in the current launch
+};
+std::map<uintptr_t, Alloc> g_allocs;
+
+const size_t kLdsLimit = 163840;
+
+// allocation containing [p, p + n)
+Alloc *find_alloc(const void *p, size_t n, uintptr_t *base)
+{
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    auto it = g_allocs.upper_bound(a);
+    if (it == g_allocs.begin()) return nullptr;
+    --it;
+    if (a + n > it->first + it->second.bytes) return nullptr;
+    *base = it->first;
+    return &it->second;
+}
+
+void begin_launch()
+{
+    for (auto &kv : g_allocs) std::fill(kv.second.writes.begin(), kv.second.writes.end(), 0);
+}
+
+bool check_read(const char *what, const void *p, size_t n)
+{
+    uintptr_t base;
+    if (!find_alloc(p, n, &base)) {
+        fail("%s: read of %zu bytes at %p outside every allocation", what, n, p);
+        return false;
+    }
+    return true;
+}
+
+bool check_write(const char *what, void *p, size_t n)
+{
+    uintptr_t base;
+    Alloc *a = find_alloc(p, n, &base);
+    if (!a) {
+        fail("%s: write of %zu bytes at %p outside every allocation", what, n, p);
+        return false;
+    }
+    const size_t off = reinterpret_cast<uintptr_t>(p) - base;
+    for (size_t i = 0; i < n; ++i) {
+        if (a->writes[off + i]) {
+            fail("%s: byte written twice in one launch (offset %zu)", what, off + i);
+            return false;
+        }
+        a->writes[off + i] = 1;
+    }
+    return true;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// HIP runtime stubs (host memory stands in for device memory)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+hipError_t hipMalloc(void **p, size_t n)
+{
+    const size_t sz = n ? n : 1;
+    void *q = std::aligned_alloc(256, (sz + 255) / 256 * 256);
+    if (!q) return hipErrorOutOfMemory;
+    std::memset(q, 0xA5, sz);
+    Alloc a;
+    a.bytes = sz;
+    a.writes.assign(sz, 0);
+    g_allocs[reinterpret_cast<uintptr_t>(q)] = std::move(a);
+    *p = q;
+    return hipSuccess;
+}
+hipError_t hipFree(void *p)
+{
+    if (!p) return hipSuccess;
+    g_allocs.erase(reinterpret_cast<uintptr_t>(p));
+    std::free(p);
+    return hipSuccess;
+}
+hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind)
+{
+    std::memmove(d, s, n);
+    return hipSuccess;
+}
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t)
+{
+    std::memmove(d, s, n);
+    return hipSuccess;
+}
+hipError_t hipMemset(void *d, int v, size_t n)
+{
+    std::memset(d, v, n);
+    return hipSuccess;
+}
+hipError_t hipGetDevice(int *d)
+{
+    *d = 0;
+    return hipSuccess;
+}
+hipError_t hipGetDevicePropertiesR0600(hipDeviceProp_tR0600 *p, int)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->sharedMemPerBlock = kLdsLimit;
+    p->multiProcessorCount = 256;
+    return hipSuccess;
+}
+const char *hipGetErrorString(hipError_t) { return "emulated error"; }
+hipError_t hipGetLastError(void) { return hipSuccess; }
+static int g_dummy[64];
+hipError_t hipEventCreate(hipEvent_t *e)
+{
+    *e = reinterpret_cast<hipEvent_t>(&g_dummy[0]);
+    return hipSuccess;
+}
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
+hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t)
+{
+    *ms = 1.0f;
+    return hipSuccess;
+}
+hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipStreamCreate(hipStream_t *s)
+{
+    *s = reinterpret_cast<hipStream_t>(&g_dummy[1]);
+    return hipSuccess;
+}
+hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { return hipStreamCreate(s); }
+hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+// Graphs: launches run when issued; a captured graph replays nothing, so
+// the emulation only checks the first (capturing) execution.
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipSuccess; }
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g)
+{
+    *g = reinterpret_cast<hipGraph_t>(&g_dummy[2]);
+    return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t, hipGraphNode_t *, char *, size_t)
+{
+    *e = reinterpret_cast<hipGraphExec_t>(&g_dummy[3]);
+    return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipSuccess; }
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// kernel launcher emulation
+// ---------------------------------------------------------------------------
+namespace pfh {
+
+static bool mode_cin(int m) { return m == M_COPY_C || m == M_C2C || m == M_C2R || m == M_C2R_ODD; }
+static bool mode_cout(int m) { return m == M_COPY_C || m == M_C2C || m == M_R2C || m == M_R2C_ODD; }
+static bool mode_pre(int m)
+{
+    return !(m == M_COPY_C || m == M_COPY_R || m == M_C2C || m == M_R2C || m == M_DHT);
+}
+
+static void offsets(const LineGeom &g, int64_t line, int64_t &io, int64_t &oo)
+{
+    io = oo = 0;
+    for (int d = 0; d < g.nd; ++d) {
+        const int64_t q = line / g.ext[d], r = line - q * g.ext[d];
+        io += r * g.is[d];
+        oo += r * g.os[d];
+        line = q;
+    }
+}
+
+// shape checks shared by the single-stage kernels and the batch items
+template <typename T>
+static void check_shape(const char *what, const StageArgs<T> &a, int threads, size_t lds, int emax, bool fast,
+                        int fem)
+{
+    const int cb = 2 * (int)sizeof(T);
+    if (a.lt * a.tpl > threads) fail("%s: lt*tpl %d > threads %d", what, a.lt * a.tpl, threads);
+    if ((size_t)(a.twn + (int64_t)a.lt * a.ls) * cb > lds) fail("%s: LDS %zu too small", what, lds);
+    if (lds > kLdsLimit - 256) fail("%s: LDS %zu above the budget", what, lds);
+    if (a.ntiles * a.lt < a.nlines) fail("%s: tiles do not cover the lines", what);
+    int need_c = a.m > 0 ? a.m : 1;
+    need_c = std::max(need_c, mode_cin(a.mode) ? a.nin : (a.nin + 1) / 2);
+    need_c = std::max(need_c, mode_cout(a.mode) ? a.nout : (a.nout + 1) / 2);
+    if (a.ls < need_c) fail("%s: line stride %d < %d", what, a.ls, need_c);
+    int prod = 1;
+    for (int p = 0; p < a.npass; ++p) {
+        const int R = a.radix[p];
+        prod *= R;
+        if (a.twoff[p] < 0 || a.twoff[p] > a.twn) fail("%s: pass %d table offset", what, p);
+        const int em = fast ? fem : emax;
+        const bool generic = !(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16 || R == 32 || R == 7 ||
+                               R == 11 || R == 13);
+        if (generic) {
+            if ((int64_t)a.tpl * em < a.m) fail("%s: pass %d (generic radix %d) not covered", what, p, R);
+        } else {
+            const int MB = std::max(1, em / R);
+            if ((int64_t)a.tpl * MB < a.m / R) fail("%s: pass %d (radix %d) not covered", what, p, R);
+        }
+    }
+    if (a.m > 0 && prod != a.m) fail("%s: radix product %d != m %d", what, prod, a.m);
+    if (mode_pre(a.mode) && (int64_t)a.tpl * (fast ? fem : emax) < a.m) fail("%s: pre-processing not covered", what);
+}
+
+// identity-on-points emulation of one line stage
+template <typename T>
+static void emulate(const char *what, const StageArgs<T> &a)
+{
+    if (a.gate && *a.gate) return;
+    const bool ci = mode_cin(a.mode), co = mode_cout(a.mode);
+    const size_t ei = ci ? 2 * sizeof(T) : sizeof(T), eo = co ? 2 * sizeof(T) : sizeof(T);
+    std::vector<T> re(a.nin), im(a.nin);
+    for (int64_t line = 0; line < a.nlines; ++line) {
+        int64_t io, oo;
+        offsets(a.g, line, io, oo);
+        for (int j = 0; j < a.nin; ++j) {
+            const char *p = static_cast<const char *>(a.src) + (io + (int64_t)j * a.ips) * (int64_t)ei;
+            if (!check_read(what, p, ei)) return;
+            const T *v = reinterpret_cast<const T *>(p);
+            re[j] = v[0];
+            im[j] = ci ? v[1] : (T)0;
+        }
+        for (int k = 0; k < a.nout; ++k) {
+            char *p = static_cast<char *>(a.dst) + (oo + (int64_t)k * a.ops) * (int64_t)eo;
+            if (!check_write(what, p, eo)) return;
+            T *v = reinterpret_cast<T *>(p);
+            const T r = k < a.nin ? re[k] : (T)0, i = k < a.nin ? im[k] : (T)0;
+            v[0] = r;
+            if (co) v[1] = (a.mode == M_R2C || a.mode == M_R2C_ODD) ? (T)0 : i;
+        }
+    }
+}
+
+template <typename T>
+static void check_grid(const char *what, const StageArgs<T> &a, int64_t grid)
+{
+    if (a.flags & F_XCD_REMAP) {
+        if (grid != 8LL * a.xcd_per || 8LL * a.xcd_per < a.ntiles) fail("%s: XCD grid", what);
+    } else if (grid < a.ntiles) {
+        fail("%s: grid %" PRId64 " < tiles %" PRId64, what, grid, a.ntiles);
+    }
+}
+
+int g_launches = 0;
+static const bool g_trace = std::getenv("PFH_EMU_TRACE") != nullptr;
+
+template <typename T>
+static void trace(const char *what, const StageArgs<T> &a, int threads, int64_t grid)
+{
+    if (!g_trace) return;
+    std::printf("  %s mode %d n %d m %d nlines %" PRId64 " lt %d tpl %d threads %d tiles %" PRId64 " grid %" PRId64
+                " ips %" PRId64 " ops %" PRId64 " col %d/%d flags %d digits",
+                what, a.mode, a.n, a.m, a.nlines, a.lt, a.tpl, threads, a.ntiles, grid, a.ips, a.ops, a.in_col,
+                a.out_col, a.flags);
+    for (int d = 0; d < a.g.nd; ++d)
+        std::printf(" [%" PRId64 ":%" PRId64 "/%" PRId64 "]", a.g.ext[d], a.g.is[d], a.g.os[d]);
+    std::printf("\n");
+}
+
+template <typename T>
+hipError_t launch_line_stage(const StageArgs<T> &a, bool wide, int threads, size_t lds_bytes, int64_t grid,
+                             hipStream_t)
+{
+    ++g_launches;
+    begin_launch();
+    if (threads > (wide ? kMaxThreadsWide : kMaxThreads)) fail("line_stage: %d threads", threads);
+    trace("line_stage", a, threads, grid);
+    check_shape("line_stage", a, threads, lds_bytes, Regs<T>::emax, false, 0);
+    check_grid("line_stage", a, grid);
+    if (a.twn > 0) check_read("line_stage tables", a.tw, (size_t)a.twn * 2 * sizeof(T));
+    emulate("line_stage", a);
+    return hipSuccess;
+}
+template hipError_t launch_line_stage<float>(const StageArgs<float> &, bool, int, size_t, int64_t, hipStream_t);
+template hipError_t launch_line_stage<double>(const StageArgs<double> &, bool, int, size_t, int64_t, hipStream_t);
+
+template <typename T>
+hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int em, int threads, size_t lds_bytes,
+                            int64_t grid, hipStream_t)
+{
+    ++g_launches;
+    begin_launch();
+    if (threads > 512) fail("line_fast: %d threads", threads);
+    if (!fast_plan_available((int)sizeof(T), r0, r1, r2, em)) fail("line_fast: plan not instantiated");
+    if (a.radix[0] != r0 || a.radix[1] != r1 || (r2 && a.radix[2] != r2)) fail("line_fast: radices differ");
+    if (!(a.mode == M_C2C || a.mode == M_R2C || a.mode == M_C2R)) fail("line_fast: mode %d", a.mode);
+    trace("line_fast", a, threads, grid);
+    check_shape("line_fast", a, threads, lds_bytes, Regs<T>::emax, true, em);
+    if (a.twn > 0) check_read("line_fast tables", a.tw, (size_t)a.twn * 2 * sizeof(T));
+    check_grid("line_fast", a, grid);
+    emulate("line_fast", a);
+    return hipSuccess;
+}
+template hipError_t launch_line_fast<float>(const StageArgs<float> &, int, int, int, int, int, size_t, int64_t,
+                                            hipStream_t);
+template hipError_t launch_line_fast<double>(const StageArgs<double> &, int, int, int, int, int, size_t, int64_t,
+                                             hipStream_t);
+
+bool fast_plan_available(int tbytes, int r0, int r1, int r2, int em)
+{
+#define PFH_FAST_HAS(TT, A, B, C, E) \
+    if ((int)sizeof(TT) == tbytes && r0 == A && r1 == B && r2 == C && em == E) return true;
+    PFH_FAST_PLANS(PFH_FAST_HAS)
+#undef PFH_FAST_HAS
+    return false;
+}
+
+template <typename T>
+hipError_t launch_line_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total,
+                             int threads, size_t lds_bytes, int64_t grid, hipStream_t)
+{
+    ++g_launches;
+    begin_launch();
+    if (threads > kMaxThreadsWide) fail("line_batch: %d threads", threads);
+    if (grid <= 0 || grid > total) fail("line_batch: grid %" PRId64 " for %" PRId64 " tiles", grid, total);
+    check_read("line_batch items", items, sizeof(StageArgs<T>) * nitems);
+    check_read("line_batch prefix", prefix, sizeof(int64_t) * (nitems + 1));
+    if (prefix[0] != 0 || prefix[nitems] != total) fail("line_batch: prefix ends");
+    for (int q = 0; q < nitems; ++q) {
+        const StageArgs<T> &a = items[q];
+        if (prefix[q + 1] - prefix[q] != a.ntiles) fail("line_batch: item %d tiles", q);
+        if (a.mode != M_C2C) fail("line_batch: item %d mode", q);
+        check_shape("line_batch item", a, threads, lds_bytes, Regs<T>::emax, false, 0);
+        emulate("line_batch item", a);
+    }
+    return hipSuccess;
+}
+template hipError_t launch_line_batch<float>(const StageArgs<float> *, const int64_t *, int, int64_t, int, size_t,
+                                             int64_t, hipStream_t);
+template hipError_t launch_line_batch<double>(const StageArgs<double> *, const int64_t *, int, int64_t, int, size_t,
+                                              int64_t, hipStream_t);
+
+template <typename T>
+int line_batch_blocks_per_cu(int, size_t)
+{
+    return 2;
+}
+template int line_batch_blocks_per_cu<float>(int, size_t);
+template int line_batch_blocks_per_cu<double>(int, size_t);
+
+hipError_t set_line_stage_lds_limit(size_t) { return hipSuccess; }
+hipError_t set_fast_lds_limit(size_t) { return hipSuccess; }
+
+template <typename T>
+hipError_t launch_endpoint_check(const EndpointArgs &e, const void *data, EndpointPartial *part, int *gate,
+                                 int *status, hipStream_t)
+{
+    ++g_launches;
+    begin_launch();
+    const int64_t count = e.outer * e.stride;
+    for (int64_t b = 0; b < e.extra; ++b)
+        for (int p = 0; p < e.planes; ++p) {
+            const int64_t k = p == 0 ? 0 : e.clen - 1;
+            for (int64_t i = 0; i < count; ++i) {
+                const int64_t o = i / e.stride, q = i - o * e.stride;
+                const char *v = static_cast<const char *>(data) +
+                                (b * e.len + k * e.stride + o * e.clen * e.stride + q) * 2 * (int64_t)sizeof(T);
+                if (!check_read("endpoint_check", v, 2 * sizeof(T))) return hipSuccess;
+            }
+        }
+    check_write("endpoint_check partials", part, sizeof(EndpointPartial) * e.extra * e.planes * e.nblk);
+    check_read("endpoint_check status", status, sizeof(int));
+    *gate = 0; // accept
+    return hipSuccess;
+}
+template hipError_t launch_endpoint_check<float>(const EndpointArgs &, const void *, EndpointPartial *, int *, int *,
+                                                 hipStream_t);
+template hipError_t launch_endpoint_check<double>(const EndpointArgs &, const void *, EndpointPartial *, int *,
+                                                  int *, hipStream_t);
+
+template <typename T>
+hipError_t launch_local_endpoint_check(const void *data, int64_t lines, int64_t clen, int nyq,
+                                       unsigned long long *first, int *gate, int *status, hipStream_t)
+{
+    ++g_launches;
+    (void)nyq;
+    check_read("local_endpoint_check", data, (size_t)(lines * clen) * 2 * sizeof(T));
+    check_read("local_endpoint_check first", first, sizeof(*first));
+    check_read("local_endpoint_check status", status, sizeof(int));
+    *gate = 0;
+    return hipSuccess;
+}
+template hipError_t launch_local_endpoint_check<float>(const void *, int64_t, int64_t, int, unsigned long long *,
+                                                       int *, int *, hipStream_t);
+template hipError_t launch_local_endpoint_check<double>(const void *, int64_t, int64_t, int, unsigned long long *,
+                                                        int *, int *, hipStream_t);
+
+hipError_t launch_fill_i32(int *p, int64_t n, int v, hipStream_t)
+{
+    for (int64_t i = 0; i < n; ++i) p[i] = v;
+    return hipSuccess;
+}
+
+hipError_t launch_permute_words(const PermuteArgs &p, const void *src, void *dst, int word, hipStream_t)
+{
+    ++g_launches;
+    begin_launch();
+    const int64_t total = p.len * p.extra;
+    for (int64_t i = 0; i < total; ++i) {
+        const int64_t e = i / p.len;
+        int64_t rem = i - e * p.len, so = 0;
+        for (int d = p.nd - 1; d >= 0; --d) {
+            so += (rem % p.dext[d]) * p.sstride[d];
+            rem /= p.dext[d];
+        }
+        const char *s = static_cast<const char *>(src) + (e * p.len + so) * word;
+        char *d = static_cast<char *>(dst) + i * word;
+        if (!check_read("permute", s, word) || !check_write("permute", d, word)) return hipSuccess;
+        std::memcpy(d, s, word);
+    }
+    return hipSuccess;
+}
+
+} // namespace pfh
+
+// ---------------------------------------------------------------------------
+// scenarios
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct Arr {
+    std::vector<int> perm;
+    std::vector<int64_t> shape;
+    int64_t elems = 0, eb = 0;
+};
+
+Arr layout(pfh_plan p, int side, int nd)
+{
+    int perm[PFH_MAX_DIMS];
+    size_t shape[PFH_MAX_DIMS], el = 0, eb = 0;
+    if (pfh_plan_layout(p, side, perm, shape, &el, &eb) != PFH_OK) fail("layout query");
+    Arr a;
+    for (int i = 0; i < nd; ++i) {
+        a.perm.push_back(perm[i]);
+        a.shape.push_back((int64_t)shape[i]);
+    }
+    a.elems = (int64_t)el;
+    a.eb = (int64_t)eb;
+    return a;
+}
+
+// memory offset (elements) of logical coordinates c in extra block e
+int64_t offset_of(const Arr &a, const std::vector<int64_t> &c, int64_t e)
+{
+    int64_t len = 1;
+    for (int64_t s : a.shape) len *= s;
+    int64_t off = 0;
+    for (size_t p = 0; p < a.perm.size(); ++p) off = off * a.shape[a.perm[p]] + c[a.perm[p]];
+    return e * len + off;
+}
+
+void *dmalloc(size_t n, bool zero = false)
+{
+    void *p = nullptr;
+    if (hipMalloc(&p, n) != hipSuccess) return nullptr;
+    if (zero) std::memset(p, 0, n);
+    return p;
+}
+
+// Runs op on coded input and checks the identity-on-points semantics.
+template <typename T>
+void run_case(const char *name, pfh_plan plan, int nd, int op, bool real_in, bool real_out, int red_axis,
+              int64_t red_n)
+{
+    const bool fwd_like = (op == PFH_FORWARD || op == PFH_ADJOINT_INVERSE || op == PFH_ADJOINT_BACKWARD);
+    Arr ain = layout(plan, fwd_like ? 0 : 1, nd), aout = layout(plan, fwd_like ? 1 : 0, nd);
+    const int64_t ei = real_in ? sizeof(T) : 2 * sizeof(T), eo = real_out ? sizeof(T) : 2 * sizeof(T);
+    void *src = dmalloc(ain.elems * ei), *dst = dmalloc(aout.elems * eo);
+    T *s = static_cast<T *>(src);
+    for (int64_t i = 0; i < ain.elems; ++i) {
+        if (real_in) {
+            s[i] = (T)(i + 1);
+        } else {
+            s[2 * i] = (T)(i + 1);
+            s[2 * i + 1] = (T)(-(i + 1));
+        }
+    }
+    size_t wsb = 0;
+    pfh_plan_workspace_bytes(plan, op, &wsb);
+    void *ws = dmalloc(wsb, true);
+    const int before = g_fail;
+    pfh_status st = pfh_execute(plan, op, src, ain.elems, dst, aout.elems, ws, wsb, nullptr, 0);
+    if (st != PFH_OK) {
+        pfh_error_info inf;
+        pfh_last_error(&inf);
+        fail("%s op %d: status %d (%s)", name, op, (int)st, inf.message);
+    }
+    // semantics: out(c) = in(c) on the common points
+    int64_t len_out = 1;
+    for (int64_t v : aout.shape) len_out *= v;
+    const int64_t extra = aout.elems / len_out;
+    const T *d = static_cast<const T *>(dst);
+    std::vector<int64_t> c(nd, 0);
+    int64_t bad = 0;
+    for (int64_t e = 0; e < extra && bad < 3; ++e) {
+        for (int64_t lin = 0; lin < len_out && bad < 3; ++lin) {
+            int64_t r = lin;
+            for (int a = nd - 1; a >= 0; --a) {
+                c[a] = r % aout.shape[a];
+                r /= aout.shape[a];
+            }
+            const int64_t oo = offset_of(aout, c, e);
+            double er = 0, eim = 0;
+            const bool inside = red_axis < 0 || c[red_axis] < ain.shape[red_axis];
+            if (inside) {
+                const int64_t io = offset_of(ain, c, e);
+                er = real_in ? (double)s[io] : (double)s[2 * io];
+                eim = real_in ? 0.0 : (double)s[2 * io + 1];
+            }
+            (void)red_n;
+            const double gr = real_out ? (double)d[oo] : (double)d[2 * oo];
+            const double gi = real_out ? 0.0 : (double)d[2 * oo + 1];
+            const double wi = (real_in || real_out) ? 0.0 : eim;
+            if (gr != er || gi != wi) {
+                fail("%s op %d: extra %" PRId64 " logical index %" PRId64 ": got (%g, %g) want (%g, %g)", name, op, e,
+                     lin, gr, gi, er, wi);
+                ++bad;
+            }
+        }
+    }
+    std::printf("%s op %d: %s\n", name, op, g_fail == before ? "ok" : "FAILED");
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    (void)hipFree(ws);
+}
+
+pfh_desc route_desc(int kind, int prec, std::vector<size_t> shape, std::vector<size_t> extra)
+{
+    pfh_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.kind = kind;
+    d.precision = prec;
+    d.ndim = (int)shape.size();
+    for (size_t i = 0; i < shape.size(); ++i) d.shape[i] = shape[i];
+    d.extra_ndim = (int)extra.size();
+    for (size_t i = 0; i < extra.size(); ++i) d.extra_shape[i] = extra[i];
+    d.permute_dims = 1;
+    return d;
+}
+
+template <typename T>
+void route_cases(int prec)
+{
+    const char *pn = prec == PFH_F32 ? "f32" : "f64";
+    struct C {
+        std::vector<size_t> shape, extra;
+    };
+    std::vector<C> cases = {{{64, 64, 64}, {}},   {{64, 64, 64}, {3}},  {{48, 60, 64}, {}}, {{16, 96, 40}, {2}},
+                            {{3, 4, 5}, {}},      {{128, 32, 8}, {}},   {{2, 3, 2, 3}, {}}, {{8, 12, 16, 4}, {}},
+                            {{513, 4, 6}, {}},    {{40, 48, 36}, {3}},  {{67, 32, 40}, {}}, {{37, 41, 64}, {2}}};
+    for (const C &c : cases) {
+        char name[128];
+        std::string sh;
+        for (size_t v : c.shape) sh += std::to_string(v) + "x";
+        sh.pop_back();
+        if (!c.extra.empty()) sh += "+extra" + std::to_string(c.extra[0]);
+        const int nd = (int)c.shape.size();
+        // C2C
+        {
+            pfh_desc d = route_desc(PFH_C2C, prec, c.shape, c.extra);
+            pfh_plan p = nullptr;
+            if (pfh_plan_create(&d, &p) != PFH_OK) {
+                fail("C2C %s %s: plan", pn, sh.c_str());
+                continue;
+            }
+            std::snprintf(name, sizeof(name), "C2C %s %s", pn, sh.c_str());
+            for (int op : {PFH_FORWARD, PFH_INVERSE, PFH_ADJOINT_FORWARD})
+                run_case<T>(name, p, nd, op, false, false, -1, 0);
+            pfh_plan_destroy(p);
+        }
+        // R2C
+        {
+            pfh_desc d = route_desc(PFH_R2C, prec, c.shape, c.extra);
+            pfh_plan p = nullptr;
+            if (pfh_plan_create(&d, &p) != PFH_OK) {
+                fail("R2C %s %s: plan", pn, sh.c_str());
+                continue;
+            }
+            std::snprintf(name, sizeof(name), "R2C %s %s", pn, sh.c_str());
+            const int ra = nd - 1;
+            run_case<T>(name, p, nd, PFH_FORWARD, true, false, ra, (int64_t)c.shape[ra]);
+            run_case<T>(name, p, nd, PFH_ADJOINT_INVERSE, true, false, ra, (int64_t)c.shape[ra]);
+            run_case<T>(name, p, nd, PFH_INVERSE, false, true, ra, (int64_t)c.shape[ra]);
+            run_case<T>(name, p, nd, PFH_BACKWARD, false, true, ra, (int64_t)c.shape[ra]);
+            pfh_plan_destroy(p);
+        }
+        // R2R (DCT-II, DST-I, DCT-IV, DHT per axis)
+        {
+            pfh_desc d = route_desc(PFH_R2R, prec, c.shape, c.extra);
+            const int kinds[4] = {PFH_DCT2, PFH_DST1, PFH_DCT4, PFH_DHT};
+            for (int a = 0; a < nd; ++a) d.axis_kind[a] = kinds[a % 4];
+            pfh_plan p = nullptr;
+            if (pfh_plan_create(&d, &p) != PFH_OK) {
+                fail("R2R %s %s: plan", pn, sh.c_str());
+                continue;
+            }
+            std::snprintf(name, sizeof(name), "R2R %s %s", pn, sh.c_str());
+            for (int op : {PFH_FORWARD, PFH_INVERSE})
+                run_case<T>(name, p, nd, op, true, true, -1, 0);
+            pfh_plan_destroy(p);
+        }
+    }
+}
+
+void batch_cases()
+{
+    std::vector<std::vector<size_t>> shapes = {{48, 60, 64}, {12, 7, 9}, {84, 16, 20}, {5, 5, 5}, {64, 32, 16}};
+    std::vector<pfh_desc> ds;
+    for (auto &s : shapes) ds.push_back(route_desc(PFH_C2C, PFH_F32, s, {3}));
+    for (int tune : {0, 1}) {
+        pfh_batch b = nullptr;
+        if (pfh_batch_create(ds.data(), (int)ds.size(), tune, &b) != PFH_OK) {
+            fail("batch create (autotune %d)", tune);
+            continue;
+        }
+        size_t wsb = 0;
+        pfh_batch_workspace_bytes(b, &wsb);
+        void *ws = dmalloc(wsb, true);
+        std::vector<const void *> srcs;
+        std::vector<void *> dsts;
+        std::vector<int64_t> n;
+        for (auto &s : shapes) {
+            const int64_t e = (int64_t)(s[0] * s[1] * s[2] * 3);
+            n.push_back(e);
+            float *x = static_cast<float *>(dmalloc(e * 8));
+            for (int64_t i = 0; i < 2 * e; ++i) x[i] = (float)(i + 1);
+            srcs.push_back(x);
+            dsts.push_back(dmalloc(e * 8));
+        }
+        const int before = g_fail;
+        for (int op : {PFH_FORWARD, PFH_INVERSE})
+            if (pfh_batch_execute(b, op, srcs.data(), dsts.data(), ws, wsb, nullptr, 0) != PFH_OK)
+                fail("batch execute op %d", op);
+        // the batch must give the single plans' layout
+        for (size_t i = 0; i < shapes.size(); ++i) {
+            pfh_plan p = nullptr;
+            pfh_desc d = ds[i];
+            pfh_plan_create(&d, &p);
+            void *ref = dmalloc(n[i] * 8);
+            size_t w2 = 0;
+            pfh_plan_workspace_bytes(p, PFH_INVERSE, &w2);
+            void *ws2 = dmalloc(w2, true);
+            pfh_execute(p, PFH_INVERSE, srcs[i], n[i], ref, n[i], ws2, w2, nullptr, 0);
+            if (std::memcmp(ref, dsts[i], n[i] * 8) != 0) fail("batch problem %zu differs from its plan", i);
+            (void)hipFree(ref);
+            (void)hipFree(ws2);
+            pfh_plan_destroy(p);
+        }
+        std::printf("batch autotune %d: %s\n", tune, g_fail == before ? "ok" : "FAILED");
+        for (auto *x : srcs) (void)hipFree(const_cast<void *>(x));
+        for (auto *x : dsts) (void)hipFree(x);
+        (void)hipFree(ws);
+        pfh_batch_destroy(b);
+    }
+}
+
+} // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc == 6) {
+        // one plan: emu c2c|r2c n0 n1 n2 f32|f64 (planner defaults)
+        const bool r2c = std::strcmp(argv[1], "r2c") == 0, f32 = std::strcmp(argv[5], "f32") == 0;
+        std::vector<size_t> shape = {(size_t)std::atol(argv[2]), (size_t)std::atol(argv[3]), (size_t)std::atol(argv[4])};
+        pfh_desc d = route_desc(r2c ? PFH_R2C : PFH_C2C, f32 ? PFH_F32 : PFH_F64, shape, {});
+        pfh_plan p = nullptr;
+        if (pfh_plan_create(&d, &p) != PFH_OK) return 2;
+        for (int op : {PFH_FORWARD, PFH_INVERSE}) {
+            const bool fl = op == PFH_FORWARD;
+            if (f32)
+                run_case<float>(argv[1], p, 3, op, r2c && fl, r2c && !fl, r2c ? 2 : -1, 0);
+            else
+                run_case<double>(argv[1], p, 3, op, r2c && fl, r2c && !fl, r2c ? 2 : -1, 0);
+        }
+        pfh_plan_destroy(p);
+        std::printf("launches %d, failures %d\n", pfh::g_launches, g_fail);
+        return g_fail ? 1 : 0;
+    }
+    // 1 MiB slab intermediates so that the small shapes here are pipelined
+    setenv("PFH_SLAB_MB", "1", 0);
+    route_cases<double>(PFH_F64);
+    route_cases<float>(PFH_F32);
+    batch_cases();
+    std::printf("launches %d, failures %d\n", pfh::g_launches, g_fail);
+    return g_fail ? 1 : 0;
+}
