@@ -270,6 +270,34 @@ pfh_status pfh_batch_execute(pfh_batch batch, int op, const void *const *srcs, v
  * per tile on column-access stages), as chosen by the autotuner. */
 pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines);
 
+/* ---------------------------------------------------------------------- */
+/* Pointwise operations between transforms (pseudo-spectral steps).        */
+/* ---------------------------------------------------------------------- */
+
+/* data *= prod_a factors_a[i_a] over the plan's input (side 0) or output
+ * (side 1) array of the forward direction, every extra block alike: the
+ * reference's LocalGrid per-axis coordinate arrays seen in the pencil's
+ * memory order (crates/pencil-array/src/grid.rs:20-120) combined with
+ * pointwise_many_in_place (pointwise.rs:421-490). factors: device array of
+ * the plan's real type, the per-axis arrays concatenated in logical axis
+ * order (length sum of the side's logical shape). Complex data has both
+ * parts scaled. elems must equal that array's element count. */
+pfh_status pfh_pointwise_scale_axes(pfh_plan plan, int side, void *data, size_t elems, const void *factors,
+                                    pfh_stream_t stream);
+
+/* u = w + a w^2 elementwise over n reals (pointwise2, pointwise.rs:76-88). */
+pfh_status pfh_pointwise_quadratic(int precision, const void *w, void *u, size_t n, double a, pfh_stream_t stream);
+
+/* out = g (1 + 2 a w): the transpose of the derivative of the step above. */
+pfh_status pfh_pointwise_quadratic_adjoint(int precision, const void *w, const void *g, void *out, size_t n,
+                                           double a, pfh_stream_t stream);
+
+/* *result = sum of x_i^2 over n reals, accumulated in f64 in a fixed order
+ * (the local part of l2_norm, collectives.rs:519-560). workspace: device
+ * memory of at least 8192 bytes; result: device pointer to one double. */
+pfh_status pfh_sum_squares(int precision, const void *x, size_t n, void *workspace, size_t workspace_bytes,
+                           double *result, pfh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -619,6 +619,7 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
     static const int block_env = env_int("PFH_BLOCK", 256);
     const int target = tu.block > 0 ? tu.block : block_env;
     int lt = (st.in_col || st.out_col) ? lt_col : std::max(1, target / tpl);
+    if (tu.fill_block && (st.in_col || st.out_col)) lt = std::max(lt, target / tpl);
     while (lt > 1 && (int64_t)(lt / 2) >= nlines) lt /= 2;
     while (lt * tpl < 64) tpl *= 2;
     const int cap = tu.block_cap > 0 ? std::min(tu.block_cap, max_threads) : max_threads;
@@ -2087,9 +2088,9 @@ static pfh_status batch_run(pfh_batch_s &B, int op, const void *const *srcs, voi
 static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int count)
 {
     std::vector<Tuning> cands;
-    for (int block : {256, 512})
+    for (int block : {128, 256, 512})
         for (int maxr : {8, 16})
-            for (int col : {8, 16}) {
+            for (int col : {0, 32}) {
                 Tuning t;
                 t.block = block;
                 t.max_radix = maxr;
@@ -2097,6 +2098,8 @@ static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int coun
                 t.block_cap = kMaxThreadsWide;
                 t.allow_fast = false;
                 t.allow_xcd = false;
+                t.slab_mb = 0;
+                t.fill_block = true;
                 cands.push_back(t);
             }
     // scratch buffers for the measurement
@@ -2178,6 +2181,8 @@ pfh_status pfh_batch_create(const pfh_desc *descs, int count, int autotune, pfh_
     t.allow_xcd = false;
     t.block = 256;
     t.block_cap = kMaxThreadsWide;
+    t.slab_mb = 0;
+    t.fill_block = true;
     pfh_status st = autotune ? batch_autotune(*b, descs, count) : batch_plan(*b, descs, count, t);
     if (st) {
         delete b;
@@ -2218,6 +2223,81 @@ pfh_status pfh_batch_execute(pfh_batch batch, int op, const void *const *srcs, v
         if (!srcs[i] || !dsts[i]) return set_err(PFH_ERR_INVALID_ARGUMENT, "null problem buffer");
     pfh_status st = batch_run(*batch, op, srcs, dsts, workspace, reinterpret_cast<hipStream_t>(stream), use_graph);
     if (st) return st;
+    return ok();
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// pointwise operations
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+pfh_status pfh_pointwise_scale_axes(pfh_plan plan, int side, void *data, size_t elems, const void *factors,
+                                    pfh_stream_t stream)
+{
+    if (!plan || !data || !factors || (side != 0 && side != 1)) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    const PlanImpl &P = plan->impl;
+    if (P.kind < PFH_C2C || P.kind > PFH_R2R) return set_err(PFH_ERR_INVALID_ARGUMENT, "needs a C2C, R2C or R2R plan");
+    const OpPlan &o = P.ops[PFH_FORWARD];
+    const Layout &L = side == 0 ? o.in : o.out;
+    if ((int64_t)elems != L.total())
+        return set_err(side == 0 ? PFH_ERR_INPUT_LAYOUT_MISMATCH : PFH_ERR_OUTPUT_LAYOUT_MISMATCH, "array length",
+                       (size_t)L.total(), elems);
+    const int tb = P.precision == PFH_F32 ? 4 : 8;
+    ScaleAxesArgs a{};
+    a.nd = L.N;
+    a.complex_data = L.elem == 2 * tb ? 1 : 0;
+    a.len = L.len();
+    a.extra = L.extra;
+    for (int p = 0; p < L.N; ++p) {
+        a.mext[p] = L.shape[L.perm[p]];
+        int64_t off = 0;
+        for (int b = 0; b < L.perm[p]; ++b) off += L.shape[b];
+        a.foff[p] = off;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    PFH_HIP_CHECK(P.precision == PFH_F32 ? launch_scale_axes<float>(a, data, factors, s)
+                                         : launch_scale_axes<double>(a, data, factors, s));
+    return ok();
+}
+
+pfh_status pfh_pointwise_quadratic(int precision, const void *w, void *u, size_t n, double a, pfh_stream_t stream)
+{
+    if (!w || !u || (precision != PFH_F32 && precision != PFH_F64)) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    PFH_HIP_CHECK(precision == PFH_F32 ? launch_quadratic<float>(w, u, (int64_t)n, a, s)
+                                       : launch_quadratic<double>(w, u, (int64_t)n, a, s));
+    return ok();
+}
+
+pfh_status pfh_pointwise_quadratic_adjoint(int precision, const void *w, const void *g, void *out, size_t n,
+                                           double a, pfh_stream_t stream)
+{
+    if (!w || !g || !out || (precision != PFH_F32 && precision != PFH_F64))
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    PFH_HIP_CHECK(precision == PFH_F32 ? launch_quadratic_adjoint<float>(w, g, out, (int64_t)n, a, s)
+                                       : launch_quadratic_adjoint<double>(w, g, out, (int64_t)n, a, s));
+    return ok();
+}
+
+pfh_status pfh_sum_squares(int precision, const void *x, size_t n, void *workspace, size_t workspace_bytes,
+                           double *result, pfh_stream_t stream)
+{
+    if (!x || !result || (precision != PFH_F32 && precision != PFH_F64))
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    const size_t need = sizeof(double) * kSumSquaresParts;
+    if (!workspace || workspace_bytes < need)
+        return set_err(PFH_ERR_WORKSPACE_TOO_SMALL, "sum of squares workspace", need, workspace ? workspace_bytes : 0);
+    int64_t parts = (int64_t)n / 16384;
+    if (parts < 1) parts = 1;
+    if (parts > kSumSquaresParts) parts = kSumSquaresParts;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    double *partial = static_cast<double *>(workspace);
+    PFH_HIP_CHECK(precision == PFH_F32 ? launch_sum_squares<float>(x, (int64_t)n, partial, (int)parts, result, s)
+                                       : launch_sum_squares<double>(x, (int64_t)n, partial, (int)parts, result, s));
     return ok();
 }
 

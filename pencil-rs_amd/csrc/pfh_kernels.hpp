@@ -73,4 +73,23 @@ hipError_t launch_fill_i32(int *p, int64_t n, int v, hipStream_t s);
 
 hipError_t launch_permute_words(const PermuteArgs &p, const void *src, void *dst, int word, hipStream_t s);
 
+// Pointwise operations (pfh_pointwise.hip).
+struct ScaleAxesArgs {
+    int nd;              // spatial positions
+    int complex_data;    // interleaved complex values (both parts scaled)
+    int64_t len, extra;  // elements per extra block, extra blocks
+    int64_t mext[8];     // extent of each memory position (slowest first)
+    int64_t foff[8];     // offset of that position's axis in the factor array
+};
+constexpr int kSumSquaresParts = 1024;
+
+template <typename T>
+hipError_t launch_scale_axes(const ScaleAxesArgs &s, void *data, const void *factors, hipStream_t st);
+template <typename T>
+hipError_t launch_quadratic(const void *w, void *u, int64_t n, double a, hipStream_t st);
+template <typename T>
+hipError_t launch_quadratic_adjoint(const void *w, const void *g, void *out, int64_t n, double a, hipStream_t st);
+template <typename T>
+hipError_t launch_sum_squares(const void *x, int64_t n, double *partial, int nparts, double *result, hipStream_t st);
+
 } // namespace pfh

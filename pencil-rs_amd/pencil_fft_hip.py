@@ -129,6 +129,7 @@ EXPORTED_SYMBOLS = (
     "pfh_plan_workspace_bytes", "pfh_plan_layout", "pfh_execute", "pfh_execute_in_place", "pfh_local_execute",
     "pfh_local_execute_in_place", "pfh_execute_with_timing", "pfh_workspace_take_status", "pfh_batch_create",
     "pfh_batch_destroy", "pfh_batch_workspace_bytes", "pfh_batch_execute", "pfh_batch_tuning",
+    "pfh_pointwise_scale_axes", "pfh_pointwise_quadratic", "pfh_pointwise_quadratic_adjoint", "pfh_sum_squares",
 )
 
 
@@ -163,6 +164,11 @@ def load_library(path: str = LIB_PATH):
     lib.pfh_batch_execute.argtypes = [vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), vp, sz, vp, i]
     ip = ctypes.POINTER(i)
     lib.pfh_batch_tuning.argtypes = [vp, ip, ip, ip]
+    d = ctypes.c_double
+    lib.pfh_pointwise_scale_axes.argtypes = [vp, i, vp, sz, vp, vp]
+    lib.pfh_pointwise_quadratic.argtypes = [i, vp, vp, sz, d, vp]
+    lib.pfh_pointwise_quadratic_adjoint.argtypes = [i, vp, vp, vp, sz, d, vp]
+    lib.pfh_sum_squares.argtypes = [i, vp, sz, vp, sz, vp, vp]
     _lib = lib
     return lib
 
@@ -658,3 +664,38 @@ class BatchC2cPlan:
 
     def backward(self, srcs, dsts=None, workspace=None, use_graph=True):
         return self.execute(BACKWARD, srcs, dsts, workspace, use_graph)
+
+
+# ------------------------------------------------------------- pointwise --
+def scale_axes(plan, side: int, data: torch.Tensor, factors: torch.Tensor) -> torch.Tensor:
+    """data *= prod_a factors_a[i_a] in place over the plan's input (side 0)
+    or output (side 1) array; `factors` is the per-axis arrays concatenated
+    in logical axis order, in the plan's real dtype (LocalGrid coordinates,
+    crates/pencil-array/src/grid.rs:20-120)."""
+    _check(_lib.pfh_pointwise_scale_axes(plan._h, side, _ptr(data), data.numel(), _ptr(factors), _stream()))
+    return data
+
+
+def quadratic(w: torch.Tensor, a: float, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """out = w + a w^2 (real tensors)."""
+    out = torch.empty_like(w) if out is None else out
+    _check(_lib.pfh_pointwise_quadratic(_precision(w.dtype), _ptr(w), _ptr(out), w.numel(), float(a), _stream()))
+    return out
+
+
+def quadratic_adjoint(w: torch.Tensor, g: torch.Tensor, a: float, out: Optional[torch.Tensor] = None):
+    """out = g (1 + 2 a w)."""
+    out = torch.empty_like(g) if out is None else out
+    _check(_lib.pfh_pointwise_quadratic_adjoint(_precision(w.dtype), _ptr(w), _ptr(g), _ptr(out), w.numel(),
+                                                float(a), _stream()))
+    return out
+
+
+def sum_squares(x: torch.Tensor, workspace: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Sum of squares of every real component (complex: both parts), f64,
+    fixed summation order; returns a one-element device tensor."""
+    ws = torch.empty(8192, dtype=torch.uint8, device=x.device) if workspace is None else workspace
+    res = torch.empty(1, dtype=torch.float64, device=x.device)
+    n = x.numel() * (2 if x.is_complex() else 1)
+    _check(_lib.pfh_sum_squares(_precision(x.dtype), _ptr(x), n, _ptr(ws), ws.numel(), _ptr(res), _stream()))
+    return res

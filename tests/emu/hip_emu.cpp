@@ -460,6 +460,79 @@ hipError_t launch_permute_words(const PermuteArgs &p, const void *src, void *dst
     return hipSuccess;
 }
 
+template <typename T>
+hipError_t launch_scale_axes(const ScaleAxesArgs &s, void *data, const void *factors, hipStream_t)
+{
+    ++g_launches;
+    const int64_t total = s.len * s.extra;
+    const size_t eb = (s.complex_data ? 2 : 1) * sizeof(T);
+    check_read("scale_axes", data, (size_t)total * eb);
+    int64_t nf = 0;
+    for (int p = 0; p < s.nd; ++p) nf = std::max(nf, s.foff[p] + s.mext[p]);
+    check_read("scale_axes factors", factors, (size_t)nf * sizeof(T));
+    T *d = static_cast<T *>(data);
+    const T *f = static_cast<const T *>(factors);
+    for (int64_t i = 0; i < total; ++i) {
+        int64_t rem = i % s.len;
+        T w = (T)1;
+        for (int p = s.nd - 1; p >= 0; --p) {
+            w *= f[s.foff[p] + rem % s.mext[p]];
+            rem /= s.mext[p];
+        }
+        if (s.complex_data) {
+            d[2 * i] *= w;
+            d[2 * i + 1] *= w;
+        } else {
+            d[i] *= w;
+        }
+    }
+    return hipSuccess;
+}
+template hipError_t launch_scale_axes<float>(const ScaleAxesArgs &, void *, const void *, hipStream_t);
+template hipError_t launch_scale_axes<double>(const ScaleAxesArgs &, void *, const void *, hipStream_t);
+
+template <typename T>
+hipError_t launch_quadratic(const void *w, void *u, int64_t n, double a, hipStream_t)
+{
+    check_read("quadratic", w, n * sizeof(T));
+    check_read("quadratic out", u, n * sizeof(T));
+    for (int64_t i = 0; i < n; ++i) {
+        const T v = static_cast<const T *>(w)[i];
+        static_cast<T *>(u)[i] = v + (T)a * v * v;
+    }
+    return hipSuccess;
+}
+template hipError_t launch_quadratic<float>(const void *, void *, int64_t, double, hipStream_t);
+template hipError_t launch_quadratic<double>(const void *, void *, int64_t, double, hipStream_t);
+
+template <typename T>
+hipError_t launch_quadratic_adjoint(const void *w, const void *g, void *out, int64_t n, double a, hipStream_t)
+{
+    check_read("quadratic_adjoint", w, n * sizeof(T));
+    check_read("quadratic_adjoint g", g, n * sizeof(T));
+    check_read("quadratic_adjoint out", out, n * sizeof(T));
+    for (int64_t i = 0; i < n; ++i)
+        static_cast<T *>(out)[i] =
+            static_cast<const T *>(g)[i] * ((T)1 + (T)2 * (T)a * static_cast<const T *>(w)[i]);
+    return hipSuccess;
+}
+template hipError_t launch_quadratic_adjoint<float>(const void *, const void *, void *, int64_t, double, hipStream_t);
+template hipError_t launch_quadratic_adjoint<double>(const void *, const void *, void *, int64_t, double,
+                                                     hipStream_t);
+
+template <typename T>
+hipError_t launch_sum_squares(const void *x, int64_t n, double *partial, int nparts, double *result, hipStream_t)
+{
+    check_read("sum_squares", x, n * sizeof(T));
+    check_read("sum_squares partials", partial, nparts * sizeof(double));
+    double acc = 0;
+    for (int64_t i = 0; i < n; ++i) acc += (double)static_cast<const T *>(x)[i] * (double)static_cast<const T *>(x)[i];
+    *result = acc;
+    return hipSuccess;
+}
+template hipError_t launch_sum_squares<float>(const void *, int64_t, double *, int, double *, hipStream_t);
+template hipError_t launch_sum_squares<double>(const void *, int64_t, double *, int, double *, hipStream_t);
+
 } // namespace pfh
 
 // ---------------------------------------------------------------------------

@@ -51,7 +51,7 @@ def test_small_batch_matches_oracle(dt, autotune):
     plan = pf.BatchC2cPlan(shapes, dtype=torch_dt, autotune=autotune, extra_shapes=[(3,)] * len(shapes))
     if autotune:
         block, maxr, col = plan.tuning()
-        assert block in (256, 512) and maxr in (8, 16) and col in (8, 16)
+        assert block in (128, 256, 512) and maxr in (8, 16) and col in (0, 32)
     srcs = [_tdev(x) for x in xs]
     for op in (O.FORWARD, O.INVERSE, O.BACKWARD):
         for use_graph in (False, True):

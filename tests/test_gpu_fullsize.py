@@ -100,3 +100,57 @@ def test_adjoint_dot_products_f64(gpu, kind, shape, kw):
         lhs, rhs = _dot(ax, y), _dot(x, aty)
         rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
         assert rel <= 1e-12, (kind, shape, op, lhs, rhs, rel)
+
+
+def _axis_lines(t, axis, fn):
+    """Applies fn to the lines of a logical 3D tensor along `axis` (made the
+    contiguous last axis, transformed, moved back)."""
+    nd = t.dim()
+    order = [a for a in range(nd) if a != axis] + [axis]
+    moved = t.permute(order).contiguous()
+    out = fn(moved.reshape(-1)).reshape(moved.shape)
+    inv = [order.index(a) for a in range(nd)]
+    return out.permute(inv).contiguous()
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(600)
+def test_mixed_r2r_and_dht_256x384x240(gpu):
+    """256x384x240 real U[-1,1) (seed 3): R2rPlan with (DCT-II, DST-I,
+    DCT-IV) per axis and a DhtPlan over all axes, f32 and f64, against the
+    FP64 oracle (which follows the reference's embedding algorithm,
+    r2r.rs:697-796), against per-axis LocalR2rPlan / LocalDhtPlan results,
+    and the normalized inverse (DCT-III, DST-I, DCT-IV) round trip."""
+    shape = (256, 384, 240)
+    kinds = [pf.R2rKind.DctII, pf.R2rKind.DstI, pf.R2rKind.DctIV]
+    rng = np.random.default_rng(3)
+    x32 = rng.uniform(-1.0, 1.0, size=shape).astype(np.float32)
+    x64 = x32.astype(np.float64)
+    print("oracle f64 R2R and DHT of the f32-rounded input ...", flush=True)
+    ref_r2r = O.transform(O.KIND_R2R, x64, O.FORWARD, shape=shape, r2r=[int(k) for k in kinds], threads=16)
+    ref_dht = O.transform(O.KIND_R2R, x64, O.FORWARD, shape=shape, r2r=[O.DHT] * 3, threads=16)
+    for dt, bound in ((torch.float32, 2e-6), (torch.float64, 1e-12)):
+        xd = torch.from_numpy(x32).cuda().to(dt)
+        for name, plan, ref, ks in (("r2r", pf.R2rPlan(shape, kinds, dtype=dt), ref_r2r, kinds),
+                                    ("dht", pf.DhtPlan(shape, dtype=dt), ref_dht, [pf.R2rKind.Dht] * 3)):
+            y = plan.forward(xd.reshape(-1))
+            err = rel_l2(_host(y), ref)
+            print(f"{dt} {name}: forward rel-L2 vs FP64 oracle {err:.3e}", flush=True)
+            assert err <= bound, (dt, name, err)
+            # the route against the local line plans applied axis by axis
+            loc = xd
+            for axis in (2, 1, 0):
+                n, k = shape[axis], ks[axis]
+                lp = pf.LocalDhtPlan(n, dtype=dt) if k == pf.R2rKind.Dht else pf.LocalR2rPlan(n, k, dtype=dt)
+                loc = _axis_lines(loc, axis, lp.forward)
+            # route output memory order is the reversed axes
+            y_log = y.reshape(shape[::-1]).permute(2, 1, 0)
+            lerr = float(torch.linalg.norm((y_log - loc).double()) / torch.linalg.norm(loc.double()))
+            print(f"{dt} {name}: route vs local plans rel-L2 {lerr:.3e}", flush=True)
+            assert lerr <= bound, (dt, name, lerr)
+            back = plan.inverse(y)
+            rt = float(torch.linalg.norm((back - xd.reshape(-1)).double()) / torch.linalg.norm(xd.double()))
+            print(f"{dt} {name}: inverse round trip rel-L2 {rt:.3e}", flush=True)
+            assert rt <= bound, (dt, name, rt)
+            del plan, y, back, loc
+        torch.cuda.empty_cache()
