@@ -330,9 +330,29 @@ static void choose_fast(Step &st, int tbytes)
     if (!enabled) return;
     if (!(st.mode == M_C2C || st.mode == M_R2C || st.mode == M_C2R)) return;
     const int m = st.m;
-    if (m < 64 || (m & (m - 1)) != 0) return;
     int r0 = 0, r1 = 0, r2 = 0, em = 0;
     if (tbytes == 4) {
+        // mixed-radix lengths with a compile-time plan (first radix <= 16)
+        switch (m) {
+        case 48: r0 = 16; r1 = 3; em = 16; break;
+        case 60: r0 = 4; r1 = 3; r2 = 5; em = 16; break;
+        case 80: r0 = 16; r1 = 5; em = 16; break;
+        case 84: r0 = 4; r1 = 3; r2 = 7; em = 16; break;
+        case 96: r0 = 8; r1 = 4; r2 = 3; em = 16; break;
+        case 112: r0 = 16; r1 = 7; em = 16; break;
+        case 120: r0 = 8; r1 = 3; r2 = 5; em = 16; break;
+        case 160: r0 = 8; r1 = 4; r2 = 5; em = 16; break;
+        case 192: r0 = 16; r1 = 4; r2 = 3; em = 16; break;
+        case 240: r0 = 16; r1 = 3; r2 = 5; em = 16; break;
+        case 320: r0 = 16; r1 = 4; r2 = 5; em = 16; break;
+        case 384: r0 = 16; r1 = 8; r2 = 3; em = 16; break;
+        default: break;
+        }
+    }
+    if (r0 == 0 && (m < 64 || (m & (m - 1)) != 0)) return;
+    if (r0 != 0) {
+        // mixed radix plan chosen above
+    } else if (tbytes == 4) {
         switch (m) {
         case 64: r0 = 8; r1 = 8; em = 16; break;
         case 128: r0 = 8; r1 = 16; em = 16; break;
@@ -1896,13 +1916,19 @@ pfh_status pfh_workspace_take_status(pfh_plan plan, void *workspace, pfh_stream_
 // problem. The launches can be captured once into a hipGraph and replayed.
 // ---------------------------------------------------------------------------
 
+// One persistent launch: the items of one route stage that share a kernel
+// (a compile-time radix plan, or the generic kernel's narrow / wide radix set).
 struct BatchStage {
     void *d_items = nullptr;     // StageArgs<T>[nitems]
     int64_t *d_prefix = nullptr; // [nitems + 1]
     int nitems = 0;
     int64_t total = 0;
     size_t lds = 0; // largest (table + lines) footprint of an item
+    int threads = 64;
     int64_t grid = 0;
+    bool fast = false, wide = false;
+    int fr[3] = {0, 0, 0};
+    int fem = 0;
 };
 
 struct pfh_batch_s {
@@ -1968,7 +1994,7 @@ static pfh_status batch_plan(pfh_batch_s &B, const pfh_desc *descs, int count, c
         for (int op : {PFH_FORWARD, PFH_INVERSE, PFH_BACKWARD}) {
             const OpPlan &o = P->ops[op];
             for (const Step &s : o.steps) {
-                if (s.mode != M_C2C || s.endpoint_check || s.local_endpoint_check || s.fast)
+                if (s.mode != M_C2C || s.endpoint_check || s.local_endpoint_check)
                     return set_err(PFH_ERR_UNSUPPORTED, "batched problems must select every axis");
                 threads = std::max(threads, s.threads);
             }
@@ -1992,33 +2018,66 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
     B.release_stages();
     int nst = 0;
     for (auto &P : B.plans) nst = std::max(nst, (int)P->ops[op].steps.size());
-    B.stages.resize(nst);
+    const int cus = B.plans.empty() ? 256 : B.plans[0]->dev.cus;
     for (int s = 0; s < nst; ++s) {
-        std::vector<StageArgs<T>> items;
-        std::vector<int64_t> prefix(1, 0);
-        BatchStage &bs = B.stages[s];
+        // group this stage's items by kernel
+        struct Group {
+            BatchStage bs;
+            std::vector<StageArgs<T>> items;
+            std::vector<int64_t> prefix{0};
+        };
+        std::vector<Group> groups;
         for (int i = 0; i < B.count; ++i) {
             const PlanImpl &P = *B.plans[i];
             const OpPlan &o = P.ops[op];
             if (s >= (int)o.steps.size()) continue;
             const Step &st = o.steps[s];
+            bool wide = false;
+            for (int R : st.radices) wide = wide || is_wide_radix(R);
+            Group *g = nullptr;
+            for (Group &c : groups) {
+                const BatchStage &k = c.bs;
+                if (k.fast != st.fast) continue;
+                if (st.fast ? (k.fr[0] == st.fr[0] && k.fr[1] == st.fr[1] && k.fr[2] == st.fr[2] && k.fem == st.fem)
+                            : (k.wide == wide)) {
+                    g = &c;
+                    break;
+                }
+            }
+            if (!g) {
+                groups.emplace_back();
+                g = &groups.back();
+                g->bs.fast = st.fast;
+                g->bs.wide = !st.fast && wide;
+                for (int r = 0; r < 3; ++r) g->bs.fr[r] = st.fr[r];
+                g->bs.fem = st.fem;
+            }
             WsView wv;
             pfh_status e = view_workspace(P, o, static_cast<char *>(ws) + B.ws_off[i], (size_t)(B.ws_bytes), wv);
             if (e) return e;
             void *bufs[4] = {const_cast<void *>(srcs[i]), dsts[i], wv.w[0], wv.w[1]};
-            items.push_back(fill_args<T>(P, st, bufs, nullptr));
-            prefix.push_back(prefix.back() + st.tiles);
-            bs.lds = std::max(bs.lds, st.lds_bytes);
+            g->items.push_back(fill_args<T>(P, st, bufs, nullptr));
+            g->prefix.push_back(g->prefix.back() + st.tiles);
+            g->bs.lds = std::max(g->bs.lds, st.lds_bytes);
+            g->bs.threads = std::max(g->bs.threads, st.threads);
         }
-        bs.nitems = (int)items.size();
-        bs.total = prefix.back();
-        PFH_HIP_CHECK(hipMalloc(&bs.d_items, std::max<size_t>(1, items.size()) * sizeof(StageArgs<T>)));
-        PFH_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&bs.d_prefix), prefix.size() * sizeof(int64_t)));
-        PFH_HIP_CHECK(hipMemcpy(bs.d_items, items.data(), items.size() * sizeof(StageArgs<T>), hipMemcpyHostToDevice));
-        PFH_HIP_CHECK(hipMemcpy(bs.d_prefix, prefix.data(), prefix.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-        const int per_cu = line_batch_blocks_per_cu<T>(B.threads, bs.lds);
-        const int cus = B.plans.empty() ? 256 : B.plans[0]->dev.cus;
-        bs.grid = std::min<int64_t>(bs.total, (int64_t)per_cu * cus);
+        for (Group &g : groups) {
+            BatchStage bs = g.bs;
+            bs.nitems = (int)g.items.size();
+            bs.total = g.prefix.back();
+            PFH_HIP_CHECK(hipMalloc(&bs.d_items, std::max<size_t>(1, g.items.size()) * sizeof(StageArgs<T>)));
+            B.stages.push_back(bs); // owned from here (freed by release_stages)
+            BatchStage &own = B.stages.back();
+            PFH_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&own.d_prefix), g.prefix.size() * sizeof(int64_t)));
+            PFH_HIP_CHECK(
+                hipMemcpy(own.d_items, g.items.data(), g.items.size() * sizeof(StageArgs<T>), hipMemcpyHostToDevice));
+            PFH_HIP_CHECK(
+                hipMemcpy(own.d_prefix, g.prefix.data(), g.prefix.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+            const int per_cu = own.fast ? line_fast_batch_blocks_per_cu<T>(own.fr[0], own.fr[1], own.fr[2], own.fem,
+                                                                           own.threads, own.lds)
+                                        : line_batch_blocks_per_cu<T>(own.wide, own.threads, own.lds);
+            own.grid = std::min<int64_t>(own.total, (int64_t)per_cu * cus);
+        }
     }
     (void)stream;
     B.cur_op = op;
@@ -2031,9 +2090,15 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
 template <typename T>
 static pfh_status batch_launch(pfh_batch_s &B, hipStream_t stream)
 {
-    for (const BatchStage &bs : B.stages)
-        PFH_HIP_CHECK(launch_line_batch<T>(reinterpret_cast<const StageArgs<T> *>(bs.d_items), bs.d_prefix, bs.nitems,
-                                           bs.total, B.threads, bs.lds, bs.grid, stream));
+    for (const BatchStage &bs : B.stages) {
+        const StageArgs<T> *items = reinterpret_cast<const StageArgs<T> *>(bs.d_items);
+        if (bs.fast)
+            PFH_HIP_CHECK(launch_line_fast_batch<T>(items, bs.d_prefix, bs.nitems, bs.total, bs.fr[0], bs.fr[1],
+                                                    bs.fr[2], bs.fem, bs.threads, bs.lds, bs.grid, stream));
+        else
+            PFH_HIP_CHECK(launch_line_batch<T>(items, bs.d_prefix, bs.nitems, bs.total, bs.wide, bs.threads, bs.lds,
+                                               bs.grid, stream));
+    }
     return PFH_OK;
 }
 
@@ -2096,7 +2161,7 @@ static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int coun
                 t.max_radix = maxr;
                 t.col_lines = col;
                 t.block_cap = kMaxThreadsWide;
-                t.allow_fast = false;
+                t.allow_fast = true;
                 t.allow_xcd = false;
                 t.slab_mb = 0;
                 t.fill_block = true;
@@ -2177,7 +2242,7 @@ pfh_status pfh_batch_create(const pfh_desc *descs, int count, int autotune, pfh_
     pfh_batch b = new (std::nothrow) pfh_batch_s;
     if (!b) return set_err(PFH_ERR_ALLOCATION_FAILED, "batch");
     Tuning t;
-    t.allow_fast = false;
+    t.allow_fast = true;
     t.allow_xcd = false;
     t.block = 256;
     t.block_cap = kMaxThreadsWide;

@@ -660,8 +660,8 @@ __global__ __launch_bounds__(WIDE ? kMaxThreadsWide : kMaxThreads) void k_line_s
 // (item, tile) pairs, item = one route stage of one problem (its arguments
 // live in device memory), prefix[q] = first work index of item q. Each block
 // keeps its item index and only moves forward.
-template <typename T>
-__global__ __launch_bounds__(kMaxThreadsWide) void k_line_batch(const StageArgs<T> *__restrict__ items,
+template <typename T, bool WIDE>
+__global__ __launch_bounds__(WIDE ? kMaxThreadsWide : kMaxThreads) void k_line_batch(const StageArgs<T> *__restrict__ items,
                                                                 const int64_t *__restrict__ prefix, int nitems,
                                                                 int64_t total)
 {
@@ -688,7 +688,7 @@ __global__ __launch_bounds__(kMaxThreadsWide) void k_line_batch(const StageArgs<
         const int64_t tile = w - prefix[q];
         __syncthreads(); // the previous tile is done with LDS
         for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
-        stage_body<T, true, 1>(a, tile, tws, tws + a.twn, a.radix, a.twoff);
+        stage_body<T, WIDE, 1>(a, tile, tws, tws + a.twn, a.radix, a.twoff);
     }
 }
 
@@ -938,14 +938,57 @@ __device__ __forceinline__ void fast_r2c_store(const StageArgs<T> &a, const cpx<
     }
 }
 
+// One tile of a fast stage: tws holds the staged tables, data the tile's
+// lines. Threads beyond the stage's lt x tpl shape (a batched launch sized
+// for its largest item) take indexes past every line and only join the
+// barriers.
 template <typename T, int R0, int R1, int R2, int EM>
-__global__ __launch_bounds__(512) void k_line_fast(const StageArgs<T> a)
+__device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, const cpx<T> *tws, cpx<T> *data)
 {
     constexpr int M = R0 * R1 * (R2 ? R2 : 1);
     constexpr int RL = R2 ? R2 : R1;
+    const int tid = threadIdx.x;
+    const int lt = a.lt, tpl = a.tpl;
+    const bool act = tid < lt * tpl;
+    const int l = act ? tid / tpl : 0;
+    const int tl = act ? tid % tpl : (1 << 28);
+    cpx<T> *ln = data + l * a.ls;
+    const int mode = a.mode;
+    if (mode == M_C2R) {
+        stage_load(a, data, tile, tid, l, tl);
+        fast_c2r_pre<T, EM, M>(a, ln, tws, tl, tpl);
+        fast_mid<T, R0, EM, M>(ln, tws, 1, tl, tpl);
+    } else {
+        // column access: lanes across the tile's lines; row access: along a line
+        const int fl = !act ? 0 : (a.in_col ? tid % lt : l);
+        const int ft = !act ? (1 << 28) : (a.in_col ? tid / lt : tl);
+        const int64_t line = tile * lt + fl;
+        int64_t io = 0, oo = 0;
+        const bool valid = act && line < a.nlines;
+        if (valid) line_offsets(a.g, line, io, oo);
+        fast_first<T, R0, EM, M>(a, data + fl * a.ls, valid, io, ft, tpl, mode == M_C2C,
+                                 (a.flags & F_CONJ_IN) != 0);
+    }
+    if constexpr (R2 != 0) fast_mid<T, R1, EM, M>(ln, tws + a.twoff[1], R0, tl, tpl);
+    if (mode == M_R2C) {
+        fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
+        if (act) fast_r2c_store(a, data, tws, tile, tid, l, tl);
+        return;
+    }
+    const int fl = !act ? 0 : (a.out_col ? tid % lt : l);
+    const int ft = !act ? (1 << 28) : (a.out_col ? tid / lt : tl);
+    const int64_t line = tile * lt + fl;
+    int64_t io = 0, oo = 0;
+    const bool valid = act && line < a.nlines;
+    if (valid) line_offsets(a.g, line, io, oo);
+    fast_last<T, RL, EM, M>(a, data + fl * a.ls, tws + a.twoff[R2 ? 2 : 1], valid, oo, ft, tpl, mode == M_C2R);
+}
+
+template <typename T, int R0, int R1, int R2, int EM>
+__global__ __launch_bounds__(512) void k_line_fast(const StageArgs<T> a)
+{
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
-    cpx<T> *data = tws + a.twn;
     if (a.gate != nullptr && *a.gate != 0) return;
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
@@ -955,38 +998,38 @@ __global__ __launch_bounds__(512) void k_line_fast(const StageArgs<T> a)
         if (tile >= a.ntiles) return;
     }
     for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
-    const int lt = a.lt, tpl = a.tpl;
-    const int l = tid / tpl, tl = tid % tpl;
-    cpx<T> *ln = data + l * a.ls;
-    const int mode = a.mode;
-    if (mode == M_C2R) {
-        stage_load(a, data, tile, tid, l, tl);
-        fast_c2r_pre<T, EM, M>(a, ln, tws, tl, tpl);
-        fast_mid<T, R0, EM, M>(ln, tws, 1, tl, tpl);
-    } else {
-        // column access: lanes across the tile's lines; row access: along a line
-        const int fl = a.in_col ? tid % lt : l;
-        const int ft = a.in_col ? tid / lt : tl;
-        const int64_t line = tile * lt + fl;
-        int64_t io = 0, oo = 0;
-        const bool valid = line < a.nlines;
-        if (valid) line_offsets(a.g, line, io, oo);
-        fast_first<T, R0, EM, M>(a, data + fl * a.ls, valid, io, ft, tpl, mode == M_C2C,
-                                 (a.flags & F_CONJ_IN) != 0);
+    fast_body<T, R0, R1, R2, EM>(a, tile, tws, tws + a.twn);
+}
+
+// Persistent batched fast stage: the work list of k_line_batch, every item
+// using this compile-time radix plan.
+template <typename T, int R0, int R1, int R2, int EM>
+__global__ __launch_bounds__(512) void k_line_fast_batch(const StageArgs<T> *__restrict__ items,
+                                                        const int64_t *__restrict__ prefix, int nitems, int64_t total)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    int q = 0;
+    {
+        const int64_t w0 = blockIdx.x;
+        int lo = 0, hi = nitems;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (prefix[mid] <= w0) lo = mid;
+            else hi = mid;
+        }
+        q = lo;
     }
-    if constexpr (R2 != 0) fast_mid<T, R1, EM, M>(ln, tws + a.twoff[1], R0, tl, tpl);
-    if (mode == M_R2C) {
-        fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
-        fast_r2c_store(a, data, tws, tile, tid, l, tl);
-        return;
+    for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        while (q + 1 < nitems && prefix[q + 1] <= w) ++q;
+        const StageArgs<T> &a = items[q];
+        const int64_t tile = w - prefix[q];
+        __syncthreads(); // the previous tile is done with LDS
+        for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
+        fast_body<T, R0, R1, R2, EM>(a, tile, tws, tws + a.twn);
     }
-    const int fl = a.out_col ? tid % lt : l;
-    const int ft = a.out_col ? tid / lt : tl;
-    const int64_t line = tile * lt + fl;
-    int64_t io = 0, oo = 0;
-    const bool valid = line < a.nlines;
-    if (valid) line_offsets(a.g, line, io, oo);
-    fast_last<T, RL, EM, M>(a, data + fl * a.ls, tws + a.twoff[R2 ? 2 : 1], valid, oo, ft, tpl, mode == M_C2R);
 }
 
 template <typename T>
@@ -1034,6 +1077,8 @@ hipError_t set_fast_lds_limit(size_t bytes)
 #define PFH_FAST_ATTR(TT, A, B, C, E)                                                                    \
     {                                                                                                     \
         hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_line_fast<TT, A, B, C, E>), bytes); \
+        if (e == hipSuccess)                                                                              \
+            e = raise_lds_limit(reinterpret_cast<const void *>(&k_line_fast_batch<TT, A, B, C, E>), bytes); \
         if (e != hipSuccess) return e;                                                                    \
     }
     PFH_FAST_PLANS(PFH_FAST_ATTR)
@@ -1259,28 +1304,79 @@ hipError_t launch_line_stage(const StageArgs<T> &a, bool wide, int threads, size
 }
 
 template <typename T>
-hipError_t launch_line_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total,
+hipError_t launch_line_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total, bool wide,
                              int threads, size_t lds_bytes, int64_t grid, hipStream_t s)
 {
     if (total <= 0 || grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_line_batch<T>, dim3((unsigned)grid), dim3(threads), lds_bytes, s, items, prefix, nitems, total);
+    if (wide)
+        hipLaunchKernelGGL((k_line_batch<T, true>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, items, prefix,
+                           nitems, total);
+    else
+        hipLaunchKernelGGL((k_line_batch<T, false>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, items, prefix,
+                           nitems, total);
     return hipGetLastError();
 }
 
-template hipError_t launch_line_batch<float>(const StageArgs<float> *, const int64_t *, int, int64_t, int, size_t, int64_t,
-                                             hipStream_t);
-template hipError_t launch_line_batch<double>(const StageArgs<double> *, const int64_t *, int, int64_t, int, size_t,
-                                              int64_t, hipStream_t);
+template hipError_t launch_line_batch<float>(const StageArgs<float> *, const int64_t *, int, int64_t, bool, int,
+                                             size_t, int64_t, hipStream_t);
+template hipError_t launch_line_batch<double>(const StageArgs<double> *, const int64_t *, int, int64_t, bool, int,
+                                              size_t, int64_t, hipStream_t);
 
 template <typename T>
-int line_batch_blocks_per_cu(int threads, size_t lds_bytes)
+int line_batch_blocks_per_cu(bool wide, int threads, size_t lds_bytes)
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_line_batch<T>, threads, lds_bytes) != hipSuccess) return 1;
+    const hipError_t e =
+        wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_line_batch<T, true>, threads, lds_bytes)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_line_batch<T, false>, threads, lds_bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
     return n > 0 ? n : 1;
 }
-template int line_batch_blocks_per_cu<float>(int, size_t);
-template int line_batch_blocks_per_cu<double>(int, size_t);
+template int line_batch_blocks_per_cu<float>(bool, int, size_t);
+template int line_batch_blocks_per_cu<double>(bool, int, size_t);
+
+template <typename T>
+hipError_t launch_line_fast_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total, int r0,
+                                  int r1, int r2, int em, int threads, size_t lds_bytes, int64_t grid, hipStream_t s)
+{
+    if (total <= 0 || grid <= 0) return hipSuccess;
+#define PFH_FAST_BATCH_CASE(TT, A, B, C, E)                                                                      \
+    if constexpr (std::is_same<T, TT>::value) if (r0 == A && r1 == B && r2 == C && em == E) {                  \
+        hipLaunchKernelGGL((k_line_fast_batch<T, A, B, C, E>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, \
+                           items, prefix, nitems, total);                                                        \
+        return hipGetLastError();                                                                                \
+    }
+    PFH_FAST_PLANS(PFH_FAST_BATCH_CASE)
+#undef PFH_FAST_BATCH_CASE
+    return hipErrorInvalidValue;
+}
+template hipError_t launch_line_fast_batch<float>(const StageArgs<float> *, const int64_t *, int, int64_t, int, int,
+                                                  int, int, int, size_t, int64_t, hipStream_t);
+template hipError_t launch_line_fast_batch<double>(const StageArgs<double> *, const int64_t *, int, int64_t, int, int,
+                                                   int, int, int, size_t, int64_t, hipStream_t);
+
+template <typename T>
+int line_fast_batch_blocks_per_cu(int r0, int r1, int r2, int em, int threads, size_t lds_bytes)
+{
+    int n = 0;
+#define PFH_FAST_OCC(TT, A, B, C, E)                                                                             \
+    if constexpr (std::is_same<T, TT>::value) if (r0 == A && r1 == B && r2 == C && em == E) {                  \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_line_fast_batch<T, A, B, C, E>, threads,          \
+                                                         lds_bytes) != hipSuccess) {                             \
+            (void)hipGetLastError();                                                                             \
+            return 1;                                                                                            \
+        }                                                                                                        \
+        return n > 0 ? n : 1;                                                                                    \
+    }
+    PFH_FAST_PLANS(PFH_FAST_OCC)
+#undef PFH_FAST_OCC
+    return 1;
+}
+template int line_fast_batch_blocks_per_cu<float>(int, int, int, int, int, size_t);
+template int line_fast_batch_blocks_per_cu<double>(int, int, int, int, int, size_t);
 
 template hipError_t launch_line_stage<float>(const StageArgs<float> &, bool, int, size_t, int64_t, hipStream_t);
 template hipError_t launch_line_stage<double>(const StageArgs<double> &, bool, int, size_t, int64_t, hipStream_t);
@@ -1297,8 +1393,10 @@ hipError_t set_line_stage_lds_limit(size_t bytes)
                          reinterpret_cast<const void *>(&k_line_stage<double, false, 2>),
                          reinterpret_cast<const void *>(&k_line_stage<double, true, 1>),
                          reinterpret_cast<const void *>(&k_line_stage<double, true, 2>),
-                         reinterpret_cast<const void *>(&k_line_batch<float>),
-                         reinterpret_cast<const void *>(&k_line_batch<double>)};
+                         reinterpret_cast<const void *>(&k_line_batch<float, true>),
+                         reinterpret_cast<const void *>(&k_line_batch<float, false>),
+                         reinterpret_cast<const void *>(&k_line_batch<double, true>),
+                         reinterpret_cast<const void *>(&k_line_batch<double, false>)};
     for (const void *f : fns) {
         hipError_t e = raise_lds_limit(f, bytes);
         if (e != hipSuccess) return e;

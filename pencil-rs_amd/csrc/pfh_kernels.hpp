@@ -47,14 +47,21 @@ hipError_t launch_line_stage(const StageArgs<T> &a, bool wide, int threads, size
 
 hipError_t set_line_stage_lds_limit(size_t bytes);
 
-// Persistent batched stage over a device work list (see k_line_batch).
+// Persistent batched stage over a device work list (see k_line_batch):
+// the generic kernel (narrow or wide radix set) or one compile-time radix
+// plan for every item (k_line_fast_batch).
 template <typename T>
-hipError_t launch_line_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total,
+hipError_t launch_line_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total, bool wide,
                              int threads, size_t lds_bytes, int64_t grid, hipStream_t s);
 template <typename T>
-int line_batch_blocks_per_cu(int threads, size_t lds_bytes);
+int line_batch_blocks_per_cu(bool wide, int threads, size_t lds_bytes);
+template <typename T>
+hipError_t launch_line_fast_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total, int r0,
+                                  int r1, int r2, int em, int threads, size_t lds_bytes, int64_t grid, hipStream_t s);
+template <typename T>
+int line_fast_batch_blocks_per_cu(int r0, int r1, int r2, int em, int threads, size_t lds_bytes);
 
-// Power-of-two stages with a compile-time radix sequence (see k_line_fast).
+// Stages with a compile-time radix sequence (see k_line_fast).
 template <typename T>
 hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int em, int threads, size_t lds_bytes,
                             int64_t grid, hipStream_t s);

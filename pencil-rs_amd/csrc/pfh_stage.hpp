@@ -106,6 +106,18 @@ struct StageArgs {
     X(float, 16, 32, 0, 32)    \
     X(float, 8, 8, 8, 16)      \
     X(float, 32, 32, 0, 32)    \
+    X(float, 16, 3, 0, 16)     \
+    X(float, 4, 3, 5, 16)      \
+    X(float, 16, 5, 0, 16)     \
+    X(float, 4, 3, 7, 16)      \
+    X(float, 8, 4, 3, 16)      \
+    X(float, 16, 7, 0, 16)     \
+    X(float, 8, 3, 5, 16)      \
+    X(float, 8, 4, 5, 16)      \
+    X(float, 16, 4, 3, 16)     \
+    X(float, 16, 3, 5, 16)     \
+    X(float, 16, 4, 5, 16)     \
+    X(float, 16, 8, 3, 16)     \
     X(double, 8, 8, 0, 8)      \
     X(double, 4, 4, 8, 8)      \
     X(double, 4, 8, 8, 8)      \

@@ -355,37 +355,71 @@ bool fast_plan_available(int tbytes, int r0, int r1, int r2, int em)
 }
 
 template <typename T>
-hipError_t launch_line_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total,
-                             int threads, size_t lds_bytes, int64_t grid, hipStream_t)
+static void emulate_batch(const char *what, const StageArgs<T> *items, const int64_t *prefix, int nitems,
+                          int64_t total, int threads, size_t lds_bytes, int64_t grid, bool fast, int em,
+                          const int *fr)
 {
     ++g_launches;
     begin_launch();
-    if (threads > kMaxThreadsWide) fail("line_batch: %d threads", threads);
-    if (grid <= 0 || grid > total) fail("line_batch: grid %" PRId64 " for %" PRId64 " tiles", grid, total);
-    check_read("line_batch items", items, sizeof(StageArgs<T>) * nitems);
-    check_read("line_batch prefix", prefix, sizeof(int64_t) * (nitems + 1));
-    if (prefix[0] != 0 || prefix[nitems] != total) fail("line_batch: prefix ends");
+    if (threads > kMaxThreadsWide) fail("%s: %d threads", what, threads);
+    if (grid <= 0 || grid > total) fail("%s: grid %" PRId64 " for %" PRId64 " tiles", what, grid, total);
+    check_read("batch items", items, sizeof(StageArgs<T>) * nitems);
+    check_read("batch prefix", prefix, sizeof(int64_t) * (nitems + 1));
+    if (prefix[0] != 0 || prefix[nitems] != total) fail("%s: prefix ends", what);
     for (int q = 0; q < nitems; ++q) {
         const StageArgs<T> &a = items[q];
-        if (prefix[q + 1] - prefix[q] != a.ntiles) fail("line_batch: item %d tiles", q);
-        if (a.mode != M_C2C) fail("line_batch: item %d mode", q);
-        check_shape("line_batch item", a, threads, lds_bytes, Regs<T>::emax, false, 0);
-        emulate("line_batch item", a);
+        if (prefix[q + 1] - prefix[q] != a.ntiles) fail("%s: item %d tiles", what, q);
+        if (a.mode != M_C2C) fail("%s: item %d mode", what, q);
+        if (fast && (a.radix[0] != fr[0] || a.radix[1] != fr[1] || (fr[2] && a.radix[2] != fr[2])))
+            fail("%s: item %d radices differ from the launch's plan", what, q);
+        check_shape(what, a, threads, lds_bytes, Regs<T>::emax, fast, em);
+        emulate(what, a);
     }
-    return hipSuccess;
 }
-template hipError_t launch_line_batch<float>(const StageArgs<float> *, const int64_t *, int, int64_t, int, size_t,
-                                             int64_t, hipStream_t);
-template hipError_t launch_line_batch<double>(const StageArgs<double> *, const int64_t *, int, int64_t, int, size_t,
-                                              int64_t, hipStream_t);
 
 template <typename T>
-int line_batch_blocks_per_cu(int, size_t)
+hipError_t launch_line_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total, bool wide,
+                             int threads, size_t lds_bytes, int64_t grid, hipStream_t)
+{
+    const int none[3] = {0, 0, 0};
+    (void)wide;
+    emulate_batch("line_batch", items, prefix, nitems, total, threads, lds_bytes, grid, false, 0, none);
+    return hipSuccess;
+}
+template hipError_t launch_line_batch<float>(const StageArgs<float> *, const int64_t *, int, int64_t, bool, int,
+                                             size_t, int64_t, hipStream_t);
+template hipError_t launch_line_batch<double>(const StageArgs<double> *, const int64_t *, int, int64_t, bool, int,
+                                              size_t, int64_t, hipStream_t);
+
+template <typename T>
+hipError_t launch_line_fast_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total, int r0,
+                                  int r1, int r2, int em, int threads, size_t lds_bytes, int64_t grid, hipStream_t)
+{
+    if (!fast_plan_available((int)sizeof(T), r0, r1, r2, em)) fail("line_fast_batch: plan not instantiated");
+    const int fr[3] = {r0, r1, r2};
+    emulate_batch("line_fast_batch", items, prefix, nitems, total, threads, lds_bytes, grid, true, em, fr);
+    return hipSuccess;
+}
+template hipError_t launch_line_fast_batch<float>(const StageArgs<float> *, const int64_t *, int, int64_t, int, int,
+                                                  int, int, int, size_t, int64_t, hipStream_t);
+template hipError_t launch_line_fast_batch<double>(const StageArgs<double> *, const int64_t *, int, int64_t, int, int,
+                                                   int, int, int, size_t, int64_t, hipStream_t);
+
+template <typename T>
+int line_batch_blocks_per_cu(bool, int, size_t)
 {
     return 2;
 }
-template int line_batch_blocks_per_cu<float>(int, size_t);
-template int line_batch_blocks_per_cu<double>(int, size_t);
+template int line_batch_blocks_per_cu<float>(bool, int, size_t);
+template int line_batch_blocks_per_cu<double>(bool, int, size_t);
+
+template <typename T>
+int line_fast_batch_blocks_per_cu(int, int, int, int, int, size_t)
+{
+    return 2;
+}
+template int line_fast_batch_blocks_per_cu<float>(int, int, int, int, int, size_t);
+template int line_fast_batch_blocks_per_cu<double>(int, int, int, int, int, size_t);
 
 hipError_t set_line_stage_lds_limit(size_t) { return hipSuccess; }
 hipError_t set_fast_lds_limit(size_t) { return hipSuccess; }
