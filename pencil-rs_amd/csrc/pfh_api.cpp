@@ -1377,6 +1377,7 @@ static StageArgs<T> fill_args(const PlanImpl &P, const Step &s, void *const bufs
     a.post0 = s.post0;
     a.post1 = s.post1;
     a.twn = s.twn;
+    a.fem = s.fast ? s.fem : 0;
     a.lt = s.lt;
     a.tpl = s.tpl;
     a.ls = s.ls;
@@ -1926,7 +1927,8 @@ struct BatchStage {
     size_t lds = 0; // largest (table + lines) footprint of an item
     int threads = 64;
     int64_t grid = 0;
-    bool fast = false, wide = false;
+    bool multi = false;            // k_line_multi_batch over items of several compile-time plans
+    bool fast = false, wide = false; // else one compile-time plan, or the generic kernel
     int fr[3] = {0, 0, 0};
     int fem = 0;
 };
@@ -2034,12 +2036,16 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
             const Step &st = o.steps[s];
             bool wide = false;
             for (int R : st.radices) wide = wide || is_wide_radix(R);
+            // every compile-time plan of the standard register budget shares
+            // one launch; the others get a launch per plan / radix set
+            const bool multi = B.tuning.batch_multi && st.fast && st.fem == (sizeof(T) == 4 ? 16 : 8);
             Group *g = nullptr;
             for (Group &c : groups) {
                 const BatchStage &k = c.bs;
-                if (k.fast != st.fast) continue;
-                if (st.fast ? (k.fr[0] == st.fr[0] && k.fr[1] == st.fr[1] && k.fr[2] == st.fr[2] && k.fem == st.fem)
-                            : (k.wide == wide)) {
+                if (k.multi != multi || k.fast != (st.fast && !multi)) continue;
+                if (multi || (st.fast ? (k.fr[0] == st.fr[0] && k.fr[1] == st.fr[1] && k.fr[2] == st.fr[2] &&
+                                         k.fem == st.fem)
+                                      : (k.wide == wide))) {
                     g = &c;
                     break;
                 }
@@ -2047,7 +2053,8 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
             if (!g) {
                 groups.emplace_back();
                 g = &groups.back();
-                g->bs.fast = st.fast;
+                g->bs.multi = multi;
+                g->bs.fast = st.fast && !multi;
                 g->bs.wide = !st.fast && wide;
                 for (int r = 0; r < 3; ++r) g->bs.fr[r] = st.fr[r];
                 g->bs.fem = st.fem;
@@ -2073,9 +2080,10 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
                 hipMemcpy(own.d_items, g.items.data(), g.items.size() * sizeof(StageArgs<T>), hipMemcpyHostToDevice));
             PFH_HIP_CHECK(
                 hipMemcpy(own.d_prefix, g.prefix.data(), g.prefix.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-            const int per_cu = own.fast ? line_fast_batch_blocks_per_cu<T>(own.fr[0], own.fr[1], own.fr[2], own.fem,
-                                                                           own.threads, own.lds)
-                                        : line_batch_blocks_per_cu<T>(own.wide, own.threads, own.lds);
+            const int per_cu = own.multi  ? line_multi_batch_blocks_per_cu<T>(own.threads, own.lds)
+                               : own.fast ? line_fast_batch_blocks_per_cu<T>(own.fr[0], own.fr[1], own.fr[2],
+                                                                             own.fem, own.threads, own.lds)
+                                          : line_batch_blocks_per_cu<T>(own.wide, own.threads, own.lds);
             own.grid = std::min<int64_t>(own.total, (int64_t)per_cu * cus);
         }
     }
@@ -2092,7 +2100,10 @@ static pfh_status batch_launch(pfh_batch_s &B, hipStream_t stream)
 {
     for (const BatchStage &bs : B.stages) {
         const StageArgs<T> *items = reinterpret_cast<const StageArgs<T> *>(bs.d_items);
-        if (bs.fast)
+        if (bs.multi)
+            PFH_HIP_CHECK(launch_line_multi_batch<T>(items, bs.d_prefix, bs.nitems, bs.total, bs.threads, bs.lds,
+                                                     bs.grid, stream));
+        else if (bs.fast)
             PFH_HIP_CHECK(launch_line_fast_batch<T>(items, bs.d_prefix, bs.nitems, bs.total, bs.fr[0], bs.fr[1],
                                                     bs.fr[2], bs.fem, bs.threads, bs.lds, bs.grid, stream));
         else
@@ -2155,8 +2166,10 @@ static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int coun
     std::vector<Tuning> cands;
     for (int block : {128, 256, 512})
         for (int maxr : {8, 16})
-            for (int col : {0, 32}) {
+            for (int col : {0, 32})
+                for (bool multi : {true, false}) {
                 Tuning t;
+                t.batch_multi = multi;
                 t.block = block;
                 t.max_radix = maxr;
                 t.col_lines = col;
@@ -2166,7 +2179,7 @@ static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int coun
                 t.slab_mb = 0;
                 t.fill_block = true;
                 cands.push_back(t);
-            }
+                }
     // scratch buffers for the measurement
     std::vector<void *> bufs;
     std::vector<const void *> srcs(count);
@@ -2266,12 +2279,13 @@ pfh_status pfh_batch_workspace_bytes(pfh_batch batch, size_t *bytes)
     return ok();
 }
 
-pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines)
+pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines, int *one_launch)
 {
     if (!batch) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
     if (block) *block = batch->tuning.block;
     if (max_radix) *max_radix = batch->tuning.max_radix;
     if (col_lines) *col_lines = batch->tuning.col_lines;
+    if (one_launch) *one_launch = batch->tuning.batch_multi ? 1 : 0;
     return ok();
 }
 

@@ -1032,6 +1032,49 @@ __global__ __launch_bounds__(512) void k_line_fast_batch(const StageArgs<T> *__r
     }
 }
 
+// Persistent batched stage over items with different compile-time radix
+// plans: each tile dispatches on its item's plan (uniform per workgroup).
+// Plans with 32 values per thread are left out (they would raise the
+// kernel's register count for every item).
+template <typename T>
+__global__ __launch_bounds__(512) void k_line_multi_batch(const StageArgs<T> *__restrict__ items,
+                                                         const int64_t *__restrict__ prefix, int nitems,
+                                                         int64_t total)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T> *tws = reinterpret_cast<cpx<T> *>(smem_raw);
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    int q = 0;
+    {
+        const int64_t w0 = blockIdx.x;
+        int lo = 0, hi = nitems;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (prefix[mid] <= w0) lo = mid;
+            else hi = mid;
+        }
+        q = lo;
+    }
+    for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        while (q + 1 < nitems && prefix[q + 1] <= w) ++q;
+        const StageArgs<T> &a = items[q];
+        const int64_t tile = w - prefix[q];
+        __syncthreads(); // the previous tile is done with LDS
+        for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
+        const int r0 = a.radix[0], r1 = a.radix[1], r2 = a.npass > 2 ? a.radix[2] : 0, em = a.fem;
+#define PFH_MULTI_CASE(TT, A, B, C, E)                                                                        \
+    if constexpr (std::is_same<T, TT>::value && E != 32) {                                                   \
+        if (r0 == A && r1 == B && r2 == C && em == E) {                                                      \
+            fast_body<T, A, B, C, E>(a, tile, tws, tws + a.twn);                                            \
+            continue;                                                                                        \
+        }                                                                                                    \
+    }
+        PFH_FAST_PLANS(PFH_MULTI_CASE)
+#undef PFH_MULTI_CASE
+    }
+}
+
 template <typename T>
 hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int em, int threads, size_t lds_bytes,
                             int64_t grid, hipStream_t s)
@@ -1359,6 +1402,33 @@ template hipError_t launch_line_fast_batch<double>(const StageArgs<double> *, co
                                                    int, int, int, size_t, int64_t, hipStream_t);
 
 template <typename T>
+hipError_t launch_line_multi_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total,
+                                   int threads, size_t lds_bytes, int64_t grid, hipStream_t s)
+{
+    if (total <= 0 || grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_line_multi_batch<T>, dim3((unsigned)grid), dim3(threads), lds_bytes, s, items, prefix, nitems,
+                       total);
+    return hipGetLastError();
+}
+template hipError_t launch_line_multi_batch<float>(const StageArgs<float> *, const int64_t *, int, int64_t, int,
+                                                   size_t, int64_t, hipStream_t);
+template hipError_t launch_line_multi_batch<double>(const StageArgs<double> *, const int64_t *, int, int64_t, int,
+                                                    size_t, int64_t, hipStream_t);
+
+template <typename T>
+int line_multi_batch_blocks_per_cu(int threads, size_t lds_bytes)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_line_multi_batch<T>, threads, lds_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n > 0 ? n : 1;
+}
+template int line_multi_batch_blocks_per_cu<float>(int, size_t);
+template int line_multi_batch_blocks_per_cu<double>(int, size_t);
+
+template <typename T>
 int line_fast_batch_blocks_per_cu(int r0, int r1, int r2, int em, int threads, size_t lds_bytes)
 {
     int n = 0;
@@ -1396,7 +1466,9 @@ hipError_t set_line_stage_lds_limit(size_t bytes)
                          reinterpret_cast<const void *>(&k_line_batch<float, true>),
                          reinterpret_cast<const void *>(&k_line_batch<float, false>),
                          reinterpret_cast<const void *>(&k_line_batch<double, true>),
-                         reinterpret_cast<const void *>(&k_line_batch<double, false>)};
+                         reinterpret_cast<const void *>(&k_line_batch<double, false>),
+                         reinterpret_cast<const void *>(&k_line_multi_batch<float>),
+                         reinterpret_cast<const void *>(&k_line_multi_batch<double>)};
     for (const void *f : fns) {
         hipError_t e = raise_lds_limit(f, bytes);
         if (e != hipSuccess) return e;

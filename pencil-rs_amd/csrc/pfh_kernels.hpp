@@ -60,6 +60,12 @@ hipError_t launch_line_fast_batch(const StageArgs<T> *items, const int64_t *pref
                                   int r1, int r2, int em, int threads, size_t lds_bytes, int64_t grid, hipStream_t s);
 template <typename T>
 int line_fast_batch_blocks_per_cu(int r0, int r1, int r2, int em, int threads, size_t lds_bytes);
+// items with any compile-time plan of 16 (f32) / 8 (f64) values per thread
+template <typename T>
+hipError_t launch_line_multi_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total,
+                                   int threads, size_t lds_bytes, int64_t grid, hipStream_t s);
+template <typename T>
+int line_multi_batch_blocks_per_cu(int threads, size_t lds_bytes);
 
 // Stages with a compile-time radix sequence (see k_line_fast).
 template <typename T>

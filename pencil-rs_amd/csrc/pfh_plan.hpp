@@ -133,6 +133,7 @@ struct Tuning {
     bool allow_xcd = true;
     int slab_mb = -1;    // slab intermediate target in MiB (0: no slab pipelining, -1: default)
     bool fill_block = false; // column-access tiles also take `block` threads (batched launches)
+    bool batch_multi = true; // batched stages: one launch over every compile-time plan
 };
 
 struct PlanImpl {

@@ -89,6 +89,7 @@ struct StageArgs {
     int twoff[kMaxPasses];  // pass twiddle offsets (complex units) into tw
     int pre0, pre1, post0, post1; // table offsets into tw
     int twn;                // complex entries of tw staged into LDS
+    int fem;                // values per thread and pass of the compile-time plan (0: generic kernel)
     int lt, tpl;            // lines per tile, threads per line
     int ls;                 // LDS stride of one line (complex units)
     int in_col, out_col;    // column (lines adjacent in memory) access on load / store

@@ -163,7 +163,7 @@ def load_library(path: str = LIB_PATH):
     lib.pfh_batch_workspace_bytes.argtypes = [vp, ctypes.POINTER(sz)]
     lib.pfh_batch_execute.argtypes = [vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), vp, sz, vp, i]
     ip = ctypes.POINTER(i)
-    lib.pfh_batch_tuning.argtypes = [vp, ip, ip, ip]
+    lib.pfh_batch_tuning.argtypes = [vp, ip, ip, ip, ip]
     d = ctypes.c_double
     lib.pfh_pointwise_scale_axes.argtypes = [vp, i, vp, sz, vp, vp]
     lib.pfh_pointwise_quadratic.argtypes = [i, vp, vp, sz, d, vp]
@@ -632,10 +632,11 @@ class BatchC2cPlan:
             self._h = None
 
     def tuning(self):
-        """(threads per block target, largest radix, lines per column tile)."""
-        b, r, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(_lib.pfh_batch_tuning(self._h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(c)))
-        return b.value, r.value, c.value
+        """(threads per block target, largest radix, lines per column tile,
+        one launch per stage over all compile-time plans)."""
+        b, r, c, m = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(_lib.pfh_batch_tuning(self._h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(c), ctypes.byref(m)))
+        return b.value, r.value, c.value, bool(m.value)
 
     def allocate_workspace(self) -> torch.Tensor:
         return torch.zeros(max(self.workspace_bytes, 256), dtype=torch.uint8, device="cuda")

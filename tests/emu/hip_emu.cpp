@@ -406,6 +406,43 @@ template hipError_t launch_line_fast_batch<double>(const StageArgs<double> *, co
                                                    int, int, int, size_t, int64_t, hipStream_t);
 
 template <typename T>
+hipError_t launch_line_multi_batch(const StageArgs<T> *items, const int64_t *prefix, int nitems, int64_t total,
+                                   int threads, size_t lds_bytes, int64_t grid, hipStream_t)
+{
+    // every item must name an instantiated plan of the standard register budget
+    for (int q = 0; q < nitems; ++q) {
+        const StageArgs<T> &a = items[q];
+        const int r2 = a.npass > 2 ? a.radix[2] : 0;
+        if (!fast_plan_available((int)sizeof(T), a.radix[0], a.radix[1], r2, a.fem) || a.fem == 32)
+            fail("line_multi_batch: item %d has no plan in the kernel", q);
+    }
+    ++g_launches;
+    begin_launch();
+    if (threads > 512) fail("line_multi_batch: %d threads", threads);
+    if (grid <= 0 || grid > total) fail("line_multi_batch: grid");
+    if (prefix[0] != 0 || prefix[nitems] != total) fail("line_multi_batch: prefix ends");
+    for (int q = 0; q < nitems; ++q) {
+        const StageArgs<T> &a = items[q];
+        if (prefix[q + 1] - prefix[q] != a.ntiles) fail("line_multi_batch: item %d tiles", q);
+        check_shape("line_multi_batch", a, threads, lds_bytes, Regs<T>::emax, true, a.fem);
+        emulate("line_multi_batch", a);
+    }
+    return hipSuccess;
+}
+template hipError_t launch_line_multi_batch<float>(const StageArgs<float> *, const int64_t *, int, int64_t, int,
+                                                   size_t, int64_t, hipStream_t);
+template hipError_t launch_line_multi_batch<double>(const StageArgs<double> *, const int64_t *, int, int64_t, int,
+                                                    size_t, int64_t, hipStream_t);
+
+template <typename T>
+int line_multi_batch_blocks_per_cu(int, size_t)
+{
+    return 2;
+}
+template int line_multi_batch_blocks_per_cu<float>(int, size_t);
+template int line_multi_batch_blocks_per_cu<double>(int, size_t);
+
+template <typename T>
 int line_batch_blocks_per_cu(bool, int, size_t)
 {
     return 2;
