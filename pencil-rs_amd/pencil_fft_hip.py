@@ -15,7 +15,8 @@ from typing import Iterable, Optional, Sequence
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpencilfft_hip.so")
+# PFH_LIBRARY names another build of the library (A/B measurements).
+LIB_PATH = os.environ.get("PFH_LIBRARY") or os.path.join(_HERE, "libpencilfft_hip.so")
 MAX_DIMS = 4
 
 # pfh_kind
