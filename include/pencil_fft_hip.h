@@ -98,7 +98,15 @@ typedef enum pfh_kind {
     PFH_R2R = 7,        /* R2rPlan / DhtPlan  distributed/r2r.rs:71-1460 */
     /* layout permutations */
     PFH_LOCAL_TRANSPOSE = 8,    /* LocalTransposePlan       local_transpose.rs:46-277 */
-    PFH_ALLTOALLV_TRANSPOSE = 9 /* AllToAllvTransposePlan   alltoallv_transpose.rs:50-311 (one peer) */
+    PFH_ALLTOALLV_TRANSPOSE = 9, /* AllToAllvTransposePlan  alltoallv_transpose.rs:50-311 (one peer) */
+    /* C2rPlan (distributed/c2r.rs:72-361): the real route in C2R orientation
+     * (construct_oriented, distributed/r2c.rs:1204-1448). shape is the real
+     * shape; the reduction axis is the first selected axis; the complex side
+     * is in the canonical memory order and the real side in the route's last
+     * order. FORWARD = raw C2R (with the endpoint-plane check), BACKWARD = raw
+     * R2C, INVERSE = R2C normalized by every selected real extent. Out of
+     * place only, like the reference. */
+    PFH_C2R = 10
 } pfh_kind;
 
 /* R2R kind codes = the reference's descriptor codes (r2r.rs:53-64, :101-106). */

@@ -44,7 +44,13 @@ enum { PFO_OK = 0, PFO_ERR_ARG = 1, PFO_ERR_INVALID_SPECTRUM = 2 };
 
 enum { PFO_DCT1 = 1, PFO_DCT2, PFO_DCT3, PFO_DCT4, PFO_DST1, PFO_DST2, PFO_DST3, PFO_DST4, PFO_DHT };
 
-enum { PFO_KIND_C2C = 1, PFO_KIND_R2C = 2, PFO_KIND_R2R = 3 };
+/* PFO_KIND_C2R: the C2R orientation of the real route (C2rPlan,
+ * distributed/c2r.rs:72-361, construct_oriented distributed/r2c.rs:1204-1448):
+ * reduction axis = first selected axis, stage order and transitions
+ * reversed; the complex side is in the canonical memory order. Its ops:
+ * FORWARD = raw C2R, BACKWARD = raw R2C, INVERSE = R2C normalized by every
+ * selected real extent (c2r.rs:340-360), and their adjoints. */
+enum { PFO_KIND_C2C = 1, PFO_KIND_R2C = 2, PFO_KIND_R2R = 3, PFO_KIND_C2R = 4 };
 
 enum {
     PFO_OP_FORWARD = 0,

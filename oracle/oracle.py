@@ -24,7 +24,7 @@ _lib = None
 
 OK, ERR_ARG, ERR_INVALID_SPECTRUM = 0, 1, 2
 DCT1, DCT2, DCT3, DCT4, DST1, DST2, DST3, DST4, DHT = range(1, 10)
-KIND_C2C, KIND_R2C, KIND_R2R = 1, 2, 3
+KIND_C2C, KIND_R2C, KIND_R2R, KIND_C2R = 1, 2, 3, 4
 FORWARD, INVERSE, BACKWARD, ADJ_FORWARD, ADJ_INVERSE, ADJ_BACKWARD = range(6)
 MAXD = 4
 
@@ -173,15 +173,16 @@ def make_desc(kind, shape, extra=1, selected=None, r2r=None, backward_dir=None, 
 
 
 def plan_shapes(kind, shape, selected=None):
-    """(input logical shape, output logical shape) of the forward operation."""
+    """(input logical shape, output logical shape) of the forward operation
+    (for KIND_C2R the forward operation is the C2R: complex in, real out)."""
     shape = tuple(shape)
-    if kind != KIND_R2C:
+    if kind not in (KIND_R2C, KIND_C2R):
         return shape, shape
     sel = list(range(len(shape))) if selected is None else list(selected)
-    red = max(sel)
-    out = list(shape)
-    out[red] = shape[red] // 2 + 1
-    return shape, tuple(out)
+    red = max(sel) if kind == KIND_R2C else min(sel)
+    half = list(shape)
+    half[red] = shape[red] // 2 + 1
+    return (shape, tuple(half)) if kind == KIND_R2C else (tuple(half), shape)
 
 
 def execute(desc: _Desc, op: int, src: np.ndarray, out_dtype, out_count: int, threads: int = 0):
@@ -210,11 +211,16 @@ def transform(kind, x: np.ndarray, op=FORWARD, shape=None, extra=1, selected=Non
         count, dt = int(np.prod(shape)) * extra, cd
     elif kind == KIND_R2R:
         count, dt = int(np.prod(shape)) * extra, real
-    else:
+    elif kind == KIND_R2C:
         if forward_like:
             count, dt = int(np.prod(shp_out)) * extra, cd
         else:
             count, dt = int(np.prod(shp_in)) * extra, real
+    else:  # C2R orientation: the forward operation is complex -> real
+        if forward_like:
+            count, dt = int(np.prod(shp_out)) * extra, real
+        else:
+            count, dt = int(np.prod(shp_in)) * extra, cd
     return execute(desc, op, x.reshape(-1), dt, count, threads)
 
 

@@ -699,36 +699,65 @@ static Layout make_layout(int N, const std::array<int, 8> &perm, const std::arra
 }
 
 // Stage specs of one operation in execution order.
-static pfh_status route_specs(const PlanImpl &P, int op, std::vector<StageSpec> &specs)
+// The real route's operation that a plan operation runs: a C2rPlan runs the
+// route in C2R orientation, its FORWARD being the raw C2R (the route's
+// BACKWARD), BACKWARD the raw R2C (FORWARD) and INVERSE the R2C normalized
+// by the selected real extents (c2r.rs:306-360).
+static int route_op(int kind, int op)
+{
+    if (kind != PFH_C2R) return op;
+    switch (op) {
+    case PFH_FORWARD: return PFH_BACKWARD;
+    case PFH_BACKWARD: return PFH_FORWARD;
+    case PFH_INVERSE: return PFH_FORWARD;
+    case PFH_ADJOINT_FORWARD: return PFH_ADJOINT_BACKWARD;
+    case PFH_ADJOINT_BACKWARD: return PFH_ADJOINT_FORWARD;
+    case PFH_ADJOINT_INVERSE: return PFH_ADJOINT_FORWARD;
+    default: return op;
+    }
+}
+
+static pfh_status route_specs(const PlanImpl &P, int op_in, std::vector<StageSpec> &specs)
 {
     const pfh_desc &d = P.desc;
     const int N = P.N;
     const int tb = (P.precision == PFH_F32) ? 4 : 8;
-    std::vector<std::array<int, 8>> perm;
-    build_route(N, d.permute_dims != 0, perm);
+    // C2R orientation: stage s transforms axis s in the permutation of route
+    // stage N-1-s (stages and transitions reversed, r2c.rs:1340-1348)
+    const bool c2r = P.kind == PFH_C2R;
+    const bool real_kind = P.kind == PFH_R2C || c2r;
+    const bool scaled = op_in == PFH_INVERSE || op_in == PFH_ADJOINT_INVERSE;
+    const int op = route_op(P.kind, op_in);
+    std::vector<std::array<int, 8>> route, perm;
+    build_route(N, d.permute_dims != 0, route);
+    perm.resize(N);
+    for (int s = 0; s < N; ++s) perm[s] = route[c2r ? N - 1 - s : s];
     std::array<int64_t, 8> orig{}, red{};
     for (int a = 0; a < N; ++a) orig[a] = red[a] = (int64_t)d.shape[a];
     unsigned sel = d.selected ? d.selected : ((1u << N) - 1u);
     int boundary = -1, raxis = -1;
-    if (P.kind == PFH_R2C) {
-        for (int a = N - 1; a >= 0; --a)
+    if (real_kind) {
+        // R2C: the last selected axis; C2R orientation: the first
+        for (int i = 0; i < N; ++i) {
+            const int a = c2r ? i : N - 1 - i;
             if (sel & (1u << a)) {
                 raxis = a;
                 break;
             }
-        boundary = N - 1 - raxis;
+        }
+        boundary = c2r ? raxis : N - 1 - raxis;
         red[raxis] = orig[raxis] / 2 + 1;
     }
     const bool fwd = (op == PFH_FORWARD || op == PFH_ADJOINT_INVERSE || op == PFH_ADJOINT_BACKWARD);
     std::vector<StageSpec> by_stage(N);
     for (int s = 0; s < N; ++s) {
         StageSpec sp;
-        const int axis = N - 1 - s;
+        const int axis = c2r ? s : N - 1 - s;
         sp.axis = axis;
         // forward-orientation shapes / element kinds
         std::array<int64_t, 8> sh_in = orig, sh_out = orig;
         bool c_in = true, c_out = true;
-        if (P.kind == PFH_R2C) {
+        if (real_kind) {
             sh_in = (s <= boundary) ? orig : red;
             sh_out = (s < boundary) ? orig : red;
             c_in = s > boundary;
@@ -746,7 +775,7 @@ static pfh_status route_specs(const PlanImpl &P, int op, std::vector<StageSpec> 
                 sp.sign_pos = fwd ? !neg_forward : neg_forward;
                 if (op == PFH_INVERSE || op == PFH_ADJOINT_INVERSE) sp.scale = 1.0 / (double)orig[axis];
             }
-        } else if (P.kind == PFH_R2C) {
+        } else if (real_kind) {
             if (s < boundary) {
                 sp.kind = SK_COPY;
                 sp.n = orig[axis];
@@ -757,7 +786,7 @@ static pfh_status route_specs(const PlanImpl &P, int op, std::vector<StageSpec> 
                 } else {
                     sp.kind = SK_C2C;
                     sp.sign_pos = !fwd;
-                    if (op == PFH_INVERSE || op == PFH_ADJOINT_INVERSE) sp.scale = 1.0 / (double)red[axis];
+                    if (scaled) sp.scale = 1.0 / (double)red[axis];
                 }
             } else {
                 sp.n = orig[axis];
@@ -768,7 +797,7 @@ static pfh_status route_specs(const PlanImpl &P, int op, std::vector<StageSpec> 
                     sp.kind = SK_C2R;
                     if (op == PFH_ADJOINT_FORWARD) sp.flags |= F_HERM_HALF;
                 }
-                if (op == PFH_INVERSE || op == PFH_ADJOINT_INVERSE) sp.scale = 1.0 / (double)orig[axis];
+                if (scaled) sp.scale = 1.0 / (double)orig[axis];
             }
         } else { // R2R / DHT
             const int k = d.axis_kind[axis];
@@ -957,7 +986,8 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
     const int64_t target = in_place ? 0 : slab_target_bytes(P.tuning);
     std::vector<char> checked(K, 0);
     for (int e = 0; e < K; ++e)
-        checked[e] = (P.kind == PFH_R2C && specs[e].kind == SK_C2R && (op == PFH_INVERSE || op == PFH_BACKWARD));
+        checked[e] = ((P.kind == PFH_R2C || P.kind == PFH_C2R) && specs[e].kind == SK_C2R &&
+                      (route_op(P.kind, op) == PFH_INVERSE || route_op(P.kind, op) == PFH_BACKWARD));
     auto slab_axis_of = [&](int e, const Layout &xin, const Layout &mid) -> int {
         if (target <= 0 || e + 1 >= K || checked[e] || checked[e + 1]) return -1;
         const int A = xin.perm[0];
@@ -991,7 +1021,7 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
         s.out_buf = ob[e];
         st = finish_step(s, P.dev, tb, P.tuning);
         if (st) return st;
-        if (P.kind == PFH_R2C && specs[e].kind == SK_C2R && (op == PFH_INVERSE || op == PFH_BACKWARD)) {
+        if (checked[e]) {
             // acceptance check on the C2R stage's input (distributed/r2c.rs:4709-4807)
             const pfh_desc &d = P.desc;
             unsigned sel = d.selected ? d.selected : ((1u << P.N) - 1u);
@@ -1030,7 +1060,7 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
                 P.partials = std::max<int64_t>(P.partials, ep.extra * ep.planes * nb);
             }
             ep.relative = 128.0 * eps * depth;
-            ep.absolute = 128.0 * minsub * depth * ((op == PFH_BACKWARD) ? transverse : 1.0);
+            ep.absolute = 128.0 * minsub * depth * ((route_op(P.kind, op) == PFH_BACKWARD) ? transverse : 1.0);
             s.endpoint_check = true;
             s.ep = ep;
         }
@@ -1216,14 +1246,15 @@ static pfh_status create_impl(const pfh_desc *d, PlanImpl &P)
     }
     case PFH_C2C:
     case PFH_R2C:
-    case PFH_R2R: {
+    case PFH_R2R:
+    case PFH_C2R: {
         const int N = d->ndim;
         const int M = d->topology_dims ? d->topology_dims : N - 1;
         if (N < 2 || N > PFH_MAX_DIMS || M < 1 || M >= N) return set_err(PFH_ERR_INVALID_DIMENSIONS, "N >= 2, 1 <= M < N");
         P.N = N;
         for (int a = 0; a < N; ++a)
             if (d->shape[a] == 0) return set_err(PFH_ERR_ZERO_EXTENT, "zero global extent", 0, 0, (size_t)a);
-        if (P.kind == PFH_R2C) {
+        if (P.kind == PFH_R2C || P.kind == PFH_C2R) {
             unsigned sel = d->selected ? d->selected : ((1u << N) - 1u);
             if ((sel & ((1u << N) - 1u)) == 0) return set_err(PFH_ERR_INVALID_DIMENSIONS, "empty selection");
         }
@@ -1237,6 +1268,7 @@ static pfh_status create_impl(const pfh_desc *d, PlanImpl &P)
         for (int op = 0; op < 6; ++op) {
             st = build_op(P, op, false, P.ops[op]);
             if (st) return st;
+            if (P.kind == PFH_C2R) continue; // C2rPlan is out of place only (c2r.rs:69-70)
             st = build_op(P, op, true, P.ops_inplace[op]);
             if (st) return st;
         }
@@ -1797,6 +1829,7 @@ pfh_status pfh_execute_in_place(pfh_plan plan, int op, void *data, size_t capaci
                             (size_t)o.out.total(), nullptr, 0, stream, flags, nullptr);
     }
     if (P.kind <= PFH_LOCAL_DHT) return set_err(PFH_ERR_INVALID_ARGUMENT, "local plans: use pfh_local_execute_in_place");
+    if (P.kind == PFH_C2R) return set_err(PFH_ERR_INVALID_ARGUMENT, "a C2R plan is out of place only");
     const OpPlan &o = P.ops_inplace[op];
     if (!o.valid) return set_err(PFH_ERR_INVALID_ARGUMENT, "op");
     const int64_t need = std::max(o.in.bytes(), o.out.bytes());
@@ -2318,7 +2351,8 @@ pfh_status pfh_pointwise_scale_axes(pfh_plan plan, int side, void *data, size_t 
 {
     if (!plan || !data || !factors || (side != 0 && side != 1)) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
     const PlanImpl &P = plan->impl;
-    if (P.kind < PFH_C2C || P.kind > PFH_R2R) return set_err(PFH_ERR_INVALID_ARGUMENT, "needs a C2C, R2C or R2R plan");
+    if (!(P.kind == PFH_C2C || P.kind == PFH_R2C || P.kind == PFH_R2R || P.kind == PFH_C2R))
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "needs a C2C, R2C, R2R or C2R plan");
     const OpPlan &o = P.ops[PFH_FORWARD];
     const Layout &L = side == 0 ? o.in : o.out;
     if ((int64_t)elems != L.total())

@@ -20,7 +20,7 @@ LIB_PATH = os.environ.get("PFH_LIBRARY") or os.path.join(_HERE, "libpencilfft_hi
 MAX_DIMS = 4
 
 # pfh_kind
-LOCAL_C2C, LOCAL_R2C, LOCAL_R2R, LOCAL_DHT, C2C, R2C, R2R, LOCAL_TRANSPOSE, ALLTOALLV_TRANSPOSE = range(1, 10)
+LOCAL_C2C, LOCAL_R2C, LOCAL_R2R, LOCAL_DHT, C2C, R2C, R2R, LOCAL_TRANSPOSE, ALLTOALLV_TRANSPOSE, C2R_PLAN = range(1, 11)
 # pfh_op
 FORWARD, INVERSE, BACKWARD, ADJOINT_FORWARD, ADJOINT_INVERSE, ADJOINT_BACKWARD = range(6)
 EXEC_DEFER_STATUS = 1
@@ -532,6 +532,24 @@ class R2cPlan(_RoutePlan):
         d = _route_desc(R2C, shape, dtype, extra_shape, selection, permute_dims, topology_dims)
         super().__init__(d)
         self.shape = tuple(shape)
+
+
+class C2rPlan(_RoutePlan):
+    """C2rPlan (crates/pencil-fft/src/distributed/c2r.rs:72-361) on a
+    one-rank process grid: the real route in C2R orientation. `shape` is the
+    real shape; the reduction axis is the first selected axis. forward = raw
+    C2R from the canonical complex pencil (endpoint planes checked, raises
+    PfhError("InvalidSpectrum")), backward = raw R2C, inverse = R2C
+    normalized by every selected real extent. Out of place only."""
+
+    def __init__(self, shape: Sequence[int], dtype=torch.float32, extra_shape=(), selection=None,
+                 permute_dims=True, topology_dims=0):
+        d = _route_desc(C2R_PLAN, shape, dtype, extra_shape, selection, permute_dims, topology_dims)
+        super().__init__(d)
+        self.shape = tuple(shape)
+
+    def execute_in_place(self, op, data, workspace=None, flags=0):
+        raise ValueError("C2rPlan is out of place only (distributed/c2r.rs:69-70)")
 
 
 class R2rPlan(_RoutePlan):

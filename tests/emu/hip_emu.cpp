@@ -778,6 +778,21 @@ void route_cases(int prec)
             run_case<T>(name, p, nd, PFH_BACKWARD, false, true, ra, (int64_t)c.shape[ra]);
             pfh_plan_destroy(p);
         }
+        // C2R orientation (C2rPlan): forward = C2R from the canonical complex side
+        {
+            pfh_desc d = route_desc(PFH_C2R, prec, c.shape, c.extra);
+            pfh_plan p = nullptr;
+            if (pfh_plan_create(&d, &p) != PFH_OK) {
+                fail("C2R %s %s: plan", pn, sh.c_str());
+                continue;
+            }
+            std::snprintf(name, sizeof(name), "C2Rplan %s %s", pn, sh.c_str());
+            run_case<T>(name, p, nd, PFH_FORWARD, false, true, 0, (int64_t)c.shape[0]);
+            run_case<T>(name, p, nd, PFH_BACKWARD, true, false, 0, (int64_t)c.shape[0]);
+            run_case<T>(name, p, nd, PFH_INVERSE, true, false, 0, (int64_t)c.shape[0]);
+            run_case<T>(name, p, nd, PFH_ADJOINT_FORWARD, true, false, 0, (int64_t)c.shape[0]);
+            pfh_plan_destroy(p);
+        }
         // R2R (DCT-II, DST-I, DCT-IV, DHT per axis)
         {
             pfh_desc d = route_desc(PFH_R2R, prec, c.shape, c.extra);

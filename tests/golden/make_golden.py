@@ -8,6 +8,9 @@ Cases:
 - r2r: the eight FFTW kinds via scipy.fft.dct/dst(type, norm=None), which
   uses the FFTW REDFT/RODFT unnormalized convention, plus the DHT
   (Re F - Im F);
+- C2R-oriented routes (C2rPlan, crates/pencil-fft/src/distributed/c2r.rs:72-361):
+  numpy.fft.rfftn spectra of fixture fields, raw C2R output = field x the
+  selected extents, in the reversed memory order;
 - routes: the reference's fixture case list
   (tools/fftw-reference/README.md:271-294) with its input formulas
   (tools/fftw-reference/generate.jl:201-250), transformed with numpy.fft /
@@ -115,6 +118,21 @@ def main():
         g[f"route_r2c{i}_fwd"] = out_order(y)
         g[f"route_r2c{i}_meta"] = np.array(list(shape) + [extra] + [-1] + (list(sel) if sel is not None else
                                                                           list(range(len(shape)))))
+    # C2R orientation (C2rPlan): reduction axis = first selected axis, the
+    # complex input in canonical order, the raw C2R output in the reversed
+    # memory order; the input is the spectrum of a fixture field, so the raw
+    # output is the field times the product of the selected extents
+    c2r = [((3, 4), 1, None), ((3, 2, 5), 2, None), ((4, 3, 5), 1, None), ((2, 3, 2, 3), 1, [1, 3]),
+           ((5, 4, 6), 1, [0, 2]), ((6,), 3, None)]
+    for i, (shape, extra, sel) in enumerate(c2r):
+        x = field(shape, extra, 4.0, False)
+        sel_l = list(range(len(shape))) if sel is None else list(sel)
+        ra = min(sel_l)
+        axes = tuple(a + 1 for a in sel_l if a != ra) + (ra + 1,)
+        X = np.fft.rfftn(x, axes=axes)
+        g[f"route_c2r{i}_in"] = X
+        g[f"route_c2r{i}_fwd"] = out_order(x * float(np.prod([shape[a] for a in sel_l])))
+        g[f"route_c2r{i}_meta"] = np.array(list(shape) + [extra] + [-1] + sel_l)
     r2r = [((3, 4), 1, [9, 9]), ((3, 2, 4), 6, [9, 0, 9]), ((2, 3, 2, 3), 1, [9, 0, 0, 9]),
            ((2, 3, 2, 3), 1, [1, 2, 3, 4]), ((2, 3, 2, 3), 1, [5, 6, 7, 8]), ((6, 5, 4), 2, [2, 5, 4])]
     for i, (shape, extra, kinds) in enumerate(r2r):

@@ -162,6 +162,55 @@ def test_r2c_inverse_rejects_invalid_spectrum_without_writing(gpu):
     assert rel_l2(back, x.reshape(-1)) <= 1e-12
 
 
+C2R_CASES = [
+    ((3, 4), 1, None),
+    ((3, 2, 5), 2, None),
+    ((4, 3, 5), 1, None),
+    ((2, 3, 2, 3), 1, [1, 3]),
+    ((5, 4, 6), 1, [0, 2]),
+    ((16, 12, 8), 1, None),
+    ((9, 10, 16), 3, None),
+    ((12, 10, 16), 1, [1, 2]),
+]
+
+
+@pytest.mark.parametrize("permute", [True, False])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("shape,extra,sel", C2R_CASES)
+def test_c2r_plan_matches_oracle(gpu, shape, extra, sel, dt, permute):
+    """C2rPlan (distributed/c2r.rs:72-361): all six operations against the
+    oracle's C2R orientation; the forward input is the (valid) spectrum of a
+    fixture field in the canonical memory order."""
+    kw = dict(shape=shape, extra=extra, selected=sel, permute_dims=permute)
+    # a fixture field as the plan's real array (its memory order does not
+    # matter for the comparison), and its normalized spectrum as the forward input
+    real_in = fixture_field(shape, extra, 4.0, False).reshape(-1)
+    spec = O.transform(O.KIND_C2R, real_in, O.INVERSE, **kw)
+    plan = pf.C2rPlan(shape, dtype=torch.float32 if dt == np.float32 else torch.float64,
+                      extra_shape=(extra,) if extra > 1 else (), selection=sel, permute_dims=permute)
+    x_in = spec.astype(complex_of(dt))
+    x_out = real_in.astype(real_of(dt))
+    run_all_ops(plan, O.KIND_C2R, shape, extra, x_in, x_out, dt, selected=sel, permute_dims=permute)
+
+
+def test_c2r_plan_forward_rejects_invalid_spectrum_without_writing(gpu):
+    shape = (6, 4, 8)
+    rng = np.random.default_rng(5)
+    plan = pf.C2rPlan(shape, dtype=torch.float64)
+    real = uniform_real(rng, shape, np.float64).reshape(-1)
+    spec = _host(plan.backward(_tdev(real))).copy()
+    spec[5] += 0.25j  # a DC-plane value of the reduction axis 0 (canonical order: axis 0 slowest)
+    dst = torch.full((real.size,), -1.0, dtype=torch.float64, device="cuda")
+    with pytest.raises(pf.PfhError) as e:
+        plan.forward(_tdev(spec), dst)
+    assert e.value.variant == "InvalidSpectrum"
+    assert torch.all(dst == -1.0)
+    with pytest.raises(O.InvalidSpectrum):
+        O.transform(O.KIND_C2R, spec, O.FORWARD, shape=shape)
+    with pytest.raises(ValueError):
+        plan.execute_in_place(pf.FORWARD, _tdev(spec))
+
+
 R2R_CASES = [
     ((3, 4), 1, [pf.R2rKind.Dht, pf.R2rKind.Dht]),
     ((3, 2, 4), 6, [pf.R2rKind.Dht, None, pf.R2rKind.Dht]),

@@ -192,7 +192,8 @@ def _meta(m):
 
 
 def test_routes_match_numpy_goldens(gold):
-    for kind, prefix in ((O.KIND_C2C, "route_c2c"), (O.KIND_R2C, "route_r2c"), (O.KIND_R2R, "route_r2r")):
+    for kind, prefix in ((O.KIND_C2C, "route_c2c"), (O.KIND_R2C, "route_r2c"), (O.KIND_R2R, "route_r2r"),
+                         (O.KIND_C2R, "route_c2r")):
         i = 0
         while f"{prefix}{i}_in" in gold.files:
             shape, extra, tail = _meta(gold[f"{prefix}{i}_meta"])
@@ -248,6 +249,24 @@ def test_oracle_adjoints_satisfy_dot_identity(kind, shape, kw):
     for op, adj in ((O.INVERSE, O.ADJ_INVERSE), (O.BACKWARD, O.ADJ_BACKWARD)):
         bx = O.transform(kind, ax, op, shape=shape, **kw)
         btz = O.transform(kind, z, adj, shape=shape, **kw)
+        assert abs(_dot(bx, z) - _dot(ax, btz)) <= 1e-13 * abs(_dot(bx, z))
+
+
+@pytest.mark.parametrize("shape,kw", [((6, 5, 4), {}), ((5, 6, 7), {}), ((4, 6, 5), {"selected": [1, 2]})])
+def test_oracle_c2r_orientation_adjoints_satisfy_dot_identity(shape, kw):
+    """C2rPlan orientation: FORWARD (raw C2R) from a valid canonical spectrum,
+    then the R2C-direction ops (INVERSE, BACKWARD) from the real side."""
+    rng = np.random.default_rng(sum(shape))
+    r = rng.standard_normal(int(np.prod(shape)))
+    x = O.transform(O.KIND_C2R, r, O.BACKWARD, shape=shape, **kw)
+    ax = O.transform(O.KIND_C2R, x, O.FORWARD, shape=shape, **kw)
+    y = rng.standard_normal(ax.shape)
+    aty = O.transform(O.KIND_C2R, y, O.ADJ_FORWARD, shape=shape, **kw)
+    assert abs(_dot(ax, y) - _dot(x, aty)) <= 1e-13 * abs(_dot(ax, y))
+    z = rng.standard_normal(x.shape) + 1j * rng.standard_normal(x.shape)
+    for op, adj in ((O.INVERSE, O.ADJ_INVERSE), (O.BACKWARD, O.ADJ_BACKWARD)):
+        bx = O.transform(O.KIND_C2R, ax, op, shape=shape, **kw)
+        btz = O.transform(O.KIND_C2R, z, adj, shape=shape, **kw)
         assert abs(_dot(bx, z) - _dot(ax, btz)) <= 1e-13 * abs(_dot(bx, z))
 
 
