@@ -70,7 +70,8 @@ typedef enum pfh_status {
     PFH_ERR_ZERO_EXTENT = 22,             /* PencilError::ZeroGlobalExtent */
     PFH_ERR_INVALID_ARGUMENT = 23,        /* null pointer / unknown enum value (no Rust analogue: types forbid it) */
     PFH_ERR_UNSUPPORTED = 24,             /* a line length the device kernels do not cover */
-    PFH_ERR_DEVICE = 25                   /* a HIP runtime call failed */
+    PFH_ERR_DEVICE = 25,                  /* a HIP runtime call failed */
+    PFH_ERR_INVALID_GRAPH = 26            /* MixedError::InvalidGraph (mixed.rs:118-120) */
 } pfh_status;
 
 /* Payload of the last error on the calling thread (Rust error fields). */
@@ -106,8 +107,21 @@ typedef enum pfh_kind {
      * order. FORWARD = raw C2R (with the endpoint-plane check), BACKWARD = raw
      * R2C, INVERSE = R2C normalized by every selected real extent. Out of
      * place only, like the reference. */
-    PFH_C2R = 10
+    PFH_C2R = 10,
+    /* MixedC2cPlan / MixedR2cPlan (distributed/mixed.rs:86-127, 392-441):
+     * one transform per axis in axis_kind[a] (pfh_axis_transform). MIXED_C2C
+     * works on complex data, an R2R axis transforming the real and imaginary
+     * parts separately; MIXED_R2C has exactly one PFH_AXIS_RFFT axis (the R2C
+     * boundary), R2R or identity on the axes after it and any complex
+     * transform before it (validate_r2c_graph, mixed.rs:1602-1629); other
+     * graphs give PFH_ERR_INVALID_GRAPH. direction[a] applies to FFT axes. */
+    PFH_MIXED_C2C = 11,
+    PFH_MIXED_R2C = 12
 } pfh_kind;
+
+/* Axis transforms of the mixed plans (AxisTransform, mixed.rs:86-101): the
+ * R2R kind codes, plus the complex FFT and the real-to-half-complex axis. */
+enum { PFH_AXIS_NONE = 0, PFH_AXIS_FFT = 10, PFH_AXIS_RFFT = 11 };
 
 /* R2R kind codes = the reference's descriptor codes (r2r.rs:53-64, :101-106). */
 typedef enum pfh_r2r_kind {

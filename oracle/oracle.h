@@ -51,6 +51,15 @@ enum { PFO_DCT1 = 1, PFO_DCT2, PFO_DCT3, PFO_DCT4, PFO_DST1, PFO_DST2, PFO_DST3,
  * FORWARD = raw C2R, BACKWARD = raw R2C, INVERSE = R2C normalized by every
  * selected real extent (c2r.rs:340-360), and their adjoints. */
 enum { PFO_KIND_C2C = 1, PFO_KIND_R2C = 2, PFO_KIND_R2R = 3, PFO_KIND_C2R = 4 };
+/* Heterogeneous plans (distributed/mixed.rs:86-127, 392-441): one transform
+ * per logical axis in r2r[a]: 0 none, PFO_DCT1..PFO_DHT, PFO_AXIS_FFT or
+ * (mixed R2C only, exactly one axis) PFO_AXIS_RFFT. MIXED_C2C works on
+ * complex data (an R2R axis transforms real and imaginary parts
+ * separately); MIXED_R2C has the R2C boundary at the RFFT axis, real R2R or
+ * identity stages on the axes after it and complex stages on the axes
+ * before it (validate_r2c_graph, mixed.rs:1602-1629). */
+enum { PFO_KIND_MIXED_C2C = 5, PFO_KIND_MIXED_R2C = 6 };
+enum { PFO_AXIS_FFT = 10, PFO_AXIS_RFFT = 11 };
 
 enum {
     PFO_OP_FORWARD = 0,

@@ -24,7 +24,8 @@ _lib = None
 
 OK, ERR_ARG, ERR_INVALID_SPECTRUM = 0, 1, 2
 DCT1, DCT2, DCT3, DCT4, DST1, DST2, DST3, DST4, DHT = range(1, 10)
-KIND_C2C, KIND_R2C, KIND_R2R, KIND_C2R = 1, 2, 3, 4
+KIND_C2C, KIND_R2C, KIND_R2R, KIND_C2R, KIND_MIXED_C2C, KIND_MIXED_R2C = 1, 2, 3, 4, 5, 6
+AXIS_FFT, AXIS_RFFT = 10, 11
 FORWARD, INVERSE, BACKWARD, ADJ_FORWARD, ADJ_INVERSE, ADJ_BACKWARD = range(6)
 MAXD = 4
 
@@ -172,10 +173,16 @@ def make_desc(kind, shape, extra=1, selected=None, r2r=None, backward_dir=None, 
     return d
 
 
-def plan_shapes(kind, shape, selected=None):
+def plan_shapes(kind, shape, selected=None, r2r=None):
     """(input logical shape, output logical shape) of the forward operation
-    (for KIND_C2R the forward operation is the C2R: complex in, real out)."""
+    (for KIND_C2R the forward operation is the C2R: complex in, real out;
+    for KIND_MIXED_R2C the reduced axis is the AXIS_RFFT one)."""
     shape = tuple(shape)
+    if kind == KIND_MIXED_R2C:
+        red = list(r2r).index(AXIS_RFFT)
+        half = list(shape)
+        half[red] = shape[red] // 2 + 1
+        return shape, tuple(half)
     if kind not in (KIND_R2C, KIND_C2R):
         return shape, shape
     sel = list(range(len(shape))) if selected is None else list(selected)
@@ -205,13 +212,13 @@ def transform(kind, x: np.ndarray, op=FORWARD, shape=None, extra=1, selected=Non
     desc = make_desc(kind, shape, extra, selected, r2r, backward_dir, permute_dims)
     real = _real_dtype(x.dtype)
     cd = np.complex64 if real == np.float32 else np.complex128
-    shp_in, shp_out = plan_shapes(kind, shape, selected)
+    shp_in, shp_out = plan_shapes(kind, shape, selected, r2r)
     forward_like = op in (FORWARD, ADJ_INVERSE, ADJ_BACKWARD)
-    if kind == KIND_C2C:
+    if kind in (KIND_C2C, KIND_MIXED_C2C):
         count, dt = int(np.prod(shape)) * extra, cd
     elif kind == KIND_R2R:
         count, dt = int(np.prod(shape)) * extra, real
-    elif kind == KIND_R2C:
+    elif kind in (KIND_R2C, KIND_MIXED_R2C):
         if forward_like:
             count, dt = int(np.prod(shp_out)) * extra, cd
         else:

@@ -220,6 +220,7 @@ struct StageSpec {
     int flags = 0;
     double prew0 = 1, prew1 = 1, postw0 = 1, postw1 = 1;
     Layout in_natural, out_natural; // shape/elem on each side with the stage's natural perm
+    bool cplx_r2r = false;          // an R2R / DHT stage on complex data (parts transformed separately)
 };
 
 static int r2r_backward_kind(int k)
@@ -725,7 +726,8 @@ static pfh_status route_specs(const PlanImpl &P, int op_in, std::vector<StageSpe
     // C2R orientation: stage s transforms axis s in the permutation of route
     // stage N-1-s (stages and transitions reversed, r2c.rs:1340-1348)
     const bool c2r = P.kind == PFH_C2R;
-    const bool real_kind = P.kind == PFH_R2C || c2r;
+    const bool mixed_c2c = P.kind == PFH_MIXED_C2C, mixed_r2c = P.kind == PFH_MIXED_R2C;
+    const bool real_kind = P.kind == PFH_R2C || c2r || mixed_r2c;
     const bool scaled = op_in == PFH_INVERSE || op_in == PFH_ADJOINT_INVERSE;
     const int op = route_op(P.kind, op_in);
     std::vector<std::array<int, 8>> route, perm;
@@ -736,7 +738,12 @@ static pfh_status route_specs(const PlanImpl &P, int op_in, std::vector<StageSpe
     for (int a = 0; a < N; ++a) orig[a] = red[a] = (int64_t)d.shape[a];
     unsigned sel = d.selected ? d.selected : ((1u << N) - 1u);
     int boundary = -1, raxis = -1;
-    if (real_kind) {
+    if (mixed_r2c) {
+        for (int a = 0; a < N; ++a)
+            if (d.axis_kind[a] == PFH_AXIS_RFFT) raxis = a;
+        boundary = N - 1 - raxis;
+        red[raxis] = orig[raxis] / 2 + 1;
+    } else if (real_kind) {
         // R2C: the last selected axis; C2R orientation: the first
         for (int i = 0; i < N; ++i) {
             const int a = c2r ? i : N - 1 - i;
@@ -765,7 +772,24 @@ static pfh_status route_specs(const PlanImpl &P, int op_in, std::vector<StageSpe
         } else if (P.kind == PFH_R2R) {
             c_in = c_out = false;
         }
-        if (P.kind == PFH_C2C) {
+        const int code = d.axis_kind[axis];
+        if (mixed_c2c || (mixed_r2c && s != boundary)) {
+            // mixed stage (mixed.rs:3880-3990): identity, complex FFT, or
+            // R2R / DHT on complex data (parts separately) or real data
+            const bool cplx = mixed_c2c || s > boundary;
+            sp.n = (mixed_r2c && s > boundary) ? red[axis] : orig[axis];
+            if (code == PFH_AXIS_NONE) {
+                sp.kind = SK_COPY;
+            } else if (code == PFH_AXIS_FFT) {
+                sp.kind = SK_C2C;
+                const bool neg_forward = d.direction[axis] == 0;
+                sp.sign_pos = fwd ? !neg_forward : neg_forward;
+                if (scaled) sp.scale = 1.0 / (double)sp.n;
+            } else {
+                r2r_stage(sp, code, sp.n, op);
+                sp.cplx_r2r = cplx;
+            }
+        } else if (P.kind == PFH_C2C) {
             sp.n = orig[axis];
             if (!(sel & (1u << axis))) {
                 sp.kind = SK_COPY;
@@ -817,7 +841,7 @@ static pfh_status route_specs(const PlanImpl &P, int op_in, std::vector<StageSpe
             sp.in_natural = make_layout(N, perm[s], sh_out, P.extra, (c_out ? 2 : 1) * tb);
             sp.out_natural = make_layout(N, perm[s], sh_in, P.extra, (c_in ? 2 : 1) * tb);
         }
-        if (sp.kind == SK_COPY) {
+        if (sp.kind == SK_COPY || sp.cplx_r2r) {
             sp.in_c = sp.out_c = fwd ? c_in : c_out;
         }
         by_stage[s] = sp;
@@ -968,6 +992,18 @@ static int64_t slab_target_bytes(const Tuning &tu)
     return (int64_t)mb << 20;
 }
 
+// The layout of complex data as real values: one more logical axis (the
+// real / imaginary part) of extent 2 at the fastest memory position.
+static Layout real_view(const Layout &L)
+{
+    Layout R = L;
+    R.N = L.N + 1;
+    R.perm[L.N] = L.N;
+    R.shape[L.N] = 2;
+    R.elem = L.elem / 2;
+    return R;
+}
+
 static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
 {
     std::vector<StageSpec> specs;
@@ -986,7 +1022,8 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
     const int64_t target = in_place ? 0 : slab_target_bytes(P.tuning);
     std::vector<char> checked(K, 0);
     for (int e = 0; e < K; ++e)
-        checked[e] = ((P.kind == PFH_R2C || P.kind == PFH_C2R) && specs[e].kind == SK_C2R &&
+        checked[e] = ((P.kind == PFH_R2C || P.kind == PFH_C2R || P.kind == PFH_MIXED_R2C) &&
+                      specs[e].kind == SK_C2R &&
                       (route_op(P.kind, op) == PFH_INVERSE || route_op(P.kind, op) == PFH_BACKWARD));
     auto slab_axis_of = [&](int e, const Layout &xin, const Layout &mid) -> int {
         if (target <= 0 || e + 1 >= K || checked[e] || checked[e + 1]) return -1;
@@ -1008,6 +1045,14 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
         }
         return n;
     };
+    // An out-of-place operation with an endpoint check must leave dst
+    // untouched when the check rejects the spectrum (c2r.rs:62-64), so no
+    // stage before the check may use dst as an intermediate, and every stage
+    // from the check on reads the device gate.
+    int e_chk = -1;
+    for (int e = 0; e < K; ++e)
+        if (checked[e] && e_chk < 0) e_chk = e;
+    if (e_chk >= 0 && !in_place) dst_cap = 0;
     if (!assign_route(specs, in_place, dst_cap, pair_score, X, Y, ib, ob, wb))
         return set_err(PFH_ERR_PREPARATION_FAILED, "no buffer assignment");
     out.steps.clear();
@@ -1017,6 +1062,14 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
         if (st) return st;
         s.in = X[e];
         s.out = Y[e];
+        if (specs[e].cplx_r2r) {
+            // complex data seen as real lines: the (re, im) part is one more,
+            // fastest axis of extent 2, so the stage runs 2 real lines per
+            // complex line with the same fused layout change
+            s.in = real_view(X[e]);
+            s.out = real_view(Y[e]);
+            s.in_cplx = s.out_cplx = false;
+        }
         s.in_buf = ib[e];
         s.out_buf = ob[e];
         st = finish_step(s, P.dev, tb, P.tuning);
@@ -1041,6 +1094,42 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
                 }
                 transverse *= (double)d.shape[a];
             }
+            if (P.kind == PFH_MIXED_R2C) {
+                // MixedR2cPlan reverse endpoint contract (mixed.rs:409-427):
+                // over the non-identity complex-suffix axes, depth adds
+                // ceil_log2(E_a) + allowance_a and the raw absolute threshold
+                // scales by factor_a
+                depth = 1.0;
+                transverse = 1.0;
+                for (int a = 0; a < ra; ++a) {
+                    const int c = d.axis_kind[a];
+                    if (c == PFH_AXIS_NONE) continue;
+                    const int64_t n = (int64_t)d.shape[a];
+                    int64_t Ea, fa;
+                    int allow = 2;
+                    if (c == PFH_AXIS_FFT) {
+                        Ea = n;
+                        fa = n;
+                        allow = 0;
+                    } else if (c == PFH_DCT1) {
+                        Ea = fa = 2 * (n - 1);
+                    } else if (c == PFH_DST1) {
+                        Ea = fa = 2 * (n + 1);
+                    } else if (c == PFH_DCT4 || c == PFH_DST4) {
+                        Ea = 8 * n;
+                        fa = 2 * n;
+                    } else if (c == PFH_DHT) {
+                        Ea = fa = n;
+                    } else {
+                        Ea = 4 * n;
+                        fa = 2 * n;
+                    }
+                    int bits = 0;
+                    for (int64_t v = Ea - 1; Ea > 1 && v; v >>= 1) ++bits;
+                    depth += bits + allow;
+                    transverse *= (double)fa;
+                }
+            }
             const double eps = (tb == 4) ? (double)FLT_EPSILON : DBL_EPSILON;
             const double minsub = (tb == 4) ? (double)1.40129846e-45f : 4.9406564584124654e-324;
             EndpointArgs ep{};
@@ -1064,6 +1153,7 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
             s.endpoint_check = true;
             s.ep = ep;
         }
+        if (e_chk >= 0 && e > e_chk) s.gated = true;
         out.steps.push_back(s);
     }
     out.pairs.clear();
@@ -1075,7 +1165,7 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
         Layout T = Y[e];
         T.shape[A] = S;
         f = out.steps[e];
-        f.out = T;
+        f.out = specs[e].cplx_r2r ? real_view(T) : T;
         pfh_status r = finish_step(f, P.dev, tb, P.tuning);
         if (r) return r;
         bool found = false;
@@ -1088,7 +1178,7 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
         f.nlines = f.nlines / nA * S;
         set_tiles(f, P.tuning);
         g2 = out.steps[e + 1];
-        g2.in = T;
+        g2.in = specs[e + 1].cplx_r2r ? real_view(T) : T;
         return finish_step(g2, P.dev, tb, P.tuning);
     };
     for (int e = 0; e + 1 < K;) {
@@ -1247,7 +1337,9 @@ static pfh_status create_impl(const pfh_desc *d, PlanImpl &P)
     case PFH_C2C:
     case PFH_R2C:
     case PFH_R2R:
-    case PFH_C2R: {
+    case PFH_C2R:
+    case PFH_MIXED_C2C:
+    case PFH_MIXED_R2C: {
         const int N = d->ndim;
         const int M = d->topology_dims ? d->topology_dims : N - 1;
         if (N < 2 || N > PFH_MAX_DIMS || M < 1 || M >= N) return set_err(PFH_ERR_INVALID_DIMENSIONS, "N >= 2, 1 <= M < N");
@@ -1263,6 +1355,29 @@ static pfh_status create_impl(const pfh_desc *d, PlanImpl &P)
                 const int k = d->axis_kind[a];
                 if (k < 0 || k > PFH_DHT) return set_err(PFH_ERR_INVALID_ARGUMENT, "axis kind");
                 if (k == PFH_DCT1 && d->shape[a] == 1) return set_err(PFH_ERR_INVALID_LENGTH, "DCT-I needs length >= 2");
+            }
+        }
+        if (P.kind == PFH_MIXED_C2C || P.kind == PFH_MIXED_R2C) {
+            // validate_c2c_graph / validate_r2c_graph (mixed.rs:1592-1629) and
+            // validate_r2c_directions (:1631-1643)
+            int rfft = -1;
+            for (int a = 0; a < N; ++a) {
+                const int k = d->axis_kind[a];
+                if (k < 0 || k > PFH_AXIS_RFFT) return set_err(PFH_ERR_INVALID_ARGUMENT, "axis transform");
+                if (k == PFH_AXIS_RFFT) {
+                    if (P.kind == PFH_MIXED_C2C || rfft >= 0)
+                        return set_err(PFH_ERR_INVALID_GRAPH, "invalid mixed transform axis graph");
+                    rfft = a;
+                }
+                if (k == PFH_DCT1 && d->shape[a] == 1) return set_err(PFH_ERR_INVALID_LENGTH, "DCT-I needs length >= 2");
+                if (d->direction[a] != 0 && k != PFH_AXIS_FFT)
+                    return set_err(PFH_ERR_PREPARATION_FAILED, "a backward direction needs an FFT axis");
+            }
+            if (P.kind == PFH_MIXED_R2C) {
+                if (rfft < 0) return set_err(PFH_ERR_INVALID_GRAPH, "invalid mixed transform axis graph");
+                for (int a = rfft + 1; a < N; ++a)
+                    if (d->axis_kind[a] == PFH_AXIS_FFT)
+                        return set_err(PFH_ERR_INVALID_GRAPH, "invalid mixed transform axis graph");
             }
         }
         for (int op = 0; op < 6; ++op) {
@@ -1730,6 +1845,7 @@ const char *pfh_status_string(pfh_status s)
     case PFH_ERR_INVALID_ARGUMENT: return "invalid argument";
     case PFH_ERR_UNSUPPORTED: return "length not supported by the device kernels";
     case PFH_ERR_DEVICE: return "HIP runtime error";
+    case PFH_ERR_INVALID_GRAPH: return "invalid mixed transform axis graph";
     }
     return "unknown status";
 }
@@ -2351,8 +2467,9 @@ pfh_status pfh_pointwise_scale_axes(pfh_plan plan, int side, void *data, size_t 
 {
     if (!plan || !data || !factors || (side != 0 && side != 1)) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
     const PlanImpl &P = plan->impl;
-    if (!(P.kind == PFH_C2C || P.kind == PFH_R2C || P.kind == PFH_R2R || P.kind == PFH_C2R))
-        return set_err(PFH_ERR_INVALID_ARGUMENT, "needs a C2C, R2C, R2R or C2R plan");
+    if (!(P.kind == PFH_C2C || P.kind == PFH_R2C || P.kind == PFH_R2R || P.kind == PFH_C2R ||
+          P.kind == PFH_MIXED_C2C || P.kind == PFH_MIXED_R2C))
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "needs a distributed transform plan");
     const OpPlan &o = P.ops[PFH_FORWARD];
     const Layout &L = side == 0 ? o.in : o.out;
     if ((int64_t)elems != L.total())

@@ -20,7 +20,8 @@ LIB_PATH = os.environ.get("PFH_LIBRARY") or os.path.join(_HERE, "libpencilfft_hi
 MAX_DIMS = 4
 
 # pfh_kind
-LOCAL_C2C, LOCAL_R2C, LOCAL_R2R, LOCAL_DHT, C2C, R2C, R2R, LOCAL_TRANSPOSE, ALLTOALLV_TRANSPOSE, C2R_PLAN = range(1, 11)
+(LOCAL_C2C, LOCAL_R2C, LOCAL_R2R, LOCAL_DHT, C2C, R2C, R2R, LOCAL_TRANSPOSE, ALLTOALLV_TRANSPOSE, C2R_PLAN, MIXED_C2C,
+ MIXED_R2C) = range(1, 13)
 # pfh_op
 FORWARD, INVERSE, BACKWARD, ADJOINT_FORWARD, ADJOINT_INVERSE, ADJOINT_BACKWARD = range(6)
 EXEC_DEFER_STATUS = 1
@@ -40,6 +41,20 @@ class R2rKind:
         return R2rKind._BACKWARD[kind]
 
 
+class AxisTransform:
+    """AxisTransform (crates/pencil-fft/src/distributed/mixed.rs:86-101) of
+    the mixed plans: None (identity), FFT, RFFT (the one real-to-half-complex
+    axis of a MixedR2cPlan), or an R2rKind code."""
+
+    NONE = 0
+    FFT = 10
+    RFFT = 11
+
+    @staticmethod
+    def r2r(kind: int) -> int:
+        return int(kind)
+
+
 STATUS_NAMES = {
     0: "Ok", 1: "InvalidLength", 2: "LengthOverflow", 3: "NonIntegralBatch", 4: "BufferLengthMismatch",
     5: "BatchCountMismatch", 6: "ScratchTooSmall", 7: "InvalidSpectrumEndpoint", 8: "InvalidDimensions",
@@ -47,7 +62,7 @@ STATUS_NAMES = {
     13: "WorkspaceMismatch", 14: "WorkspaceTooSmall", 15: "PreparationFailed", 16: "AllocationFailed",
     17: "InvalidSpectrum", 18: "IncompatibleDistribution", 19: "SourceLayoutMismatch",
     20: "DestinationLayoutMismatch", 21: "InvalidPermutation", 22: "ZeroGlobalExtent", 23: "InvalidArgument",
-    24: "Unsupported", 25: "Device",
+    24: "Unsupported", 25: "Device", 26: "InvalidGraph",
 }
 
 
@@ -550,6 +565,40 @@ class C2rPlan(_RoutePlan):
 
     def execute_in_place(self, op, data, workspace=None, flags=0):
         raise ValueError("C2rPlan is out of place only (distributed/c2r.rs:69-70)")
+
+
+class MixedC2cPlan(_RoutePlan):
+    """MixedC2cPlan (crates/pencil-fft/src/distributed/mixed.rs:392-439,
+    1700-1800) on a one-rank process grid: one AxisTransform per axis on
+    complex data; an R2R axis transforms the real and imaginary parts
+    separately. Output memory order: the reversed axes (permute_dims)."""
+
+    _kind = MIXED_C2C
+
+    def __init__(self, shape: Sequence[int], transforms: Sequence[int], dtype=torch.complex64, extra_shape=(),
+                 directions=None, permute_dims=True, topology_dims=0):
+        d = _route_desc(self._kind, shape, dtype, extra_shape, None, permute_dims, topology_dims)
+        for a, t in enumerate(transforms):
+            d.axis_kind[a] = int(t or 0)
+        if directions is not None:
+            for a, v in enumerate(directions):
+                d.direction[a] = int(v)
+        super().__init__(d)
+        self.shape = tuple(shape)
+        self.transforms = tuple(transforms)
+
+
+class MixedR2cPlan(MixedC2cPlan):
+    """MixedR2cPlan (crates/pencil-fft/src/distributed/mixed.rs:441-446,
+    2740+): exactly one AxisTransform.RFFT axis (the R2C boundary), real R2R
+    or identity stages on the axes after it, complex stages before it; the
+    inverse checks the boundary planes (InvalidSpectrum)."""
+
+    _kind = MIXED_R2C
+
+    def __init__(self, shape: Sequence[int], transforms: Sequence[int], dtype=torch.float32, extra_shape=(),
+                 directions=None, permute_dims=True, topology_dims=0):
+        super().__init__(shape, transforms, dtype, extra_shape, directions, permute_dims, topology_dims)
 
 
 class R2rPlan(_RoutePlan):

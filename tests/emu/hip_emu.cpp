@@ -793,6 +793,37 @@ void route_cases(int prec)
             run_case<T>(name, p, nd, PFH_ADJOINT_FORWARD, true, false, 0, (int64_t)c.shape[0]);
             pfh_plan_destroy(p);
         }
+        // mixed plans: complex R2R stages run as real lines with a (re, im) digit
+        {
+            pfh_desc d = route_desc(PFH_MIXED_C2C, prec, c.shape, c.extra);
+            const int codes[4] = {PFH_AXIS_FFT, PFH_DCT2, PFH_DHT, PFH_DST1};
+            for (int a = 0; a < nd; ++a) d.axis_kind[a] = codes[a % 4];
+            pfh_plan p = nullptr;
+            if (pfh_plan_create(&d, &p) != PFH_OK) {
+                fail("MixedC2C %s %s: plan", pn, sh.c_str());
+                continue;
+            }
+            std::snprintf(name, sizeof(name), "MixedC2C %s %s", pn, sh.c_str());
+            for (int op : {PFH_FORWARD, PFH_INVERSE, PFH_ADJOINT_FORWARD})
+                run_case<T>(name, p, nd, op, false, false, -1, 0);
+            pfh_plan_destroy(p);
+        }
+        {
+            pfh_desc d = route_desc(PFH_MIXED_R2C, prec, c.shape, c.extra);
+            // FFT / R2R before the RFFT axis (axis 1), real R2R after it
+            const int codes[4] = {PFH_AXIS_FFT, PFH_AXIS_RFFT, PFH_DCT4, PFH_DHT};
+            for (int a = 0; a < nd; ++a) d.axis_kind[a] = codes[a];
+            pfh_plan p = nullptr;
+            if (pfh_plan_create(&d, &p) != PFH_OK) {
+                fail("MixedR2C %s %s: plan", pn, sh.c_str());
+                continue;
+            }
+            std::snprintf(name, sizeof(name), "MixedR2C %s %s", pn, sh.c_str());
+            run_case<T>(name, p, nd, PFH_FORWARD, true, false, 1, (int64_t)c.shape[1]);
+            run_case<T>(name, p, nd, PFH_INVERSE, false, true, 1, (int64_t)c.shape[1]);
+            run_case<T>(name, p, nd, PFH_ADJOINT_INVERSE, true, false, 1, (int64_t)c.shape[1]);
+            pfh_plan_destroy(p);
+        }
         // R2R (DCT-II, DST-I, DCT-IV, DHT per axis)
         {
             pfh_desc d = route_desc(PFH_R2R, prec, c.shape, c.extra);

@@ -133,6 +133,42 @@ def main():
         g[f"route_c2r{i}_in"] = X
         g[f"route_c2r{i}_fwd"] = out_order(x * float(np.prod([shape[a] for a in sel_l])))
         g[f"route_c2r{i}_meta"] = np.array(list(shape) + [extra] + [-1] + sel_l)
+    # heterogeneous plans (distributed/mixed.rs): per-axis codes 0 none,
+    # 1..8 R2R, 9 DHT, 10 FFT, 11 RFFT; an R2R axis of complex data transforms
+    # the real and imaginary parts separately
+    def mixed_axis(y, code, axis):
+        if code == 0:
+            return y
+        if code == 10:
+            return np.fft.fft(y, axis=axis)
+        if np.iscomplexobj(y):
+            return r2r_axis(y.real, code, axis) + 1j * r2r_axis(y.imag, code, axis)
+        return r2r_axis(y, code, axis)
+
+    mixed_c2c = [((4, 5, 6), 1, [10, 2, 10]), ((5, 6, 4), 2, [9, 10, 4]), ((3, 4, 5), 1, [0, 10, 5]),
+                 ((6, 4, 5), 1, [1, 9, 10]), ((4, 3, 2, 5), 1, [10, 6, 0, 8])]
+    for i, (shape, extra, codes) in enumerate(mixed_c2c):
+        x = field(shape, extra, 5.0, True)
+        y = x.copy()
+        for a, c in enumerate(codes):
+            y = mixed_axis(y, c, a + 1)
+        g[f"route_mixc2c{i}_in"] = x
+        g[f"route_mixc2c{i}_fwd"] = out_order(y)
+        g[f"route_mixc2c{i}_meta"] = np.array(list(shape) + [extra] + [-1] + list(codes))
+    mixed_r2c = [((4, 6, 5), 1, [10, 11, 2]), ((5, 4, 6), 2, [9, 11, 0]), ((6, 5, 8), 1, [2, 10, 11]),
+                 ((4, 6, 5, 3), 1, [10, 6, 11, 4])]
+    for i, (shape, extra, codes) in enumerate(mixed_r2c):
+        x = field(shape, extra, 6.0, False)
+        ra = codes.index(11)
+        y = x.copy()
+        for a in range(len(shape) - 1, ra, -1):  # real prefix, route order
+            y = mixed_axis(y, codes[a], a + 1)
+        y = np.fft.rfft(y, axis=ra + 1)
+        for a in range(ra - 1, -1, -1):
+            y = mixed_axis(y, codes[a], a + 1)
+        g[f"route_mixr2c{i}_in"] = x
+        g[f"route_mixr2c{i}_fwd"] = out_order(y)
+        g[f"route_mixr2c{i}_meta"] = np.array(list(shape) + [extra] + [-1] + list(codes))
     r2r = [((3, 4), 1, [9, 9]), ((3, 2, 4), 6, [9, 0, 9]), ((2, 3, 2, 3), 1, [9, 0, 0, 9]),
            ((2, 3, 2, 3), 1, [1, 2, 3, 4]), ((2, 3, 2, 3), 1, [5, 6, 7, 8]), ((6, 5, 4), 2, [2, 5, 4])]
     for i, (shape, extra, kinds) in enumerate(r2r):

@@ -211,6 +211,103 @@ def test_c2r_plan_forward_rejects_invalid_spectrum_without_writing(gpu):
         plan.execute_in_place(pf.FORWARD, _tdev(spec))
 
 
+F, R = pf.AxisTransform.FFT, pf.AxisTransform.RFFT
+MIXED_C2C_CASES = [
+    ((4, 5, 6), 1, [F, 2, F]),
+    ((5, 6, 4), 2, [9, F, 4]),
+    ((3, 4, 5), 1, [0, F, 5]),
+    ((6, 4, 5), 1, [1, 9, F]),
+    ((4, 3, 2, 5), 1, [F, 6, 0, 8]),
+    ((16, 12, 20), 3, [F, 3, 7]),
+]
+MIXED_R2C_CASES = [
+    ((4, 6, 5), 1, [F, R, 2]),
+    ((5, 4, 6), 2, [9, R, 0]),
+    ((6, 5, 8), 1, [2, F, R]),
+    ((4, 6, 5, 3), 1, [F, 6, R, 4]),
+    ((12, 16, 10), 1, [3, R, 9]),
+]
+
+
+@pytest.mark.parametrize("permute", [True, False])
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+@pytest.mark.parametrize("shape,extra,codes", MIXED_C2C_CASES)
+def test_mixed_c2c_plan_matches_oracle(gpu, shape, extra, codes, dt, permute):
+    """MixedC2cPlan (distributed/mixed.rs:392-439): FFT / R2R / identity per
+    axis on complex data, all six operations against the oracle."""
+    x = fixture_field(shape, extra, 5.0, True).astype(dt).reshape(-1)
+    plan = pf.MixedC2cPlan(shape, codes, dtype=torch.from_numpy(x).dtype, extra_shape=(extra,) if extra > 1 else (),
+                           permute_dims=permute)
+    y = O.transform(O.KIND_MIXED_C2C, x.astype(np.complex128), O.FORWARD, shape=shape, extra=extra, r2r=codes,
+                    permute_dims=permute).astype(dt)
+    run_all_ops(plan, O.KIND_MIXED_C2C, shape, extra, x, y, dt, r2r=codes, permute_dims=permute)
+
+
+@pytest.mark.parametrize("permute", [True, False])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("shape,extra,codes", MIXED_R2C_CASES)
+def test_mixed_r2c_plan_matches_oracle(gpu, shape, extra, codes, dt, permute):
+    """MixedR2cPlan (distributed/mixed.rs:441-446): real R2R prefix, the RFFT
+    boundary, complex FFT / R2R suffix; all six operations against the oracle
+    (the inverse with the mixed endpoint thresholds)."""
+    x = fixture_field(shape, extra, 6.0, False).astype(dt).reshape(-1)
+    plan = pf.MixedR2cPlan(shape, codes, dtype=torch.from_numpy(x).dtype, extra_shape=(extra,) if extra > 1 else (),
+                           permute_dims=permute)
+    y = O.transform(O.KIND_MIXED_R2C, x.astype(np.float64), O.FORWARD, shape=shape, extra=extra, r2r=codes,
+                    permute_dims=permute).astype(complex_of(dt))
+    run_all_ops(plan, O.KIND_MIXED_R2C, shape, extra, x, y, dt, r2r=codes, permute_dims=permute)
+
+
+def test_mixed_plans_reject_invalid_graphs(gpu):
+    for codes in ([F, 2, F], [R, R, 0], [0, R, F]):
+        with pytest.raises(pf.PfhError) as e:
+            pf.MixedR2cPlan((4, 5, 6), codes)
+        assert e.value.variant == "InvalidGraph"
+    with pytest.raises(pf.PfhError) as e:
+        pf.MixedC2cPlan((4, 5, 6), [R, 0, 0])
+    assert e.value.variant == "InvalidGraph"
+    with pytest.raises(pf.PfhError) as e:
+        pf.MixedC2cPlan((4, 5, 6), [2, F, 0], directions=[1, 0, 0])
+    assert e.value.variant == "PreparationFailed"
+
+
+def test_mixed_r2c_inverse_rejects_invalid_spectrum(gpu):
+    shape, codes = (6, 4, 8), [F, R, 2]
+    rng = np.random.default_rng(6)
+    plan = pf.MixedR2cPlan(shape, codes, dtype=torch.float64)
+    x = uniform_real(rng, shape, np.float64).reshape(-1)
+    spec = _host(plan.forward(_tdev(x))).copy()
+    # output memory order [axis2][axis1][axis0]: element 1 is axis1 = 0 (the DC plane), axis0 = 1
+    spec[1] += 0.5j
+    dst = torch.full((x.size,), -1.0, dtype=torch.float64, device="cuda")
+    with pytest.raises(pf.PfhError) as e:
+        plan.inverse(_tdev(spec), dst)
+    assert e.value.variant == "InvalidSpectrum"
+    assert torch.all(dst == -1.0)
+    with pytest.raises(O.InvalidSpectrum):
+        O.transform(O.KIND_MIXED_R2C, spec, O.INVERSE, shape=shape, r2r=codes)
+
+
+def test_r2c_partial_selection_inverse_rejects_without_writing(gpu):
+    """With selection [0, 1] the C2R stage is followed by a real copy stage
+    (axis 2); a rejected spectrum must leave dst untouched all the same."""
+    shape = (6, 8, 5)
+    rng = np.random.default_rng(8)
+    plan = pf.R2cPlan(shape, dtype=torch.float64, selection=[0, 1])
+    x = uniform_real(rng, shape, np.float64).reshape(-1)
+    spec = _host(plan.forward(_tdev(x))).copy()
+    _, out_shape, _, _ = plan.layout(1)
+    # memory [axis2][axis1][axis0]; element 1: axis1 = 0 (the DC plane of the reduced axis 1)
+    spec[1] += 0.5j
+    dst = torch.full((x.size,), -1.0, dtype=torch.float64, device="cuda")
+    with pytest.raises(pf.PfhError) as e:
+        plan.inverse(_tdev(spec), dst)
+    assert e.value.variant == "InvalidSpectrum"
+    assert torch.all(dst == -1.0)
+    with pytest.raises(O.InvalidSpectrum):
+        O.transform(O.KIND_R2C, spec, O.INVERSE, shape=shape, selected=[0, 1])
+
+
 R2R_CASES = [
     ((3, 4), 1, [pf.R2rKind.Dht, pf.R2rKind.Dht]),
     ((3, 2, 4), 6, [pf.R2rKind.Dht, None, pf.R2rKind.Dht]),

@@ -193,12 +193,13 @@ def _meta(m):
 
 def test_routes_match_numpy_goldens(gold):
     for kind, prefix in ((O.KIND_C2C, "route_c2c"), (O.KIND_R2C, "route_r2c"), (O.KIND_R2R, "route_r2r"),
-                         (O.KIND_C2R, "route_c2r")):
+                         (O.KIND_C2R, "route_c2r"), (O.KIND_MIXED_C2C, "route_mixc2c"),
+                         (O.KIND_MIXED_R2C, "route_mixr2c")):
         i = 0
         while f"{prefix}{i}_in" in gold.files:
             shape, extra, tail = _meta(gold[f"{prefix}{i}_meta"])
             x = gold[f"{prefix}{i}_in"]
-            kw = {"r2r": tail} if kind == O.KIND_R2R else {"selected": tail}
+            kw = {"r2r": tail} if kind in (O.KIND_R2R, O.KIND_MIXED_C2C, O.KIND_MIXED_R2C) else {"selected": tail}
             y = O.transform(kind, x, O.FORWARD, shape=shape, extra=extra, **kw)
             assert rel(y, gold[f"{prefix}{i}_fwd"]) < 1e-13, (prefix, i)
             back = O.transform(kind, y, O.INVERSE, shape=shape, extra=extra, **kw)
@@ -250,6 +251,40 @@ def test_oracle_adjoints_satisfy_dot_identity(kind, shape, kw):
         bx = O.transform(kind, ax, op, shape=shape, **kw)
         btz = O.transform(kind, z, adj, shape=shape, **kw)
         assert abs(_dot(bx, z) - _dot(ax, btz)) <= 1e-13 * abs(_dot(bx, z))
+
+
+@pytest.mark.parametrize("kind,shape,codes", [
+    (O.KIND_MIXED_C2C, (4, 5, 6), [10, 2, 10]),
+    (O.KIND_MIXED_C2C, (6, 4, 5), [1, 9, 10]),
+    (O.KIND_MIXED_C2C, (5, 6, 7), [3, 7, 5]),
+    (O.KIND_MIXED_R2C, (4, 6, 5), [10, 11, 2]),
+    (O.KIND_MIXED_R2C, (6, 5, 8), [6, 10, 11]),
+    (O.KIND_MIXED_R2C, (5, 7, 6), [4, 11, 1]),
+])
+def test_oracle_mixed_adjoints_satisfy_dot_identity(kind, shape, codes):
+    rng = np.random.default_rng(sum(shape) + codes[0])
+    cplx = kind == O.KIND_MIXED_C2C
+    x = rng.standard_normal(int(np.prod(shape))) + (1j * rng.standard_normal(int(np.prod(shape))) if cplx else 0)
+    ax = O.transform(kind, x, O.FORWARD, shape=shape, r2r=codes)
+    y = rng.standard_normal(ax.shape) + 1j * rng.standard_normal(ax.shape)
+    aty = O.transform(kind, y, O.ADJ_FORWARD, shape=shape, r2r=codes)
+    assert abs(_dot(ax, y) - _dot(x, aty)) <= 1e-13 * abs(_dot(ax, y))
+    z = rng.standard_normal(x.shape) + (1j * rng.standard_normal(x.shape) if cplx else 0)
+    for op, adj in ((O.INVERSE, O.ADJ_INVERSE), (O.BACKWARD, O.ADJ_BACKWARD)):
+        bx = O.transform(kind, ax, op, shape=shape, r2r=codes)
+        btz = O.transform(kind, z, adj, shape=shape, r2r=codes)
+        assert abs(_dot(bx, z) - _dot(ax, btz)) <= 1e-13 * abs(_dot(bx, z))
+
+
+def test_oracle_mixed_graph_rules():
+    """validate_r2c_graph (mixed.rs:1602-1629): one RFFT axis, no FFT after it;
+    no RFFT in a mixed C2C plan."""
+    x = np.zeros(4 * 5 * 6)
+    for codes in ([10, 2, 10], [11, 11, 0], [0, 11, 10]):
+        with pytest.raises(Exception):
+            O.transform(O.KIND_MIXED_R2C, x, O.FORWARD, shape=(4, 5, 6), r2r=codes)
+    with pytest.raises(Exception):
+        O.transform(O.KIND_MIXED_C2C, x.astype(complex), O.FORWARD, shape=(4, 5, 6), r2r=[11, 0, 0])
 
 
 @pytest.mark.parametrize("shape,kw", [((6, 5, 4), {}), ((5, 6, 7), {}), ((4, 6, 5), {"selected": [1, 2]})])
