@@ -71,7 +71,13 @@ typedef enum pfh_status {
     PFH_ERR_INVALID_ARGUMENT = 23,        /* null pointer / unknown enum value (no Rust analogue: types forbid it) */
     PFH_ERR_UNSUPPORTED = 24,             /* a line length the device kernels do not cover */
     PFH_ERR_DEVICE = 25,                  /* a HIP runtime call failed */
-    PFH_ERR_INVALID_GRAPH = 26            /* MixedError::InvalidGraph (mixed.rs:118-120) */
+    PFH_ERR_INVALID_GRAPH = 26,           /* MixedError::InvalidGraph (mixed.rs:118-120) */
+    /* pointwise validation (pointwise.rs:8-33, :216-230); batch = the failing
+     * input's index (MultiInputError::Input { index, .. }) */
+    PFH_ERR_INCOMPATIBLE_PENCILS = 27,    /* PointwiseError::IncompatiblePencils */
+    PFH_ERR_EXTRA_RANK_MISMATCH = 28,     /* PointwiseError::ExtraRankMismatch {expected = required, actual} */
+    PFH_ERR_EXTRA_EXTENT_MISMATCH = 29,   /* PointwiseError::ExtraExtentMismatch {axis = index, output = required, input = actual} */
+    PFH_ERR_EMPTY_INPUTS = 30             /* MultiInputError::EmptyInputs */
 } pfh_status;
 
 /* Payload of the last error on the calling thread (Rust error fields). */
@@ -321,6 +327,116 @@ pfh_status pfh_pointwise_quadratic_adjoint(int precision, const void *w, const v
  * memory of at least 8192 bytes; result: device pointer to one double. */
 pfh_status pfh_sum_squares(int precision, const void *x, size_t n, void *workspace, size_t workspace_bytes,
                            double *result, pfh_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Reductions over a device array (the local fold of the reference's       */
+/* checked global reductions at one rank, collectives.rs:519-620).         */
+/* ---------------------------------------------------------------------- */
+
+/* Scalar type codes: the reference's on-file IoElement codes
+ * (crates/pencil-io/src/format.rs:50-60, :95-126). */
+typedef enum pfh_scalar_type {
+    PFH_SCALAR_I8 = 1,
+    PFH_SCALAR_U8 = 2,
+    PFH_SCALAR_I16 = 3,
+    PFH_SCALAR_U16 = 4,
+    PFH_SCALAR_I32 = 5,
+    PFH_SCALAR_U32 = 6,
+    PFH_SCALAR_I64 = 7,
+    PFH_SCALAR_U64 = 8,
+    PFH_SCALAR_F32 = 9,
+    PFH_SCALAR_F64 = 10,
+    PFH_SCALAR_C64 = 11,
+    PFH_SCALAR_C128 = 12
+} pfh_scalar_type;
+
+typedef enum pfh_reduce_op {
+    PFH_REDUCE_SUM = 1,     /* global_sum  (collectives.rs:519-528) */
+    PFH_REDUCE_MIN = 2,     /* global_min  (:530-539); real types only */
+    PFH_REDUCE_MAX = 3,     /* global_max  (:541-550); real types only */
+    PFH_REDUCE_L2_NORM = 4, /* l2_norm     (:552-565) */
+    PFH_REDUCE_ANY = 5,     /* any         (:567-575) */
+    PFH_REDUCE_ALL = 6      /* all         (:577-588) */
+} pfh_reduce_op;
+
+/* outcome of a reduction */
+enum {
+    PFH_REDUCE_VALUE = 0,           /* value holds the result */
+    PFH_REDUCE_NONE = 1,            /* min / max of an empty array (Ok(None)) */
+    PFH_REDUCE_INTEGER_OVERFLOW = 2 /* CollectiveError::IntegerOverflow */
+};
+
+/* Device-resident result. value: SUM / MIN / MAX in the element type
+ * (complex: re, im); L2_NORM in the norm type (f32 for f32 and complex f32,
+ * f64 otherwise); ANY / ALL in value[0] (0 or 1). */
+typedef struct pfh_reduce_result {
+    uint8_t value[16];
+    int32_t outcome;
+    int32_t reserved[3];
+} pfh_reduce_result;
+
+/* Workspace a reduction needs (a fixed grid of partial states). */
+pfh_status pfh_reduce_workspace_bytes(size_t *bytes);
+
+/* Reduces the n elements of x (any memory order: the reductions are
+ * permutation invariant, and integer sums follow the storage order, as the
+ * reference's local fold does). Rules, as the reference's:
+ *  - integer SUM: every step of the storage-order fold is checked; an
+ *    intermediate overflow gives PFH_REDUCE_INTEGER_OVERFLOW even when the
+ *    total would fit (collectives.rs:14-17, :206-214);
+ *  - floating SUM: NaN wins; +inf and -inf together give NaN; a lone infinity
+ *    sign is kept; complex parts independently (:300-330, :380-420). Finite
+ *    values are accumulated in double-double, then rounded once;
+ *  - MIN / MAX: NaN anywhere gives NaN; an empty array gives PFH_REDUCE_NONE
+ *    (mapped_extreme_layout, :2066-2094);
+ *  - L2_NORM: scaled sum of squares (NormState, :2150-2208); NaN gives NaN,
+ *    an infinity +inf, an empty or all-zero array 0;
+ *  - ANY / ALL: x != 0 (complex: either part); any of nothing is false, all
+ *    of nothing true.
+ * The result is written to `result` (device memory) in stream order. */
+pfh_status pfh_reduce(int op, int scalar_type, const void *x, size_t n, void *workspace, size_t workspace_bytes,
+                      pfh_reduce_result *result, pfh_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Broadcast pointwise combinations (pointwise2 / pointwise3 /             */
+/* pointwise_many, pointwise.rs:40-598) with fixed operations.             */
+/* ---------------------------------------------------------------------- */
+
+/* Spatial layout of a pencil at one rank (Pencil::same_layout compares
+ * these, pencil.rs:238-240). */
+typedef struct pfh_pencil_layout {
+    int ndim;                        /* spatial rank N (1..4) */
+    size_t shape[PFH_MAX_DIMS];      /* global logical shape */
+    int perm[PFH_MAX_DIMS];          /* memory order (AxisPermutation) */
+    int topology_dims;               /* M */
+    int decomposition[PFH_MAX_DIMS]; /* ordered decomposed axes (first M used) */
+} pfh_pencil_layout;
+
+/* A device pencil array: [extra...][memory-ordered spatial] elements. */
+typedef struct pfh_array {
+    void *data;
+    int scalar_type; /* PFH_SCALAR_F32 / F64 / C64 / C128 */
+    int extra_ndim;  /* 0..4 */
+    size_t extra_shape[PFH_MAX_DIMS];
+    pfh_pencil_layout pencil;
+} pfh_array;
+
+typedef enum pfh_pointwise_op {
+    PFH_POINTWISE_LINEAR = 1,  /* out = sum_i c_i x_i        (pointwise_many, n >= 1, at most 8) */
+    PFH_POINTWISE_MUL = 2,     /* out = c_0 x_0 x_1          (pointwise2) */
+    PFH_POINTWISE_MUL_ADD = 3  /* out = c_0 x_0 x_1 + c_1 x_2 (pointwise3) */
+} pfh_pointwise_op;
+
+/* Every input has the output's spatial layout and extra rank; an input's
+ * extra extent is the output's or 1 (broadcast). Validation follows the
+ * reference's order (pointwise2: layouts of both inputs first, then extents;
+ * pointwise3 / many: input by input) and completes before any write.
+ * coeffs: 2 doubles (re, im) per coefficient. The output is complex when any
+ * input or coefficient is; all arrays share one precision. An input may be
+ * the output itself (same pointer and extra shape: the in-place forms);
+ * other overlaps are not allowed. */
+pfh_status pfh_pointwise(int op, const pfh_array *inputs, int n_inputs, const pfh_array *out, const double *coeffs,
+                         pfh_stream_t stream);
 
 #ifdef __cplusplus
 }

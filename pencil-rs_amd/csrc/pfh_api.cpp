@@ -20,6 +20,7 @@
 #include <new>
 #include <vector>
 
+#include "pfh_collectives.hpp"
 #include "pfh_plan.hpp"
 
 using namespace pfh;
@@ -1846,6 +1847,10 @@ const char *pfh_status_string(pfh_status s)
     case PFH_ERR_UNSUPPORTED: return "length not supported by the device kernels";
     case PFH_ERR_DEVICE: return "HIP runtime error";
     case PFH_ERR_INVALID_GRAPH: return "invalid mixed transform axis graph";
+    case PFH_ERR_INCOMPATIBLE_PENCILS: return "pointwise arrays must have identical spatial layouts";
+    case PFH_ERR_EXTRA_RANK_MISMATCH: return "pointwise extra rank mismatch";
+    case PFH_ERR_EXTRA_EXTENT_MISMATCH: return "pointwise extra extent mismatch";
+    case PFH_ERR_EMPTY_INPUTS: return "pointwise many operations require at least one input";
     }
     return "unknown status";
 }
@@ -2528,6 +2533,186 @@ pfh_status pfh_sum_squares(int precision, const void *x, size_t n, void *workspa
     double *partial = static_cast<double *>(workspace);
     PFH_HIP_CHECK(precision == PFH_F32 ? launch_sum_squares<float>(x, (int64_t)n, partial, (int)parts, result, s)
                                        : launch_sum_squares<double>(x, (int64_t)n, partial, (int)parts, result, s));
+    return ok();
+}
+
+// ---------------------------------------------------------------------------
+// Reductions and broadcast pointwise combinations
+// ---------------------------------------------------------------------------
+
+static_assert(sizeof(pfh_reduce_result) == sizeof(ReduceResult), "pfh_reduce_result layout");
+static_assert((int)PFH_SCALAR_C128 == (int)kScalarC128 && (int)PFH_REDUCE_ALL == (int)kReduceAll &&
+                  (int)PFH_POINTWISE_MUL_ADD == (int)kPointwiseMulAdd,
+              "enum values shared with the kernels");
+
+pfh_status pfh_reduce_workspace_bytes(size_t *bytes)
+{
+    if (!bytes) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    *bytes = (size_t)kReduceWorkspaceBytes;
+    return ok();
+}
+
+pfh_status pfh_reduce(int op, int scalar_type, const void *x, size_t n, void *workspace, size_t workspace_bytes,
+                      pfh_reduce_result *result, pfh_stream_t stream)
+{
+    if (op < PFH_REDUCE_SUM || op > PFH_REDUCE_ALL) return set_err(PFH_ERR_INVALID_ARGUMENT, "reduction");
+    if (scalar_type < PFH_SCALAR_I8 || scalar_type > PFH_SCALAR_C128)
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "scalar type");
+    const bool complex_type = scalar_type == PFH_SCALAR_C64 || scalar_type == PFH_SCALAR_C128;
+    if (complex_type && (op == PFH_REDUCE_MIN || op == PFH_REDUCE_MAX))
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "complex values have no order");
+    if (!result || (n > 0 && !x)) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    if (n > (size_t)INT64_MAX / 16) return set_err(PFH_ERR_LENGTH_OVERFLOW, "element count");
+    if (!workspace || workspace_bytes < (size_t)kReduceWorkspaceBytes)
+        return set_err(PFH_ERR_WORKSPACE_TOO_SMALL, "reduction workspace", (size_t)kReduceWorkspaceBytes,
+                       workspace ? workspace_bytes : 0);
+    PFH_HIP_CHECK(launch_reduce(op, scalar_type, x, (int64_t)n, workspace, reinterpret_cast<ReduceResult *>(result),
+                                reinterpret_cast<hipStream_t>(stream)));
+    return ok();
+}
+
+static bool layout_valid(const pfh_pencil_layout &p)
+{
+    if (p.ndim < 1 || p.ndim > PFH_MAX_DIMS || p.topology_dims < 0 || p.topology_dims > p.ndim) return false;
+    bool seen[PFH_MAX_DIMS] = {false, false, false, false};
+    for (int a = 0; a < p.ndim; ++a) {
+        if (p.perm[a] < 0 || p.perm[a] >= p.ndim || seen[p.perm[a]]) return false;
+        seen[p.perm[a]] = true;
+    }
+    return true;
+}
+
+// Pencil::same_layout (pencil.rs:238-240) at one rank: same global shape,
+// decomposition, topology rank and permutation.
+static bool same_layout(const pfh_pencil_layout &a, const pfh_pencil_layout &b)
+{
+    if (a.ndim != b.ndim || a.topology_dims != b.topology_dims) return false;
+    for (int d = 0; d < a.ndim; ++d)
+        if (a.shape[d] != b.shape[d] || a.perm[d] != b.perm[d]) return false;
+    for (int d = 0; d < a.topology_dims; ++d)
+        if (a.decomposition[d] != b.decomposition[d]) return false;
+    return true;
+}
+
+// validate_extents (pointwise.rs:178-197)
+static pfh_status check_extents(const pfh_array &in, const pfh_array &out, size_t index)
+{
+    if (in.extra_ndim != out.extra_ndim)
+        return set_err(PFH_ERR_EXTRA_RANK_MISMATCH, "pointwise extra rank mismatch", (size_t)out.extra_ndim,
+                       (size_t)in.extra_ndim, index);
+    for (int d = 0; d < in.extra_ndim; ++d)
+        if (in.extra_shape[d] != out.extra_shape[d] && in.extra_shape[d] != 1)
+            return set_err(PFH_ERR_EXTRA_EXTENT_MISMATCH, "pointwise extra extent mismatch", out.extra_shape[d],
+                           in.extra_shape[d], index, (size_t)d);
+    return PFH_OK;
+}
+
+static bool float_type(int t)
+{
+    return t == PFH_SCALAR_F32 || t == PFH_SCALAR_F64 || t == PFH_SCALAR_C64 || t == PFH_SCALAR_C128;
+}
+static bool f64_type(int t) { return t == PFH_SCALAR_F64 || t == PFH_SCALAR_C128; }
+static bool complex_type(int t) { return t == PFH_SCALAR_C64 || t == PFH_SCALAR_C128; }
+
+pfh_status pfh_pointwise(int op, const pfh_array *inputs, int n_inputs, const pfh_array *out, const double *coeffs,
+                         pfh_stream_t stream)
+{
+    if (op < PFH_POINTWISE_LINEAR || op > PFH_POINTWISE_MUL_ADD || !out || n_inputs < 0 ||
+        (n_inputs > 0 && !inputs))
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    const int want = op == PFH_POINTWISE_MUL ? 2 : op == PFH_POINTWISE_MUL_ADD ? 3 : -1;
+    if (op == PFH_POINTWISE_LINEAR) {
+        if (n_inputs == 0) return set_err(PFH_ERR_EMPTY_INPUTS, "pointwise many operations require at least one input");
+        if (n_inputs > kPointwiseMaxInputs) return set_err(PFH_ERR_UNSUPPORTED, "at most 8 inputs");
+    } else if (n_inputs != want) {
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "input count of the operation");
+    }
+    // layouts and extra ranks must be well formed before they are compared
+    if (!layout_valid(out->pencil)) return set_err(PFH_ERR_INVALID_PERMUTATION, "output pencil");
+    if (out->extra_ndim < 0 || out->extra_ndim > PFH_MAX_DIMS) return set_err(PFH_ERR_INVALID_ARGUMENT, "extra rank");
+    for (int i = 0; i < n_inputs; ++i) {
+        if (!layout_valid(inputs[i].pencil)) return set_err(PFH_ERR_INVALID_PERMUTATION, "input pencil", 0, 0, (size_t)i);
+        if (inputs[i].extra_ndim < 0 || inputs[i].extra_ndim > PFH_MAX_DIMS)
+            return set_err(PFH_ERR_INVALID_ARGUMENT, "extra rank", 0, 0, (size_t)i);
+    }
+    if (op == PFH_POINTWISE_MUL) {
+        // validate_two_inputs (pointwise.rs:155-176): both layouts, then extents
+        if (!same_layout(inputs[0].pencil, inputs[1].pencil) || !same_layout(inputs[0].pencil, out->pencil))
+            return set_err(PFH_ERR_INCOMPATIBLE_PENCILS, "pointwise arrays must have identical spatial layouts");
+        for (int i = 0; i < 2; ++i) {
+            pfh_status st = check_extents(inputs[i], *out, (size_t)i);
+            if (st) return st;
+        }
+    } else {
+        // OutputPlan::new / input (pointwise.rs:262-330), input by input
+        for (int i = 0; i < n_inputs; ++i) {
+            if (!same_layout(inputs[i].pencil, out->pencil))
+                return set_err(PFH_ERR_INCOMPATIBLE_PENCILS, "pointwise arrays must have identical spatial layouts", 0,
+                               0, (size_t)i);
+            pfh_status st = check_extents(inputs[i], *out, (size_t)i);
+            if (st) return st;
+        }
+    }
+    // element types (fixed by the Rust types in the reference)
+    const int ncoef = op == PFH_POINTWISE_LINEAR ? n_inputs : op == PFH_POINTWISE_MUL ? 1 : 2;
+    if (!coeffs || !float_type(out->scalar_type)) return set_err(PFH_ERR_INVALID_ARGUMENT, "coefficients or type");
+    bool any_complex = false;
+    for (int i = 0; i < n_inputs; ++i) {
+        if (!float_type(inputs[i].scalar_type) || f64_type(inputs[i].scalar_type) != f64_type(out->scalar_type))
+            return set_err(PFH_ERR_INVALID_ARGUMENT, "inputs and output must share one floating precision", 0, 0,
+                           (size_t)i);
+        any_complex = any_complex || complex_type(inputs[i].scalar_type);
+    }
+    for (int c = 0; c < ncoef; ++c) any_complex = any_complex || coeffs[2 * c + 1] != 0.0;
+    if (any_complex && !complex_type(out->scalar_type))
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "a complex input or coefficient needs a complex output");
+    // sizes
+    int64_t spatial = 1, extra = 1;
+    for (int d = 0; d < out->pencil.ndim; ++d) {
+        if (out->pencil.shape[d] > (size_t)INT64_MAX / (size_t)std::max<int64_t>(spatial, 1))
+            return set_err(PFH_ERR_LENGTH_OVERFLOW, "spatial length");
+        spatial *= (int64_t)out->pencil.shape[d];
+    }
+    for (int d = 0; d < out->extra_ndim; ++d) {
+        if (out->extra_shape[d] > (size_t)INT64_MAX / (size_t)std::max<int64_t>(extra, 1))
+            return set_err(PFH_ERR_LENGTH_OVERFLOW, "extra count");
+        extra *= (int64_t)out->extra_shape[d];
+    }
+    if (spatial > 0 && extra > INT64_MAX / 2 / spatial) return set_err(PFH_ERR_LENGTH_OVERFLOW, "element count");
+    if (spatial == 0 || extra == 0) return ok(); // no element is visited
+    if (!out->data) return set_err(PFH_ERR_INVALID_ARGUMENT, "output data");
+    PointwiseArgs a;
+    a.op = op;
+    a.nin = n_inputs;
+    a.out_complex = complex_type(out->scalar_type) ? 1 : 0;
+    a.extra_ndim = out->extra_ndim;
+    a.extra_count = extra;
+    a.spatial = spatial;
+    for (int d = 0; d < out->extra_ndim; ++d) a.extra_dims[d] = (int64_t)out->extra_shape[d];
+    for (int i = 0; i < n_inputs; ++i) {
+        const pfh_array &in = inputs[i];
+        if (!in.data) return set_err(PFH_ERR_INVALID_ARGUMENT, "input data", 0, 0, (size_t)i);
+        if (in.data == out->data) {
+            bool same = in.scalar_type == out->scalar_type;
+            for (int d = 0; d < out->extra_ndim; ++d) same = same && in.extra_shape[d] == out->extra_shape[d];
+            if (!same)
+                return set_err(PFH_ERR_INVALID_ARGUMENT, "an input that is the output needs its type and extra shape",
+                               0, 0, (size_t)i);
+        }
+        a.in[i] = in.data;
+        a.in_complex[i] = complex_type(in.scalar_type) ? 1 : 0;
+        // row-major strides of the input's own extra shape; 0 on broadcast axes
+        int64_t stride = 1;
+        for (int d = in.extra_ndim - 1; d >= 0; --d) {
+            a.extra_stride[i][d] = in.extra_shape[d] == 1 ? 0 : stride;
+            stride *= (int64_t)in.extra_shape[d];
+        }
+    }
+    for (int c = 0; c < ncoef; ++c) {
+        a.coef[2 * c] = coeffs[2 * c];
+        a.coef[2 * c + 1] = coeffs[2 * c + 1];
+    }
+    PFH_HIP_CHECK(launch_pointwise(a, out->data, f64_type(out->scalar_type), reinterpret_cast<hipStream_t>(stream)));
     return ok();
 }
 

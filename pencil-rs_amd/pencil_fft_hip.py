@@ -62,7 +62,8 @@ STATUS_NAMES = {
     13: "WorkspaceMismatch", 14: "WorkspaceTooSmall", 15: "PreparationFailed", 16: "AllocationFailed",
     17: "InvalidSpectrum", 18: "IncompatibleDistribution", 19: "SourceLayoutMismatch",
     20: "DestinationLayoutMismatch", 21: "InvalidPermutation", 22: "ZeroGlobalExtent", 23: "InvalidArgument",
-    24: "Unsupported", 25: "Device", 26: "InvalidGraph",
+    24: "Unsupported", 25: "Device", 26: "InvalidGraph", 27: "IncompatiblePencils", 28: "ExtraRankMismatch",
+    29: "ExtraExtentMismatch", 30: "EmptyInputs",
 }
 
 
@@ -117,6 +118,26 @@ class _ErrorInfo(ctypes.Structure):
     ]
 
 
+class _PencilLayout(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int),
+        ("shape", ctypes.c_size_t * MAX_DIMS),
+        ("perm", ctypes.c_int * MAX_DIMS),
+        ("topology_dims", ctypes.c_int),
+        ("decomposition", ctypes.c_int * MAX_DIMS),
+    ]
+
+
+class _Array(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("scalar_type", ctypes.c_int),
+        ("extra_ndim", ctypes.c_int),
+        ("extra_shape", ctypes.c_size_t * MAX_DIMS),
+        ("pencil", _PencilLayout),
+    ]
+
+
 class TransformTiming(ctypes.Structure):
     """TransformTiming / StageTiming (distributed.rs:290-325)."""
 
@@ -146,6 +167,7 @@ EXPORTED_SYMBOLS = (
     "pfh_local_execute_in_place", "pfh_execute_with_timing", "pfh_workspace_take_status", "pfh_batch_create",
     "pfh_batch_destroy", "pfh_batch_workspace_bytes", "pfh_batch_execute", "pfh_batch_tuning",
     "pfh_pointwise_scale_axes", "pfh_pointwise_quadratic", "pfh_pointwise_quadratic_adjoint", "pfh_sum_squares",
+    "pfh_reduce_workspace_bytes", "pfh_reduce", "pfh_pointwise",
 )
 
 
@@ -185,6 +207,9 @@ def load_library(path: str = LIB_PATH):
     lib.pfh_pointwise_quadratic.argtypes = [i, vp, vp, sz, d, vp]
     lib.pfh_pointwise_quadratic_adjoint.argtypes = [i, vp, vp, vp, sz, d, vp]
     lib.pfh_sum_squares.argtypes = [i, vp, sz, vp, sz, vp, vp]
+    lib.pfh_reduce_workspace_bytes.argtypes = [ctypes.POINTER(sz)]
+    lib.pfh_reduce.argtypes = [i, i, vp, sz, vp, sz, vp, vp]
+    lib.pfh_pointwise.argtypes = [i, ctypes.POINTER(_Array), i, ctypes.POINTER(_Array), ctypes.POINTER(d), vp]
     _lib = lib
     return lib
 
@@ -768,3 +793,178 @@ def sum_squares(x: torch.Tensor, workspace: Optional[torch.Tensor] = None) -> to
     n = x.numel() * (2 if x.is_complex() else 1)
     _check(_lib.pfh_sum_squares(_precision(x.dtype), _ptr(x), n, _ptr(ws), ws.numel(), _ptr(res), _stream()))
     return res
+
+
+# ------------------------------------------------------------ reductions --
+# Scalar type codes: IoElement::CODE (crates/pencil-io/src/format.rs:50-60, :95-126).
+SCALAR_TYPES = {
+    torch.int8: 1, torch.uint8: 2, torch.int16: 3, torch.uint16: 4, torch.int32: 5, torch.uint32: 6,
+    torch.int64: 7, torch.uint64: 8, torch.float32: 9, torch.float64: 10, torch.complex64: 11, torch.complex128: 12,
+}
+REDUCE_SUM, REDUCE_MIN, REDUCE_MAX, REDUCE_L2_NORM, REDUCE_ANY, REDUCE_ALL = range(1, 7)
+_REDUCE_VALUE, _REDUCE_NONE, _REDUCE_INTEGER_OVERFLOW = range(3)
+_NORM_DTYPE = {torch.float32: torch.float32, torch.complex64: torch.float32}
+
+
+class CollectiveError(Exception):
+    """CollectiveError (crates/pencil-array/src/error.rs); ``variant`` names
+    the Rust variant (``IntegerOverflow``)."""
+
+    def __init__(self, variant: str):
+        self.variant = variant
+        super().__init__(variant)
+
+
+def reduce_workspace_bytes() -> int:
+    load_library()
+    n = ctypes.c_size_t()
+    _check(_lib.pfh_reduce_workspace_bytes(ctypes.byref(n)))
+    return n.value
+
+
+def reduce_async(op: int, x: torch.Tensor, result: Optional[torch.Tensor] = None,
+                 workspace: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Runs one reduction in stream order; returns the 32-byte device result
+    (pfh_reduce_result: value bytes, then the outcome word)."""
+    load_library()
+    if x.dtype not in SCALAR_TYPES:
+        raise TypeError(f"unsupported dtype {x.dtype}")
+    ws = torch.empty(reduce_workspace_bytes(), dtype=torch.uint8, device=x.device) if workspace is None else workspace
+    res = torch.empty(32, dtype=torch.uint8, device=x.device) if result is None else result
+    xp = _ptr(x) if x.numel() else None
+    _check(_lib.pfh_reduce(op, SCALAR_TYPES[x.dtype], xp, x.numel(), _ptr(ws), ws.numel(), _ptr(res), _stream()))
+    return res
+
+
+def _reduce_value(op: int, x: torch.Tensor):
+    raw = reduce_async(op, x).cpu()
+    outcome = int(raw[16:20].view(torch.int32).item())
+    if outcome == _REDUCE_INTEGER_OVERFLOW:
+        raise CollectiveError("IntegerOverflow")
+    if outcome == _REDUCE_NONE:
+        return None
+    if op in (REDUCE_ANY, REDUCE_ALL):
+        return bool(raw[0].item())
+    dt = _NORM_DTYPE.get(x.dtype, torch.float64) if op == REDUCE_L2_NORM else x.dtype
+    width = torch.empty(0, dtype=dt).element_size()
+    return raw[:width].view(dt).item()
+
+
+def global_sum(x: torch.Tensor):
+    """global_sum (collectives.rs:519-528) of a device array at one rank:
+    integer sums are checked at every storage-order step (raises
+    CollectiveError("IntegerOverflow")); NaN wins, opposite infinities give
+    NaN, a lone infinity sign is kept (complex parts independently)."""
+    return _reduce_value(REDUCE_SUM, x)
+
+
+def global_min(x: torch.Tensor):
+    """global_min (collectives.rs:530-539): None for an empty array, NaN if
+    any value is NaN."""
+    return _reduce_value(REDUCE_MIN, x)
+
+
+def global_max(x: torch.Tensor):
+    """global_max (collectives.rs:541-550)."""
+    return _reduce_value(REDUCE_MAX, x)
+
+
+def l2_norm(x: torch.Tensor):
+    """l2_norm (collectives.rs:552-565): scaled, NaN -> NaN, infinity -> +inf,
+    empty -> 0; f32 for f32 / complex64 arrays, f64 otherwise."""
+    return _reduce_value(REDUCE_L2_NORM, x)
+
+
+def any(x: torch.Tensor) -> bool:  # noqa: A001 - the reference's name
+    """any (collectives.rs:567-575): some value is non-zero."""
+    return _reduce_value(REDUCE_ANY, x)
+
+
+def all(x: torch.Tensor) -> bool:  # noqa: A001 - the reference's name
+    """all (collectives.rs:577-588): every value is non-zero (true when empty)."""
+    return _reduce_value(REDUCE_ALL, x)
+
+
+# ----------------------------------------------------- broadcast pointwise --
+POINTWISE_LINEAR, POINTWISE_MUL, POINTWISE_MUL_ADD = 1, 2, 3
+
+
+class PencilLayout:
+    """The spatial layout of a pencil at one rank (Pencil, pencil.rs:69-112):
+    global shape, memory permutation, ordered decomposition."""
+
+    def __init__(self, shape: Sequence[int], perm: Optional[Sequence[int]] = None,
+                 decomposition: Sequence[int] = ()):
+        self.shape = tuple(int(v) for v in shape)
+        self.perm = tuple(range(len(self.shape))) if perm is None else tuple(int(v) for v in perm)
+        self.decomposition = tuple(int(v) for v in decomposition)
+
+    def local_len(self) -> int:
+        n = 1
+        for v in self.shape:
+            n *= v
+        return n
+
+    def _c(self) -> _PencilLayout:
+        c = _PencilLayout()
+        c.ndim = len(self.shape)
+        for a in range(len(self.shape)):
+            c.shape[a] = self.shape[a]
+            c.perm[a] = self.perm[a] if a < len(self.perm) else -1
+        c.topology_dims = len(self.decomposition)
+        for a, v in enumerate(self.decomposition):
+            c.decomposition[a] = v
+        return c
+
+
+class PencilArray:
+    """A device pencil array (PencilArray, array.rs:17-56): ``data`` holds
+    [extra...][memory-ordered spatial] elements of a float or complex dtype."""
+
+    def __init__(self, data: torch.Tensor, layout: PencilLayout, extra_shape: Sequence[int] = ()):
+        self.data, self.layout, self.extra_shape = data, layout, tuple(int(v) for v in extra_shape)
+        count = layout.local_len()
+        for v in self.extra_shape:
+            count *= v
+        if data.numel() != count:
+            raise ValueError(f"array holds {data.numel()} elements, its shapes need {count}")
+
+    def _c(self) -> _Array:
+        c = _Array()
+        c.data = _ptr(self.data).value if self.data.numel() else None
+        c.scalar_type = SCALAR_TYPES.get(self.data.dtype, 0)
+        c.extra_ndim = len(self.extra_shape)
+        for a, v in enumerate(self.extra_shape[:MAX_DIMS]):
+            c.extra_shape[a] = v
+        c.pencil = self.layout._c()
+        return c
+
+
+def _pointwise(op: int, inputs: Sequence[PencilArray], out: PencilArray, coeffs: Sequence[complex]):
+    load_library()
+    arr = (_Array * max(1, len(inputs)))(*[a._c() for a in inputs])
+    co = (ctypes.c_double * max(2, 2 * len(coeffs)))()
+    for k, c in enumerate(coeffs):
+        c = complex(c)
+        co[2 * k], co[2 * k + 1] = c.real, c.imag
+    oc = out._c()
+    _check(_lib.pfh_pointwise(op, arr, len(inputs), ctypes.byref(oc), co, _stream()))
+    return out
+
+
+def pointwise_linear(inputs: Sequence[PencilArray], coeffs: Sequence[complex], out: PencilArray) -> PencilArray:
+    """out = sum_i coeffs[i] * inputs[i] with broadcast extras
+    (pointwise_many, pointwise.rs:354-420; out may be inputs[0], the in-place
+    form :421-490)."""
+    return _pointwise(POINTWISE_LINEAR, inputs, out, coeffs)
+
+
+def pointwise_mul(a: PencilArray, b: PencilArray, out: PencilArray, coeff: complex = 1.0) -> PencilArray:
+    """out = coeff * a * b (pointwise2, pointwise.rs:40-133)."""
+    return _pointwise(POINTWISE_MUL, [a, b], out, [coeff])
+
+
+def pointwise_mul_add(a: PencilArray, b: PencilArray, c: PencilArray, out: PencilArray, coeff_ab: complex = 1.0,
+                      coeff_c: complex = 1.0) -> PencilArray:
+    """out = coeff_ab * a * b + coeff_c * c (pointwise3, pointwise.rs:492-598)."""
+    return _pointwise(POINTWISE_MUL_ADD, [a, b, c], out, [coeff_ab, coeff_c])

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/pencil_fft_hip.h"
+#include "../../pencil-rs_amd/csrc/pfh_collectives.hpp"
 #include "../../pencil-rs_amd/csrc/pfh_kernels.hpp"
 
 namespace {
@@ -604,6 +605,51 @@ hipError_t launch_sum_squares(const void *x, int64_t n, double *partial, int npa
 template hipError_t launch_sum_squares<float>(const void *, int64_t, double *, int, double *, hipStream_t);
 template hipError_t launch_sum_squares<double>(const void *, int64_t, double *, int, double *, hipStream_t);
 
+// Reductions: bounds of the input, the partial-state slots and the result.
+hipError_t launch_reduce(int op, int scalar_type, const void *x, int64_t n, void *workspace, ReduceResult *result,
+                         hipStream_t)
+{
+    static const int width[13] = {0, 1, 1, 2, 2, 4, 4, 8, 8, 4, 8, 8, 16};
+    if (op < kReduceSum || op > kReduceAll || scalar_type < 1 || scalar_type > 12) return hipErrorInvalidValue;
+    begin_launch();
+    if (n > 0) check_read("reduce input", x, (size_t)n * width[scalar_type]);
+    int64_t parts = n / (256 * 16);
+    parts = parts < 1 ? 1 : parts > kReduceParts ? kReduceParts : parts;
+    check_write("reduce partials", workspace, (size_t)parts * kReduceStateBytes);
+    begin_launch();
+    check_write("reduce result", result, sizeof(ReduceResult));
+    return hipSuccess;
+}
+
+// Broadcast pointwise: every (extra, spatial) element of every input lies in
+// one allocation and each output element is written once.
+hipError_t launch_pointwise(const PointwiseArgs &a, void *out, bool f64, hipStream_t)
+{
+    begin_launch();
+    const size_t real = f64 ? 8 : 4;
+    const size_t oeb = real * (a.out_complex ? 2 : 1);
+    for (int64_t e = 0; e < a.extra_count; ++e) {
+        int64_t rem = e;
+        int64_t idx[4] = {0, 0, 0, 0};
+        for (int d = a.extra_ndim - 1; d >= 0; --d) {
+            idx[d] = rem % a.extra_dims[d];
+            rem /= a.extra_dims[d];
+        }
+        for (int i = 0; i < a.nin; ++i) {
+            int64_t lin = 0;
+            for (int d = 0; d < a.extra_ndim; ++d) lin += idx[d] * a.extra_stride[i][d];
+            const size_t ieb = real * (a.in_complex[i] ? 2 : 1);
+            if (!check_read("pointwise input", static_cast<const char *>(a.in[i]) + (size_t)(lin * a.spatial) * ieb,
+                            (size_t)a.spatial * ieb))
+                return hipSuccess;
+        }
+        if (!check_write("pointwise output", static_cast<char *>(out) + (size_t)(e * a.spatial) * oeb,
+                         (size_t)a.spatial * oeb))
+            return hipSuccess;
+    }
+    return hipSuccess;
+}
+
 } // namespace pfh
 
 // ---------------------------------------------------------------------------
@@ -896,6 +942,52 @@ void batch_cases()
 
 } // namespace
 
+// Broadcast pointwise calls over tight allocations: every input and output
+// element the launch touches lies inside its own buffer.
+void pointwise_cases()
+{
+    auto layout = [](pfh_array &a, std::vector<size_t> shape, std::vector<int> perm, std::vector<size_t> extra,
+                     int type) {
+        std::memset(&a, 0, sizeof(a));
+        a.scalar_type = type;
+        a.extra_ndim = (int)extra.size();
+        for (size_t d = 0; d < extra.size(); ++d) a.extra_shape[d] = extra[d];
+        a.pencil.ndim = (int)shape.size();
+        for (size_t d = 0; d < shape.size(); ++d) {
+            a.pencil.shape[d] = shape[d];
+            a.pencil.perm[d] = perm[d];
+        }
+        size_t n = (type == PFH_SCALAR_C64 || type == PFH_SCALAR_F64) ? 8 : type == PFH_SCALAR_C128 ? 16 : 4;
+        for (size_t v : shape) n *= v;
+        for (size_t v : extra) n *= v;
+        void *p = nullptr;
+        if (hipMalloc(&p, n) != hipSuccess) fail("pointwise alloc");
+        a.data = p;
+    };
+    struct Case {
+        int op, type_in, type_out;
+        std::vector<std::vector<size_t>> extras;
+        std::vector<size_t> out_extra;
+    };
+    const std::vector<Case> cases = {
+        {PFH_POINTWISE_LINEAR, PFH_SCALAR_F32, PFH_SCALAR_F32, {{2, 1, 3}, {1, 4, 3}, {2, 4, 1}}, {2, 4, 3}},
+        {PFH_POINTWISE_MUL, PFH_SCALAR_C128, PFH_SCALAR_C128, {{3}, {1}}, {3}},
+        {PFH_POINTWISE_MUL_ADD, PFH_SCALAR_F64, PFH_SCALAR_C128, {{1, 1}, {5, 2}, {5, 1}}, {5, 2}},
+        {PFH_POINTWISE_LINEAR, PFH_SCALAR_C64, PFH_SCALAR_C64, {{}, {}}, {}},
+    };
+    const std::vector<size_t> shape = {6, 5, 4};
+    const std::vector<int> perm = {2, 0, 1};
+    for (const Case &c : cases) {
+        std::vector<pfh_array> in(c.extras.size());
+        for (size_t i = 0; i < in.size(); ++i) layout(in[i], shape, perm, c.extras[i], c.type_in);
+        pfh_array out;
+        layout(out, shape, perm, c.out_extra, c.type_out);
+        const double coef[16] = {1.0, c.type_out == c.type_in ? 0.0 : 0.5, 2.0, 0.0, -1.0, 0.0};
+        const pfh_status st = pfh_pointwise(c.op, in.data(), (int)in.size(), &out, coef, nullptr);
+        if (st != PFH_OK) fail("pointwise case op %d: %s", c.op, pfh_status_string(st));
+    }
+}
+
 int main(int argc, char **argv)
 {
     if (argc == 6) {
@@ -921,6 +1013,7 @@ int main(int argc, char **argv)
     route_cases<double>(PFH_F64);
     route_cases<float>(PFH_F32);
     batch_cases();
+    pointwise_cases();
     std::printf("launches %d, failures %d\n", pfh::g_launches, g_fail);
     return g_fail ? 1 : 0;
 }
