@@ -77,7 +77,19 @@ typedef enum pfh_status {
     PFH_ERR_INCOMPATIBLE_PENCILS = 27,    /* PointwiseError::IncompatiblePencils */
     PFH_ERR_EXTRA_RANK_MISMATCH = 28,     /* PointwiseError::ExtraRankMismatch {expected = required, actual} */
     PFH_ERR_EXTRA_EXTENT_MISMATCH = 29,   /* PointwiseError::ExtraExtentMismatch {axis = index, output = required, input = actual} */
-    PFH_ERR_EMPTY_INPUTS = 30             /* MultiInputError::EmptyInputs */
+    PFH_ERR_EMPTY_INPUTS = 30,            /* MultiInputError::EmptyInputs */
+    /* on-file formats: IoError (crates/pencil-io/src/lib.rs:196-300); the
+     * message carries the variant's string field (reason / field / what /
+     * stage / operation), actual carries errno for PFH_ERR_IO_NATIVE */
+    PFH_ERR_IO_INVALID_INPUT = 31,        /* IoError::InvalidInput */
+    PFH_ERR_IO_INVALID_PATH = 32,         /* IoError::InvalidPath */
+    PFH_ERR_IO_SIZE_LIMIT = 33,           /* IoError::SizeLimit {what} */
+    PFH_ERR_IO_METADATA_MISMATCH = 34,    /* IoError::MetadataMismatch {field} */
+    PFH_ERR_IO_INVALID_FILE = 35,         /* IoError::InvalidFile {reason} */
+    PFH_ERR_IO_INCOMPLETE_FILE = 36,      /* IoError::IncompleteFile */
+    PFH_ERR_IO_NATIVE = 37,               /* IoError::Native {operation, code} (the reference's MPI_File_open failures) */
+    PFH_ERR_IO_WRITE_INCOMPLETE = 38,     /* IoError::WriteIncomplete {stage} */
+    PFH_ERR_IO_COMMIT_UNCERTAIN = 39      /* IoError::CommitUncertain {stage} */
 } pfh_status;
 
 /* Payload of the last error on the calling thread (Rust error fields). */
@@ -437,6 +449,58 @@ typedef enum pfh_pointwise_op {
  * other overlaps are not allowed. */
 pfh_status pfh_pointwise(int op, const pfh_array *inputs, int n_inputs, const pfh_array *out, const double *coeffs,
                          pfh_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* On-file formats for device pencil arrays at one rank.                  */
+/* ---------------------------------------------------------------------- */
+
+/* The single-dataset file of write_mpi / read_mpi (crates/pencil-io/src/
+ * mpi_io.rs:198-840, header layout :1492-1930): a 96-byte little-endian
+ * prefix ("PENCILIO", version 1, header length, commit marker, N, type code,
+ * width, extra rank, payload offset, payload bytes, writer grid rank,
+ * reserved 0), the u64 arrays global[N], extra[], writer grid[], writer
+ * permutation[N], then the payload in canonical row-major
+ * [extra..., logical spatial...] order, little endian. A file is created
+ * exclusively (an existing file is never replaced), written with an
+ * INCOMPLETE marker, flushed, then committed. The writer grid of a one-rank
+ * pencil is all ones (rank = topology_dims, at least 1). Reads stage and
+ * validate the whole payload before the destination is written, in the
+ * reference's order of checks; a rejected read leaves it unchanged.
+ * workspace: device memory of pfh_io_workspace_bytes (the payload size; a
+ * write of an identity-permutation pencil needs none). Both calls are
+ * synchronous with respect to the file; the final device copy of a read is
+ * stream ordered. */
+pfh_status pfh_io_workspace_bytes(const pfh_array *array, size_t *bytes);
+pfh_status pfh_io_write(const char *path, const pfh_array *array, void *workspace, size_t workspace_bytes,
+                        pfh_stream_t stream);
+pfh_status pfh_io_read(const char *path, const pfh_array *array, void *workspace, size_t workspace_bytes,
+                       pfh_stream_t stream);
+
+/* RawByteOrder (crates/pencil-io/src/options.rs:37-57) */
+enum { PFH_BYTE_ORDER_NATIVE = 0, PFH_BYTE_ORDER_LITTLE = 1, PFH_BYTE_ORDER_BIG = 2 };
+
+/* read_mpi_raw (crates/pencil-io/src/raw.rs:24-238): headerless canonical
+ * [extra..., logical spatial...] values starting at byte `offset`, each real
+ * component (a complex value has two) in `byte_order`; the whole range must
+ * lie inside the file ("truncated raw payload"). */
+pfh_status pfh_io_read_raw(const char *path, const pfh_array *array, uint64_t offset, int byte_order,
+                           void *workspace, size_t workspace_bytes, pfh_stream_t stream);
+
+/* DatasetInfo of read_mpi_catalog (crates/pencil-io/src/catalog.rs:113-300):
+ * the validated metadata of a committed file, without its payload. */
+typedef struct pfh_dataset_info {
+    int scalar_type;
+    int ndim;
+    int extra_ndim;
+    int grid_ndim;
+    uint64_t shape[PFH_MAX_DIMS];
+    uint64_t extra_shape[PFH_MAX_DIMS];
+    uint64_t writer_grid[PFH_MAX_DIMS];
+    int writer_perm[PFH_MAX_DIMS];
+    uint64_t payload_bytes;
+} pfh_dataset_info;
+
+pfh_status pfh_io_catalog(const char *path, pfh_dataset_info *info);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,16 @@ static pfh_status ok()
     return PFH_OK;
 }
 
+namespace pfh {
+// error state shared with the other translation units of the library
+pfh_status set_error(pfh_status st, const char *msg, size_t required, size_t actual, size_t batch, size_t index,
+                     int hip_error)
+{
+    return set_err(st, msg, required, actual, batch, index, hip_error);
+}
+pfh_status clear_error() { return ok(); }
+} // namespace pfh
+
 #define PFH_HIP_CHECK(expr)                                                                     \
     do {                                                                                        \
         hipError_t e_ = (expr);                                                                 \
@@ -1851,6 +1861,15 @@ const char *pfh_status_string(pfh_status s)
     case PFH_ERR_EXTRA_RANK_MISMATCH: return "pointwise extra rank mismatch";
     case PFH_ERR_EXTRA_EXTENT_MISMATCH: return "pointwise extra extent mismatch";
     case PFH_ERR_EMPTY_INPUTS: return "pointwise many operations require at least one input";
+    case PFH_ERR_IO_INVALID_INPUT: return "invalid I/O argument";
+    case PFH_ERR_IO_INVALID_PATH: return "path cannot be represented as a native path";
+    case PFH_ERR_IO_SIZE_LIMIT: return "I/O size or native count limit exceeded";
+    case PFH_ERR_IO_METADATA_MISMATCH: return "file metadata mismatch";
+    case PFH_ERR_IO_INVALID_FILE: return "invalid I/O file";
+    case PFH_ERR_IO_INCOMPLETE_FILE: return "I/O file has no completed commit marker";
+    case PFH_ERR_IO_NATIVE: return "native backend operation failed";
+    case PFH_ERR_IO_WRITE_INCOMPLETE: return "write is incomplete; the failed file was preserved";
+    case PFH_ERR_IO_COMMIT_UNCERTAIN: return "write commit state is uncertain; the failed file was preserved";
     }
     return "unknown status";
 }

@@ -63,7 +63,9 @@ STATUS_NAMES = {
     17: "InvalidSpectrum", 18: "IncompatibleDistribution", 19: "SourceLayoutMismatch",
     20: "DestinationLayoutMismatch", 21: "InvalidPermutation", 22: "ZeroGlobalExtent", 23: "InvalidArgument",
     24: "Unsupported", 25: "Device", 26: "InvalidGraph", 27: "IncompatiblePencils", 28: "ExtraRankMismatch",
-    29: "ExtraExtentMismatch", 30: "EmptyInputs",
+    29: "ExtraExtentMismatch", 30: "EmptyInputs", 31: "InvalidInput", 32: "InvalidPath", 33: "SizeLimit",
+    34: "MetadataMismatch", 35: "InvalidFile", 36: "IncompleteFile", 37: "Native", 38: "WriteIncomplete",
+    39: "CommitUncertain",
 }
 
 
@@ -80,6 +82,8 @@ class PfhError(Exception):
         self.index = info.index
         self.hip_error = info.hip_error
         msg = info.message.decode(errors="replace")
+        # the variant's string field (IoError reason / field / what / stage / operation)
+        self.detail = msg
         super().__init__(f"{self.variant}: {msg}")
 
 
@@ -138,6 +142,20 @@ class _Array(ctypes.Structure):
     ]
 
 
+class _DatasetInfo(ctypes.Structure):
+    _fields_ = [
+        ("scalar_type", ctypes.c_int),
+        ("ndim", ctypes.c_int),
+        ("extra_ndim", ctypes.c_int),
+        ("grid_ndim", ctypes.c_int),
+        ("shape", ctypes.c_uint64 * MAX_DIMS),
+        ("extra_shape", ctypes.c_uint64 * MAX_DIMS),
+        ("writer_grid", ctypes.c_uint64 * MAX_DIMS),
+        ("writer_perm", ctypes.c_int * MAX_DIMS),
+        ("payload_bytes", ctypes.c_uint64),
+    ]
+
+
 class TransformTiming(ctypes.Structure):
     """TransformTiming / StageTiming (distributed.rs:290-325)."""
 
@@ -167,7 +185,8 @@ EXPORTED_SYMBOLS = (
     "pfh_local_execute_in_place", "pfh_execute_with_timing", "pfh_workspace_take_status", "pfh_batch_create",
     "pfh_batch_destroy", "pfh_batch_workspace_bytes", "pfh_batch_execute", "pfh_batch_tuning",
     "pfh_pointwise_scale_axes", "pfh_pointwise_quadratic", "pfh_pointwise_quadratic_adjoint", "pfh_sum_squares",
-    "pfh_reduce_workspace_bytes", "pfh_reduce", "pfh_pointwise",
+    "pfh_reduce_workspace_bytes", "pfh_reduce", "pfh_pointwise", "pfh_io_workspace_bytes", "pfh_io_write",
+    "pfh_io_read", "pfh_io_read_raw", "pfh_io_catalog",
 )
 
 
@@ -210,6 +229,12 @@ def load_library(path: str = LIB_PATH):
     lib.pfh_reduce_workspace_bytes.argtypes = [ctypes.POINTER(sz)]
     lib.pfh_reduce.argtypes = [i, i, vp, sz, vp, sz, vp, vp]
     lib.pfh_pointwise.argtypes = [i, ctypes.POINTER(_Array), i, ctypes.POINTER(_Array), ctypes.POINTER(d), vp]
+    ap = ctypes.POINTER(_Array)
+    lib.pfh_io_workspace_bytes.argtypes = [ap, ctypes.POINTER(sz)]
+    lib.pfh_io_write.argtypes = [ctypes.c_char_p, ap, vp, sz, vp]
+    lib.pfh_io_read.argtypes = [ctypes.c_char_p, ap, vp, sz, vp]
+    lib.pfh_io_read_raw.argtypes = [ctypes.c_char_p, ap, ctypes.c_uint64, i, vp, sz, vp]
+    lib.pfh_io_catalog.argtypes = [ctypes.c_char_p, ctypes.POINTER(_DatasetInfo)]
     _lib = lib
     return lib
 
@@ -968,3 +993,76 @@ def pointwise_mul_add(a: PencilArray, b: PencilArray, c: PencilArray, out: Penci
                       coeff_c: complex = 1.0) -> PencilArray:
     """out = coeff_ab * a * b + coeff_c * c (pointwise3, pointwise.rs:492-598)."""
     return _pointwise(POINTWISE_MUL_ADD, [a, b, c], out, [coeff_ab, coeff_c])
+
+
+# ------------------------------------------------------------ file formats --
+BYTE_ORDER = {"native": 0, "little": 1, "big": 2}
+
+
+def _path(path) -> bytes:
+    return os.fsencode(path)
+
+
+def _io_workspace(array: PencilArray, workspace: Optional[torch.Tensor]) -> torch.Tensor:
+    c = array._c()
+    n = ctypes.c_size_t()
+    _check(_lib.pfh_io_workspace_bytes(ctypes.byref(c), ctypes.byref(n)))
+    if workspace is not None:
+        return workspace
+    return torch.empty(max(1, n.value), dtype=torch.uint8, device=array.data.device)
+
+
+def write_mpi(path, array: PencilArray, workspace: Optional[torch.Tensor] = None):
+    """write_mpi (crates/pencil-io/src/mpi_io.rs:198-208) of a device array at
+    one rank: exclusive creation, versioned header, canonical little-endian
+    payload, commit marker after the flushed payload."""
+    load_library()
+    ws = _io_workspace(array, workspace)
+    c = array._c()
+    _check(_lib.pfh_io_write(_path(path), ctypes.byref(c), _ptr(ws), ws.numel(), _stream()))
+
+
+def read_mpi(path, array: PencilArray, workspace: Optional[torch.Tensor] = None) -> PencilArray:
+    """read_mpi (mpi_io.rs:454-472): validates the header against the
+    destination's type, shapes and payload size, stages the payload, then
+    writes the destination in its own memory order; a rejected read leaves
+    it unchanged."""
+    load_library()
+    ws = _io_workspace(array, workspace)
+    c = array._c()
+    _check(_lib.pfh_io_read(_path(path), ctypes.byref(c), _ptr(ws), ws.numel(), _stream()))
+    return array
+
+
+def read_mpi_raw(path, array: PencilArray, offset: int = 0, byte_order: str = "native",
+                 workspace: Optional[torch.Tensor] = None) -> PencilArray:
+    """read_mpi_raw (crates/pencil-io/src/raw.rs:24-238): headerless canonical
+    values at a byte offset, each real component in the given byte order."""
+    load_library()
+    ws = _io_workspace(array, workspace)
+    c = array._c()
+    _check(_lib.pfh_io_read_raw(_path(path), ctypes.byref(c), int(offset), BYTE_ORDER[byte_order], _ptr(ws),
+                                ws.numel(), _stream()))
+    return array
+
+
+class DatasetInfo:
+    """DatasetInfo (crates/pencil-io/src/catalog.rs:113-160) of one file."""
+
+    def __init__(self, c: _DatasetInfo):
+        inv = {v: k for k, v in SCALAR_TYPES.items()}
+        self.scalar_type = inv.get(c.scalar_type, c.scalar_type)
+        self.global_shape = tuple(int(c.shape[i]) for i in range(c.ndim))
+        self.extra_shape = tuple(int(c.extra_shape[i]) for i in range(c.extra_ndim))
+        self.writer_grid = tuple(int(c.writer_grid[i]) for i in range(c.grid_ndim))
+        self.writer_permutation = tuple(int(c.writer_perm[i]) for i in range(c.ndim))
+        self.payload_bytes = int(c.payload_bytes)
+
+
+def read_mpi_catalog(path) -> DatasetInfo:
+    """read_mpi_catalog (catalog.rs:272-300): the validated metadata of a
+    committed file without reading its payload."""
+    load_library()
+    info = _DatasetInfo()
+    _check(_lib.pfh_io_catalog(_path(path), ctypes.byref(info)))
+    return DatasetInfo(info)
