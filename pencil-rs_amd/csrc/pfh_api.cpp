@@ -2194,6 +2194,39 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
     int nst = 0;
     for (auto &P : B.plans) nst = std::max(nst, (int)P->ops[op].steps.size());
     const int cus = B.plans.empty() ? 256 : B.plans[0]->dev.cus;
+    // Problem groups: consecutive problems whose arrays add up to at most the
+    // group budget run all their stages before the next group starts, and
+    // every group's problems take their workspace from the start of the
+    // workspace, so a group's intermediates can stay in the last-level cache.
+    int group_mb = B.tuning.batch_group_mb;
+    if (const char *e = std::getenv("PFH_BATCH_GROUP_MB")) group_mb = std::atoi(e);
+    std::vector<std::pair<int, int>> pgroups;
+    std::vector<int64_t> off(B.ws_off);
+    if (group_mb <= 0) {
+        pgroups.push_back({0, B.count});
+    } else {
+        const int64_t budget = (int64_t)group_mb << 20;
+        int a = 0;
+        int64_t acc = 0;
+        for (int i = 0; i < B.count; ++i) {
+            const int64_t b = B.plans[i]->ops[op].in.bytes();
+            if (i > a && acc + b > budget) {
+                pgroups.push_back({a, i});
+                a = i;
+                acc = 0;
+            }
+            acc += b;
+        }
+        pgroups.push_back({a, B.count});
+        for (const auto &g : pgroups) {
+            int64_t o = 0;
+            for (int i = g.first; i < g.second; ++i) {
+                off[i] = o;
+                o += (i + 1 < B.count ? B.ws_off[i + 1] : B.ws_bytes) - B.ws_off[i];
+            }
+        }
+    }
+    for (const auto &pg : pgroups)
     for (int s = 0; s < nst; ++s) {
         // group this stage's items by kernel
         struct Group {
@@ -2202,7 +2235,7 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
             std::vector<int64_t> prefix{0};
         };
         std::vector<Group> groups;
-        for (int i = 0; i < B.count; ++i) {
+        for (int i = pg.first; i < pg.second; ++i) {
             const PlanImpl &P = *B.plans[i];
             const OpPlan &o = P.ops[op];
             if (s >= (int)o.steps.size()) continue;
@@ -2233,7 +2266,7 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
                 g->bs.fem = st.fem;
             }
             WsView wv;
-            pfh_status e = view_workspace(P, o, static_cast<char *>(ws) + B.ws_off[i], (size_t)(B.ws_bytes), wv);
+            pfh_status e = view_workspace(P, o, static_cast<char *>(ws) + off[i], (size_t)(B.ws_bytes), wv);
             if (e) return e;
             void *bufs[4] = {const_cast<void *>(srcs[i]), dsts[i], wv.w[0], wv.w[1]};
             g->items.push_back(fill_args<T>(P, st, bufs, nullptr));

@@ -893,7 +893,10 @@ void batch_cases()
     std::vector<std::vector<size_t>> shapes = {{48, 60, 64}, {12, 7, 9}, {84, 16, 20}, {5, 5, 5}, {64, 32, 16}};
     std::vector<pfh_desc> ds;
     for (auto &s : shapes) ds.push_back(route_desc(PFH_C2C, PFH_F32, s, {3}));
-    for (int tune : {0, 1}) {
+    // mode 2: problems in groups of at most 1 MiB that share one workspace region
+    for (int mode : {0, 1, 2}) {
+        const int tune = mode == 1;
+        if (mode == 2) setenv("PFH_BATCH_GROUP_MB", "1", 1);
         pfh_batch b = nullptr;
         if (pfh_batch_create(ds.data(), (int)ds.size(), tune, &b) != PFH_OK) {
             fail("batch create (autotune %d)", tune);
@@ -932,7 +935,8 @@ void batch_cases()
             (void)hipFree(ws2);
             pfh_plan_destroy(p);
         }
-        std::printf("batch autotune %d: %s\n", tune, g_fail == before ? "ok" : "FAILED");
+        std::printf("batch mode %d: %s\n", mode, g_fail == before ? "ok" : "FAILED");
+        if (mode == 2) unsetenv("PFH_BATCH_GROUP_MB");
         for (auto *x : srcs) (void)hipFree(const_cast<void *>(x));
         for (auto *x : dsts) (void)hipFree(x);
         (void)hipFree(ws);
