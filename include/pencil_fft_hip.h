@@ -89,7 +89,12 @@ typedef enum pfh_status {
     PFH_ERR_IO_INCOMPLETE_FILE = 36,      /* IoError::IncompleteFile */
     PFH_ERR_IO_NATIVE = 37,               /* IoError::Native {operation, code} (the reference's MPI_File_open failures) */
     PFH_ERR_IO_WRITE_INCOMPLETE = 38,     /* IoError::WriteIncomplete {stage} */
-    PFH_ERR_IO_COMMIT_UNCERTAIN = 39      /* IoError::CommitUncertain {stage} */
+    PFH_ERR_IO_COMMIT_UNCERTAIN = 39,     /* IoError::CommitUncertain {stage} */
+    /* named containers: NamedIoError (crates/pencil-io/src/lib.rs:302-322) */
+    PFH_ERR_IO_INVALID_NAME = 40,         /* NamedIoError::InvalidName */
+    PFH_ERR_IO_DUPLICATE_NAME = 41,       /* NamedIoError::DuplicateName */
+    PFH_ERR_IO_NOT_FOUND = 42,            /* NamedIoError::NotFound */
+    PFH_ERR_IO_INVALID_TAIL = 43          /* NamedIoError::InvalidTail */
 } pfh_status;
 
 /* Payload of the last error on the calling thread (Rust error fields). */
@@ -485,6 +490,26 @@ enum { PFH_BYTE_ORDER_NATIVE = 0, PFH_BYTE_ORDER_LITTLE = 1, PFH_BYTE_ORDER_BIG 
  * lie inside the file ("truncated raw payload"). */
 pfh_status pfh_io_read_raw(const char *path, const pfh_array *array, uint64_t offset, int byte_order,
                            void *workspace, size_t workspace_bytes, pfh_stream_t stream);
+
+/* The named append-only container (v2) of write_mpi_named /
+ * append_mpi_named / read_mpi_named (crates/pencil-io/src/named_mpi.rs):
+ * a 32-byte header ("PIONAM02", 2, 32, 0) then records, each a 72-byte
+ * header ("PIOREC02", 2, name length, metadata length, payload length, total
+ * length, type code, width, 0), the UTF-8 name, metadata (spatial rank,
+ * extra rank, spatial dims, extra dims), the canonical payload and a trailing
+ * commit word written after the flushed record. Names are non-empty UTF-8 of
+ * at most 1024 bytes (slashes are literal). write creates the file
+ * exclusively; append refuses a file whose committed prefix does not reach
+ * its end (InvalidTail) or that holds the name (DuplicateName); read finds
+ * the name in the committed prefix (NotFound), checks type and shapes
+ * (MetadataMismatch "named record") and stages the payload before writing the
+ * destination. Workspace as for pfh_io_read / pfh_io_write. */
+pfh_status pfh_io_write_named(const char *path, const char *name, const pfh_array *array, void *workspace,
+                              size_t workspace_bytes, pfh_stream_t stream);
+pfh_status pfh_io_append_named(const char *path, const char *name, const pfh_array *array, void *workspace,
+                               size_t workspace_bytes, pfh_stream_t stream);
+pfh_status pfh_io_read_named(const char *path, const char *name, const pfh_array *array, void *workspace,
+                             size_t workspace_bytes, pfh_stream_t stream);
 
 /* DatasetInfo of read_mpi_catalog (crates/pencil-io/src/catalog.rs:113-300):
  * the validated metadata of a committed file, without its payload. */

@@ -152,3 +152,103 @@ def read(b: bytes, dtype, global_shape, extra, perm) -> np.ndarray:
     a = canon.reshape(list(extra) + list(global_shape))
     order = list(range(e)) + [e + perm[q] for q in range(len(perm))]
     return np.ascontiguousarray(a.transpose(order)).reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# Named (v2) append-only container (crates/pencil-io/src/named_mpi.rs)
+# ---------------------------------------------------------------------------
+NAMED_MAGIC = b"PIONAM02"
+RECORD_MAGIC = b"PIOREC02"
+CONTAINER = 32
+RECORD_HEADER = 72
+
+
+def container_header() -> bytes:
+    """container_header (named_mpi.rs:982-988)."""
+    return NAMED_MAGIC + struct.pack("<QQQ", 2, CONTAINER, 0)
+
+
+def valid_name(name: bytes) -> bool:
+    """valid_name_bytes (lib.rs:343-348)."""
+    if not name or len(name) > 1024 or b"\0" in name:
+        return False
+    try:
+        name.decode("utf-8", errors="strict")
+    except UnicodeDecodeError:
+        return False
+    return True
+
+
+def encode_record(name: str, memory: np.ndarray, global_shape, extra, perm) -> bytes:
+    """record (named_mpi.rs:920-980) + payload + commit word."""
+    nb = name.encode("utf-8")
+    payload = canonical_payload(memory, global_shape, extra, perm)
+    meta = struct.pack("<QQ", len(global_shape), len(extra)) + b"".join(struct.pack("<Q", int(v))
+                                                                        for v in list(global_shape) + list(extra))
+    total = RECORD_HEADER + len(nb) + len(meta) + len(payload) + 8
+    dt = memory.dtype
+    head = RECORD_MAGIC + struct.pack("<QQQQQQQQ", 2, len(nb), len(meta), len(payload), total, CODES[dt],
+                                      dt.itemsize, 0)
+    return head + nb + meta + payload + struct.pack("<Q", COMMIT)
+
+
+def scan(b: bytes):
+    """scan_root (named_mpi.rs:333-505): the committed records
+    [(name, code, width, global, extra, payload_offset, payload_len)] and the
+    end of the committed prefix; raises for a bad container header."""
+    size = len(b)
+    if size < CONTAINER or b[0:8] != NAMED_MAGIC or u64(b, 8) != 2 or u64(b, 16) != CONTAINER or u64(b, 24) != 0:
+        raise IoError("InvalidFile", "named container header")
+    p, out, names = CONTAINER, [], set()
+    while p < size:
+        if size - p < RECORD_HEADER:
+            break
+        if b[p:p + 8] != RECORD_MAGIC or u64(b, p + 8) != 2 or u64(b, p + 64) != 0:
+            break
+        nl, ml, payload, total = u64(b, p + 16), u64(b, p + 24), u64(b, p + 32), u64(b, p + 40)
+        if not (0 < nl <= 1024) or ml > MAX_HEADER:
+            break
+        end = p + total
+        if end > size or end >= 1 << 63:
+            break
+        if RECORD_HEADER + nl + ml + payload + 8 != total:
+            break
+        name = b[p + RECORD_HEADER:p + RECORD_HEADER + nl]
+        if not valid_name(name) or name in names:
+            break
+        meta = p + RECORD_HEADER + nl
+        nr, er = u64(b, meta), u64(b, meta + 8)
+        if not (0 < nr <= MAX_RANK) or er > MAX_RANK or nr + er > MAX_RANK or 16 + 8 * (nr + er) != ml:
+            break
+        dims = [u64(b, meta + 16 + 8 * i) for i in range(nr + er)]
+        elems = 1
+        for d in dims:
+            elems *= d
+        code, width = u64(b, p + 48), u64(b, p + 56)
+        if WIDTHS.get(code) != width or elems * width != payload:
+            break
+        if u64(b, end - 8) != COMMIT:
+            break
+        names.add(name)
+        out.append((name.decode(), code, width, dims[:nr], dims[nr:], meta + ml, payload))
+        p = end
+    return out, p
+
+
+def read_named(b: bytes, name: str, dtype, global_shape, extra, perm) -> np.ndarray:
+    """read_mpi_named at one rank (named_mpi.rs:1430-1470)."""
+    if not valid_name(name.encode("utf-8")):
+        raise IoError("InvalidName")
+    recs, _ = scan(b)
+    hit = [r for r in recs if r[0] == name]
+    if not hit:
+        raise IoError("NotFound")
+    _, code, width, glob, ex, off, plen = hit[0]
+    dt = np.dtype(dtype)
+    if code != CODES[dt] or width != dt.itemsize or list(glob) != list(global_shape) or list(ex) != list(extra):
+        raise IoError("MetadataMismatch", "named record")
+    canon = np.frombuffer(b[off:off + plen], dtype=dt.newbyteorder("<")).astype(dt)
+    e = len(extra)
+    a = canon.reshape(list(extra) + list(global_shape))
+    order = list(range(e)) + [e + perm[q] for q in range(len(perm))]
+    return np.ascontiguousarray(a.transpose(order)).reshape(-1)

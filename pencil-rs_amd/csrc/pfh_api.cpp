@@ -1870,6 +1870,10 @@ const char *pfh_status_string(pfh_status s)
     case PFH_ERR_IO_NATIVE: return "native backend operation failed";
     case PFH_ERR_IO_WRITE_INCOMPLETE: return "write is incomplete; the failed file was preserved";
     case PFH_ERR_IO_COMMIT_UNCERTAIN: return "write commit state is uncertain; the failed file was preserved";
+    case PFH_ERR_IO_INVALID_NAME: return "invalid dataset name";
+    case PFH_ERR_IO_DUPLICATE_NAME: return "dataset name already exists";
+    case PFH_ERR_IO_NOT_FOUND: return "dataset name was not found";
+    case PFH_ERR_IO_INVALID_TAIL: return "dataset container has an invalid tail";
     }
     return "unknown status";
 }

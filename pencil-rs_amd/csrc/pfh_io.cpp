@@ -27,6 +27,7 @@
 #include <cerrno>
 #include <cstdint>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/pencil_fft_hip.h"
@@ -325,6 +326,40 @@ pfh_status pfh_io_workspace_bytes(const pfh_array *array, size_t *bytes)
     return clear_error();
 }
 
+// The array's payload in canonical order at `file_offset`: permuted on the
+// device (unless the pencil's order is canonical), then copied to the host
+// in pinned chunks, each written while the next one is copied.
+static pfh_status write_canonical(int fd, const ArrayInfo &I, const pfh_array *array, void *workspace,
+                                  uint64_t file_offset, hipStream_t s)
+{
+    if (I.bytes == 0) return PFH_OK;
+    Pinned pin;
+    IO_HIP(pin.init(std::min<uint64_t>(I.bytes, kChunk)));
+    const void *canon = array->data;
+    if (!I.identity) {
+        IO_HIP(permute(I, true, array->data, workspace, s));
+        canon = workspace;
+    }
+    const uint64_t nchunks = (I.bytes + kChunk - 1) / kChunk;
+    auto issue = [&](uint64_t k) -> hipError_t {
+        const uint64_t o = k * kChunk, len = std::min<uint64_t>(kChunk, I.bytes - o);
+        hipError_t e = hipMemcpyAsync(pin.buf[k & 1], static_cast<const uint8_t *>(canon) + o, len,
+                                      hipMemcpyDeviceToHost, s);
+        return e != hipSuccess ? e : hipEventRecord(pin.ev[k & 1], s);
+    };
+    IO_HIP(issue(0));
+    for (uint64_t k = 0; k < nchunks; ++k) {
+        IO_HIP(hipEventSynchronize(pin.ev[k & 1]));
+        if (k + 1 < nchunks) IO_HIP(issue(k + 1));
+        const uint64_t o = k * kChunk, len = std::min<uint64_t>(kChunk, I.bytes - o);
+        if (!write_all(fd, pin.buf[k & 1], (size_t)len, file_offset + o)) {
+            (void)hipStreamSynchronize(s);
+            return err(PFH_ERR_IO_WRITE_INCOMPLETE, "payload write");
+        }
+    }
+    return PFH_OK;
+}
+
 pfh_status pfh_io_write(const char *path, const pfh_array *array, void *workspace, size_t workspace_bytes,
                         pfh_stream_t stream)
 {
@@ -362,31 +397,7 @@ pfh_status pfh_io_write(const char *path, const pfh_array *array, void *workspac
     f.fd = ::open(path, O_WRONLY | O_CREAT | O_EXCL | O_CLOEXEC, 0644);
     if (f.fd < 0) return set_error(PFH_ERR_IO_NATIVE, "open", 0, (size_t)errno, 0, 0, 0);
     if (!write_all(f.fd, hdr.data(), hdr.size(), 0)) return err(PFH_ERR_IO_WRITE_INCOMPLETE, "header write");
-    Pinned pin;
-    if (I.bytes > 0) IO_HIP(pin.init(std::min<uint64_t>(I.bytes, kChunk)));
-    // payload: canonical order on the device, then chunks through pinned memory
-    const void *canon = array->data;
-    if (I.bytes > 0 && !I.identity) {
-        IO_HIP(permute(I, true, array->data, workspace, s));
-        canon = workspace;
-    }
-    const uint64_t nchunks = (I.bytes + kChunk - 1) / kChunk;
-    auto issue = [&](uint64_t k) -> hipError_t {
-        const uint64_t o = k * kChunk, len = std::min<uint64_t>(kChunk, I.bytes - o);
-        hipError_t e = hipMemcpyAsync(pin.buf[k & 1], static_cast<const uint8_t *>(canon) + o, len,
-                                      hipMemcpyDeviceToHost, s);
-        return e != hipSuccess ? e : hipEventRecord(pin.ev[k & 1], s);
-    };
-    if (nchunks > 0) IO_HIP(issue(0));
-    for (uint64_t k = 0; k < nchunks; ++k) {
-        IO_HIP(hipEventSynchronize(pin.ev[k & 1]));
-        if (k + 1 < nchunks) IO_HIP(issue(k + 1));
-        const uint64_t o = k * kChunk, len = std::min<uint64_t>(kChunk, I.bytes - o);
-        if (!write_all(f.fd, pin.buf[k & 1], (size_t)len, header_len + o)) {
-            (void)hipStreamSynchronize(s);
-            return err(PFH_ERR_IO_WRITE_INCOMPLETE, "payload write");
-        }
-    }
+    if ((st = write_canonical(f.fd, I, array, workspace, header_len, s))) return st;
     if (::fsync(f.fd) != 0) return err(PFH_ERR_IO_WRITE_INCOMPLETE, "payload flush");
     // commit marker at byte 24, flushed (write_payload_and_commit, mpi_io.rs:1146-1230)
     uint8_t marker[8];
@@ -513,6 +524,250 @@ pfh_status pfh_io_read_raw(const char *path, const pfh_array *array, uint64_t of
     const int comp = complex_type ? I.width / 2 : I.width;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if ((st = stage_payload(f.fd, offset, I, workspace, swap, comp, s))) return st;
+    if ((st = commit_staged(I, array, workspace, s))) return st;
+    return clear_error();
+}
+
+// ---------------------------------------------------------------------------
+// Named (v2) append-only container (named_mpi.rs:1-1470): a 32-byte header
+// ("PIONAM02", version 2, header size 32, reserved 0), then records of a
+// 72-byte header ("PIOREC02", version, name length, metadata length, payload
+// length, total length, type code, width, reserved 0), the UTF-8 name, the
+// metadata (spatial rank, extra rank, spatial dims, extra dims), the
+// canonical payload and a trailing commit word. The committed prefix is
+// found by scanning; the first malformed or uncommitted record ends it.
+// ---------------------------------------------------------------------------
+
+constexpr size_t kContainer = 32;
+constexpr size_t kRecordHeader = 72;
+constexpr uint64_t kNamedVersion = 2;
+constexpr size_t kMaxName = 1024;
+const char kNamedMagic[8] = {'P', 'I', 'O', 'N', 'A', 'M', '0', '2'};
+const char kRecordMagic[8] = {'P', 'I', 'O', 'R', 'E', 'C', '0', '2'};
+
+// valid_name_bytes (lib.rs:343-348): non-empty, at most 1024 bytes, no NUL,
+// well-formed UTF-8 (no overlong forms, surrogates or code points above
+// U+10FFFF, as Rust's from_utf8 requires).
+static bool valid_name(const char *name, size_t *len)
+{
+    if (!name) return false;
+    const size_t n = std::strlen(name); // a NUL ends the C string: embedded NULs cannot be passed
+    *len = n;
+    if (n == 0 || n > kMaxName) return false;
+    const auto *b = reinterpret_cast<const uint8_t *>(name);
+    for (size_t i = 0; i < n;) {
+        const uint8_t c = b[i];
+        int k;
+        uint32_t cp, lo;
+        if (c < 0x80) {
+            ++i;
+            continue;
+        } else if ((c & 0xE0) == 0xC0) {
+            k = 1, cp = c & 0x1F, lo = 0x80;
+        } else if ((c & 0xF0) == 0xE0) {
+            k = 2, cp = c & 0x0F, lo = 0x800;
+        } else if ((c & 0xF8) == 0xF0) {
+            k = 3, cp = c & 0x07, lo = 0x10000;
+        } else {
+            return false;
+        }
+        for (int j = 1; j <= k; ++j) {
+            if (i + j >= n || (b[i + j] & 0xC0) != 0x80) return false;
+            cp = (cp << 6) | (b[i + j] & 0x3F);
+        }
+        if (cp < lo || cp > 0x10FFFF || (cp >= 0xD800 && cp <= 0xDFFF)) return false;
+        i += k + 1;
+    }
+    return true;
+}
+
+struct NamedRecord {
+    std::string name;
+    uint64_t type = 0, width = 0, payload_offset = 0, payload = 0;
+    std::vector<uint64_t> global, extra;
+};
+
+// scan_root (named_mpi.rs:333-505): the committed records and the end of the
+// committed prefix.
+static pfh_status scan_named(int fd, uint64_t size, std::vector<NamedRecord> &out, uint64_t *tail)
+{
+    uint8_t h[kContainer];
+    if (size < kContainer || !read_all(fd, h, kContainer, 0) || std::memcmp(h, kNamedMagic, 8) != 0 ||
+        get_u64(h, 8) != kNamedVersion || get_u64(h, 16) != kContainer || get_u64(h, 24) != 0)
+        return err(PFH_ERR_IO_INVALID_FILE, "named container header");
+    uint64_t p = kContainer;
+    while (p < size) {
+        if (size - p < kRecordHeader) break;
+        uint8_t rh[kRecordHeader];
+        if (!read_all(fd, rh, kRecordHeader, p)) return err(PFH_ERR_IO_INVALID_FILE, "named metadata read");
+        if (std::memcmp(rh, kRecordMagic, 8) != 0 || get_u64(rh, 8) != kNamedVersion || get_u64(rh, 64) != 0) break;
+        const uint64_t nl = get_u64(rh, 16), ml = get_u64(rh, 24), payload = get_u64(rh, 32), total = get_u64(rh, 40);
+        if (nl == 0 || nl > kMaxName || ml > kMaxHeader) break;
+        if (total > size - p || p + total > (uint64_t)INT64_MAX) break;
+        if (payload > total || kRecordHeader + nl + ml + payload + 8 != total) break;
+        std::vector<uint8_t> body(nl + ml);
+        if (!read_all(fd, body.data(), body.size(), p + kRecordHeader))
+            return err(PFH_ERR_IO_INVALID_FILE, "named metadata read");
+        std::string name(reinterpret_cast<const char *>(body.data()), nl);
+        size_t len = 0;
+        if (std::memchr(name.data(), 0, nl) || !valid_name(name.c_str(), &len) || len != nl) break;
+        bool dup = false;
+        for (const NamedRecord &r : out) dup = dup || r.name == name;
+        if (dup) break;
+        if (ml < 16) break;
+        const uint8_t *meta = body.data() + nl;
+        const uint64_t nr = get_u64(meta, 0), er = get_u64(meta, 8);
+        if (nr == 0 || nr > kMaxRank || er > kMaxRank || nr + er > kMaxRank || 16 + 8 * (nr + er) != ml) break;
+        std::vector<uint64_t> dims(nr + er);
+        uint64_t elements = 1;
+        bool ok = true;
+        for (uint64_t i = 0; i < nr + er && ok; ++i) {
+            dims[i] = get_u64(meta, 16 + 8 * i);
+            ok = mul_ok(elements, dims[i], &elements);
+        }
+        if (!ok) break;
+        const uint64_t type = get_u64(rh, 48), width = get_u64(rh, 56);
+        uint64_t bytes = 0;
+        if (type_width(type) == 0 || (uint64_t)type_width(type) != width || !mul_ok(elements, width, &bytes) ||
+            bytes != payload)
+            break;
+        uint8_t marker[8];
+        if (!read_all(fd, marker, 8, p + total - 8)) return err(PFH_ERR_IO_INVALID_FILE, "named metadata read");
+        if (get_u64(marker, 0) != kCommit) break;
+        NamedRecord r;
+        r.name = name;
+        r.type = type;
+        r.width = width;
+        r.payload = payload;
+        r.global.assign(dims.begin(), dims.begin() + (ptrdiff_t)nr);
+        r.extra.assign(dims.begin() + (ptrdiff_t)nr, dims.end());
+        r.payload_offset = p + kRecordHeader + nl + ml;
+        out.push_back(std::move(r));
+        p += total;
+    }
+    *tail = p;
+    return PFH_OK;
+}
+
+// record() (named_mpi.rs:920-980): the record header, name and metadata.
+static std::vector<uint8_t> record_head(const char *name, size_t nl, const ArrayInfo &I, int type, uint64_t *total)
+{
+    const uint64_t ml = 16 + 8 * (uint64_t)(I.n + I.extra_rank);
+    *total = kRecordHeader + nl + ml + I.bytes + 8;
+    std::vector<uint8_t> h(kRecordHeader + nl + ml, 0);
+    std::memcpy(h.data(), kRecordMagic, 8);
+    put_u64(h.data(), 8, kNamedVersion);
+    put_u64(h.data(), 16, nl);
+    put_u64(h.data(), 24, ml);
+    put_u64(h.data(), 32, I.bytes);
+    put_u64(h.data(), 40, *total);
+    put_u64(h.data(), 48, (uint64_t)type);
+    put_u64(h.data(), 56, (uint64_t)I.width);
+    put_u64(h.data(), 64, 0);
+    std::memcpy(h.data() + kRecordHeader, name, nl);
+    uint8_t *meta = h.data() + kRecordHeader + nl;
+    put_u64(meta, 0, (uint64_t)I.n);
+    put_u64(meta, 8, (uint64_t)I.extra_rank);
+    for (int d = 0; d < I.n; ++d) put_u64(meta, 16 + 8 * d, I.global[d]);
+    for (int d = 0; d < I.extra_rank; ++d) put_u64(meta, 16 + 8 * (I.n + d), I.extra[d]);
+    return h;
+}
+
+static pfh_status write_record(bool append, const char *path, const char *name, const pfh_array *array,
+                               void *workspace, size_t workspace_bytes, pfh_stream_t stream)
+{
+    size_t nl = 0;
+    if (!valid_name(name, &nl)) return err(PFH_ERR_IO_INVALID_NAME, "invalid dataset name");
+    pfh_status st = check_path(path);
+    if (st) return st;
+    ArrayInfo I;
+    if ((st = array_info(array, I))) return st;
+    if (!I.identity && I.bytes > 0 && (!workspace || workspace_bytes < I.bytes))
+        return set_error(PFH_ERR_WORKSPACE_TOO_SMALL, "canonical staging", (size_t)I.bytes,
+                         workspace ? workspace_bytes : 0, 0, 0, 0);
+    uint64_t total = 0;
+    const std::vector<uint8_t> head = record_head(name, nl, I, array->scalar_type, &total);
+    Fd f;
+    uint64_t offset = kContainer;
+    if (append) {
+        f.fd = ::open(path, O_RDWR | O_CLOEXEC);
+        if (f.fd < 0) return set_error(PFH_ERR_IO_NATIVE, "open", 0, (size_t)errno, 0, 0, 0);
+        struct stat sb;
+        if (::fstat(f.fd, &sb) != 0) return set_error(PFH_ERR_IO_NATIVE, "fstat", 0, (size_t)errno, 0, 0, 0);
+        std::vector<NamedRecord> recs;
+        uint64_t tail = 0;
+        if ((st = scan_named(f.fd, (uint64_t)sb.st_size, recs, &tail))) return st;
+        if (tail != (uint64_t)sb.st_size) return err(PFH_ERR_IO_INVALID_TAIL, "dataset container has an invalid tail");
+        for (const NamedRecord &r : recs)
+            if (r.name == std::string(name, nl)) return err(PFH_ERR_IO_DUPLICATE_NAME, "dataset name already exists");
+        offset = tail;
+    } else {
+        f.fd = ::open(path, O_RDWR | O_CREAT | O_EXCL | O_CLOEXEC, 0644);
+        if (f.fd < 0) return set_error(PFH_ERR_IO_NATIVE, "open", 0, (size_t)errno, 0, 0, 0);
+        uint8_t ch[kContainer] = {0};
+        std::memcpy(ch, kNamedMagic, 8);
+        put_u64(ch, 8, kNamedVersion);
+        put_u64(ch, 16, kContainer);
+        if (!write_all(f.fd, ch, kContainer, 0)) return err(PFH_ERR_IO_WRITE_INCOMPLETE, "container header write");
+    }
+    if (offset > (uint64_t)INT64_MAX - total) return err(PFH_ERR_IO_SIZE_LIMIT, "named container");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (!write_all(f.fd, head.data(), head.size(), offset)) return err(PFH_ERR_IO_WRITE_INCOMPLETE, "record header write");
+    if ((st = write_canonical(f.fd, I, array, workspace, offset + head.size(), s))) return st;
+    if (::fsync(f.fd) != 0) return err(PFH_ERR_IO_WRITE_INCOMPLETE, "record flush");
+    uint8_t marker[8];
+    put_u64(marker, 0, kCommit);
+    if (!write_all(f.fd, marker, 8, offset + total - 8)) return err(PFH_ERR_IO_COMMIT_UNCERTAIN, "commit word write");
+    if (::fsync(f.fd) != 0) return err(PFH_ERR_IO_COMMIT_UNCERTAIN, "commit word flush");
+    const int fd = f.fd;
+    f.fd = -1;
+    if (::close(fd) != 0) return set_error(PFH_ERR_IO_NATIVE, "close", 0, (size_t)errno, 0, 0, 0);
+    return clear_error();
+}
+
+pfh_status pfh_io_write_named(const char *path, const char *name, const pfh_array *array, void *workspace,
+                              size_t workspace_bytes, pfh_stream_t stream)
+{
+    return write_record(false, path, name, array, workspace, workspace_bytes, stream);
+}
+
+pfh_status pfh_io_append_named(const char *path, const char *name, const pfh_array *array, void *workspace,
+                               size_t workspace_bytes, pfh_stream_t stream)
+{
+    return write_record(true, path, name, array, workspace, workspace_bytes, stream);
+}
+
+pfh_status pfh_io_read_named(const char *path, const char *name, const pfh_array *array, void *workspace,
+                             size_t workspace_bytes, pfh_stream_t stream)
+{
+    size_t nl = 0;
+    if (!valid_name(name, &nl)) return err(PFH_ERR_IO_INVALID_NAME, "invalid dataset name");
+    pfh_status st = check_path(path);
+    if (st) return st;
+    ArrayInfo I;
+    if ((st = array_info(array, I))) return st;
+    if (I.bytes > 0 && (!workspace || workspace_bytes < I.bytes))
+        return set_error(PFH_ERR_WORKSPACE_TOO_SMALL, "read staging", (size_t)I.bytes, workspace ? workspace_bytes : 0,
+                         0, 0, 0);
+    Fd f;
+    f.fd = ::open(path, O_RDONLY | O_CLOEXEC);
+    if (f.fd < 0) return set_error(PFH_ERR_IO_NATIVE, "open", 0, (size_t)errno, 0, 0, 0);
+    struct stat sb;
+    if (::fstat(f.fd, &sb) != 0) return set_error(PFH_ERR_IO_NATIVE, "fstat", 0, (size_t)errno, 0, 0, 0);
+    std::vector<NamedRecord> recs;
+    uint64_t tail = 0;
+    if ((st = scan_named(f.fd, (uint64_t)sb.st_size, recs, &tail))) return st;
+    const NamedRecord *r = nullptr;
+    for (const NamedRecord &x : recs)
+        if (x.name == std::string(name, nl)) r = &x;
+    if (!r) return err(PFH_ERR_IO_NOT_FOUND, "dataset name was not found");
+    bool same = r->type == (uint64_t)array->scalar_type && r->width == (uint64_t)I.width &&
+                r->global.size() == (size_t)I.n && r->extra.size() == (size_t)I.extra_rank;
+    for (int d = 0; same && d < I.n; ++d) same = r->global[d] == I.global[d];
+    for (int d = 0; same && d < I.extra_rank; ++d) same = r->extra[d] == I.extra[d];
+    if (!same) return err(PFH_ERR_IO_METADATA_MISMATCH, "named record");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if ((st = stage_payload(f.fd, r->payload_offset, I, workspace, false, 1, s))) return st;
     if ((st = commit_staged(I, array, workspace, s))) return st;
     return clear_error();
 }

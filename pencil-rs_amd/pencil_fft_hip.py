@@ -65,7 +65,7 @@ STATUS_NAMES = {
     24: "Unsupported", 25: "Device", 26: "InvalidGraph", 27: "IncompatiblePencils", 28: "ExtraRankMismatch",
     29: "ExtraExtentMismatch", 30: "EmptyInputs", 31: "InvalidInput", 32: "InvalidPath", 33: "SizeLimit",
     34: "MetadataMismatch", 35: "InvalidFile", 36: "IncompleteFile", 37: "Native", 38: "WriteIncomplete",
-    39: "CommitUncertain",
+    39: "CommitUncertain", 40: "InvalidName", 41: "DuplicateName", 42: "NotFound", 43: "InvalidTail",
 }
 
 
@@ -186,7 +186,8 @@ EXPORTED_SYMBOLS = (
     "pfh_batch_destroy", "pfh_batch_workspace_bytes", "pfh_batch_execute", "pfh_batch_tuning",
     "pfh_pointwise_scale_axes", "pfh_pointwise_quadratic", "pfh_pointwise_quadratic_adjoint", "pfh_sum_squares",
     "pfh_reduce_workspace_bytes", "pfh_reduce", "pfh_pointwise", "pfh_io_workspace_bytes", "pfh_io_write",
-    "pfh_io_read", "pfh_io_read_raw", "pfh_io_catalog",
+    "pfh_io_read", "pfh_io_read_raw", "pfh_io_catalog", "pfh_io_write_named", "pfh_io_append_named",
+    "pfh_io_read_named",
 )
 
 
@@ -235,6 +236,8 @@ def load_library(path: str = LIB_PATH):
     lib.pfh_io_read.argtypes = [ctypes.c_char_p, ap, vp, sz, vp]
     lib.pfh_io_read_raw.argtypes = [ctypes.c_char_p, ap, ctypes.c_uint64, i, vp, sz, vp]
     lib.pfh_io_catalog.argtypes = [ctypes.c_char_p, ctypes.POINTER(_DatasetInfo)]
+    for fn in (lib.pfh_io_write_named, lib.pfh_io_append_named, lib.pfh_io_read_named):
+        fn.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ap, vp, sz, vp]
     _lib = lib
     return lib
 
@@ -1066,3 +1069,37 @@ def read_mpi_catalog(path) -> DatasetInfo:
     info = _DatasetInfo()
     _check(_lib.pfh_io_catalog(_path(path), ctypes.byref(info)))
     return DatasetInfo(info)
+
+
+def _named(fn, path, name: str, array: PencilArray, workspace: Optional[torch.Tensor]):
+    load_library()
+    ws = _io_workspace(array, workspace)
+    c = array._c()
+    raw = name.encode("utf-8") if isinstance(name, str) else bytes(name)
+    if b"\0" in raw:
+        # a C string cannot carry NUL: the name is invalid (valid_name_bytes, lib.rs:343-348)
+        raw = b""
+    _check(fn(_path(path), raw, ctypes.byref(c), _ptr(ws), ws.numel(), _stream()))
+    return array
+
+
+def write_mpi_named(path, name: str, array: PencilArray, workspace: Optional[torch.Tensor] = None):
+    """write_mpi_named (crates/pencil-io/src/named_mpi.rs:996-1019): a new
+    named (v2) container holding one committed record."""
+    _named(_lib_fn("pfh_io_write_named"), path, name, array, workspace)
+
+
+def append_mpi_named(path, name: str, array: PencilArray, workspace: Optional[torch.Tensor] = None):
+    """append_mpi_named: one more committed record (InvalidTail if the
+    committed prefix does not reach the end of the file, DuplicateName if the
+    name is taken)."""
+    _named(_lib_fn("pfh_io_append_named"), path, name, array, workspace)
+
+
+def read_mpi_named(path, name: str, array: PencilArray, workspace: Optional[torch.Tensor] = None) -> PencilArray:
+    """read_mpi_named: the committed record `name` into the destination."""
+    return _named(_lib_fn("pfh_io_read_named"), path, name, array, workspace)
+
+
+def _lib_fn(name: str):
+    return getattr(load_library(), name)

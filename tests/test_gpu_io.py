@@ -139,3 +139,52 @@ def test_full_size_field_round_trip(gpu, tmp_path):
         v = np.frombuffer(f.read(4), dtype="<f4")[0]
     assert v == x.reshape(512, 512, 512)[7, 5, 3].item()
     os.remove(path)
+
+
+def test_named_container_write_append_read(gpu, tmp_path):
+    """write_mpi_named A, append_mpi_named B: the file equals the oracle's
+    container bit for bit; both records read back into other memory orders;
+    a duplicate name and a corrupted tail are refused without touching the
+    file (tests/named_mpi.rs:320-590 at one rank)."""
+    a_log = np.array([[[e * 10000 + x * 100 + y for y in range(5)] for x in range(4)] for e in range(2)],
+                     dtype=np.int32)
+    a_mem = np.ascontiguousarray(a_log.transpose(0, 2, 1)).reshape(-1)
+    A = pf.PencilArray(dev(a_mem), pf.PencilLayout((4, 5), (1, 0), (0,)), (2,))
+    B = pf.PencilArray(torch.full((12,), 3.25, dtype=torch.float64, device="cuda"), pf.PencilLayout((3, 4)))
+    path = tmp_path / "named.pio"
+    pf.write_mpi_named(path, "A", A)
+    base = path.read_bytes()
+    assert base == PIO.container_header() + PIO.encode_record("A", a_mem, (4, 5), (2,), (1, 0))
+    pf.append_mpi_named(path, "B", B)
+    raw = path.read_bytes()
+    assert raw[:len(base)] == base
+    assert raw == base + PIO.encode_record("B", np.full(12, 3.25), (3, 4), (), (0, 1))
+    ar = pf.PencilArray(torch.full((40,), -77, dtype=torch.int32, device="cuda"), pf.PencilLayout((4, 5)), (2,))
+    pf.read_mpi_named(path, "A", ar)
+    assert np.array_equal(ar.data.cpu().numpy(), a_log.reshape(-1))
+    br = pf.PencilArray(torch.full((12,), -2.0, dtype=torch.float64, device="cuda"), pf.PencilLayout((3, 4), (1, 0)))
+    pf.read_mpi_named(path, "B", br)
+    assert bool(torch.all(br.data == 3.25))
+    with pytest.raises(pf.PfhError) as e:
+        pf.append_mpi_named(path, "A", A)
+    assert e.value.variant == "DuplicateName" and path.read_bytes() == raw
+    pf.append_mpi_named(path, "utf8/α", A)
+    u = pf.PencilArray(torch.full((40,), -5, dtype=torch.int32, device="cuda"), pf.PencilLayout((4, 5), (1, 0)), (2,))
+    pf.read_mpi_named(path, "utf8/α", u)
+    assert np.array_equal(u.data.cpu().numpy(), a_mem)
+    # a record cut short: the committed prefix still reads, append is refused
+    cut = path.read_bytes()[:-3]
+    path.write_bytes(cut)
+    pf.read_mpi_named(path, "B", br)
+    before = u.data.clone()
+    with pytest.raises(pf.PfhError) as e:
+        pf.read_mpi_named(path, "utf8/α", u)
+    assert e.value.variant == "NotFound" and torch.equal(u.data, before)
+    with pytest.raises(pf.PfhError) as e:
+        pf.append_mpi_named(path, "C", A)
+    assert e.value.variant == "InvalidTail" and path.read_bytes() == cut
+    # empty extras are valid records
+    z = pf.PencilArray(torch.zeros(0, dtype=torch.int32, device="cuda"), pf.PencilLayout((2, 2)), (0,))
+    zpath = tmp_path / "zero.pio"
+    pf.write_mpi_named(zpath, "zero", z)
+    pf.read_mpi_named(zpath, "zero", z)

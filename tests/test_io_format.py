@@ -185,3 +185,117 @@ def test_abi_raw_read_checks(pf, tmp_path):
     assert name == "InvalidInput"
     name, i = abi_read(pf, path, raw=(2 ** 64 - 8, 1))
     assert (name, i.message.decode()) == ("SizeLimit", "raw payload end")
+
+
+# ------------------------------------------------------- named container --
+def named_a_memory():
+    """e * 10000 + x * 100 + y at logical (e, x, y) of a [4, 5] array with
+    extras [2], stored [e][y][x] (the [1, 0] permutation)."""
+    v = np.array([[[e * 10000 + x * 100 + y for y in range(5)] for x in range(4)] for e in range(2)], dtype=np.int32)
+    return np.ascontiguousarray(v.transpose(0, 2, 1)).reshape(-1)
+
+
+def named_file():
+    a = PIO.encode_record("A", named_a_memory(), (4, 5), (2,), (1, 0))
+    b = PIO.encode_record("B", np.full(12, 3.25), (3, 4), (), (0, 1))
+    return PIO.container_header() + a + b, len(PIO.container_header()) + len(a)
+
+
+def test_named_oracle_matches_the_reference_raw_layout():
+    """raw_oracle (crates/pencil-io/tests/named_mpi.rs:82-140), restated."""
+    raw, bstart = named_file()
+    word = lambda p: struct.unpack_from("<Q", raw, p)[0]
+    assert raw[:8] == b"PIONAM02" and word(8) == 2 and word(16) == 32
+    p = 32
+    for key, code, width, dims, extra in (("A", 5, 4, [4, 5], [2]), ("B", 10, 8, [3, 4], [])):
+        assert raw[p:p + 8] == b"PIOREC02" and word(p + 8) == 2
+        nl, ml, payload, total = word(p + 16), word(p + 24), word(p + 32), word(p + 40)
+        assert (word(p + 48), word(p + 56), word(p + 64)) == (code, width, 0)
+        assert raw[p + 72:p + 72 + nl] == key.encode()
+        meta = p + 72 + nl
+        assert word(meta) == 2 and word(meta + 8) == len(extra) and ml == 16 + 8 * (2 + len(extra))
+        for i, d in enumerate(dims + extra):
+            assert word(meta + 16 + 8 * i) == d
+        data = meta + ml
+        assert payload == int(np.prod(dims)) * int(np.prod(extra, dtype=np.int64)) * width
+        assert total == 72 + nl + ml + payload + 8
+        assert raw[p + total - 8:p + total] == (0x434F4D4D49545445).to_bytes(8, "little")
+        if key == "A":
+            expected = [e * 10000 + x * 100 + y for e in range(2) for x in range(4) for y in range(5)]
+            assert list(np.frombuffer(raw[data:data + payload], dtype="<i4")) == expected
+        else:
+            assert np.all(np.frombuffer(raw[data:data + payload], dtype="<f8") == 3.25)
+        p += total
+    assert p == len(raw)
+    assert p - total == bstart
+
+
+def bad_tails():
+    """tests/named_mpi.rs:556-575: record B's header fields corrupted, or the
+    file cut inside record B."""
+    raw, bstart = named_file()
+    out = []
+    for off, value in ((8, 99), (16, 2 ** 64 - 1), (24, 2 ** 64 - 1), (32, 1), (40, 2 ** 64 - 1), (48, 99),
+                       (56, 1), (64, 1), (73, 2 ** 64 - 1)):
+        c = bytearray(raw)
+        c[bstart + off:bstart + off + 8] = value.to_bytes(8, "little")
+        out.append(bytes(c))
+    for cut in (bstart + 1, bstart + 72, len(raw) - 8, len(raw) - 1):
+        out.append(raw[:cut])
+    return out
+
+
+@pytest.mark.parametrize("case", range(13))
+def test_named_bad_tails(pf, tmp_path, case):
+    corrupt = bad_tails()[case]
+    recs, tail = PIO.scan(corrupt)
+    assert [r[0] for r in recs] == ["A"] and tail < len(corrupt)
+    assert np.array_equal(PIO.read_named(corrupt, "A", np.int32, (4, 5), (2,), (1, 0)), named_a_memory())
+    with pytest.raises(PIO.IoError) as e:
+        PIO.read_named(corrupt, "B", np.float64, (3, 4), (), (0, 1))
+    assert e.value.variant == "NotFound"
+    path = tmp_path / "named.pio"
+    path.write_bytes(corrupt)
+    arr = fake_array(pf)
+    ws = ctypes.c_void_p(0x2000)
+    st = pf._lib.pfh_io_append_named(os.fsencode(str(path)), b"new", ctypes.byref(arr), ws, 1 << 20, None)
+    assert pf.STATUS_NAMES[st] == "InvalidTail"
+    assert path.read_bytes() == corrupt
+    b_arr = fake_array(pf, shape=(3, 4), perm=(0, 1), extra=(), code=10)
+    st = pf._lib.pfh_io_read_named(os.fsencode(str(path)), b"B", ctypes.byref(b_arr), ws, 1 << 20, None)
+    assert pf.STATUS_NAMES[st] == "NotFound"
+
+
+def test_named_abi_checks(pf, tmp_path):
+    raw, _ = named_file()
+    path = tmp_path / "named.pio"
+    path.write_bytes(raw)
+    arr = fake_array(pf)
+    ws = ctypes.c_void_p(0x2000)
+    f = os.fsencode(str(path))
+
+    def call(fn, name, a=arr):
+        st = fn(f, name, ctypes.byref(a), ws, 1 << 20, None)
+        info = pf._ErrorInfo()
+        pf._lib.pfh_last_error(ctypes.byref(info))
+        return pf.STATUS_NAMES[st], info.message.decode()
+
+    for bad in (b"", b"x" * 1025, b"\xff\xfe", b"\xed\xa0\x80", b"\xc0\xaf"):
+        assert not PIO.valid_name(bad)
+        assert call(pf._lib.pfh_io_append_named, bad)[0] == "InvalidName"
+        assert call(pf._lib.pfh_io_read_named, bad)[0] == "InvalidName"
+    assert PIO.valid_name("utf8/α".encode()) and PIO.valid_name(b"x" * 1024)
+    assert call(pf._lib.pfh_io_append_named, b"A")[0] == "DuplicateName"
+    assert call(pf._lib.pfh_io_read_named, b"absent")[0] == "NotFound"
+    assert call(pf._lib.pfh_io_read_named, b"A", fake_array(pf, extra=(3,))) == ("MetadataMismatch", "named record")
+    assert call(pf._lib.pfh_io_read_named, b"B")[0] == "MetadataMismatch"
+    assert path.read_bytes() == raw
+    # an exclusive write onto an existing container fails without touching it
+    assert call(pf._lib.pfh_io_write_named, b"C")[0] == "Native"
+    assert path.read_bytes() == raw
+    bad = bytearray(raw)
+    bad[24] = 1
+    path.write_bytes(bytes(bad))
+    assert call(pf._lib.pfh_io_read_named, b"A") == ("InvalidFile", "named container header")
+    with pytest.raises(PIO.IoError):
+        PIO.scan(bytes(bad))
