@@ -273,7 +273,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / PEAK_GBPS,
             "traffic": traffic,
-            "kernel": "line stages (k_line_fast<float,16,16|32,0,16>)",
+            "kernel": "line stages (k_line_fast<float,16,16|32,0,16,NT>)",
             "avg_launch_ms": avg_ms,
             "algorithmic_bytes_per_launch": avg_bytes,
             "launches_per_round_trip": launches / args.timing_reps,

@@ -413,7 +413,7 @@ __device__ __forceinline__ void radix_pass_generic(cpx<T> *ln, const cpx<T> *roo
 
 
 // ---- load: global -> LDS (raw line values, all loads of a chunk in flight)
-template <typename T>
+template <typename T, bool NT>
 __device__ __forceinline__ void stage_load(const StageArgs<T> &a, cpx<T> *data, int64_t tile, int tid, int l, int tl)
 {
     const int lt = a.lt, tpl = a.tpl;
@@ -444,7 +444,7 @@ __device__ __forceinline__ void stage_load(const StageArgs<T> &a, cpx<T> *data, 
 #pragma unroll
                 for (int u = 0; u < Regs<T>::unroll; ++u) {
                     const int p = base + u * step;
-                    if (p < a.nin) v[u] = s[(int64_t)p * a.ips];
+                    if (p < a.nin) v[u] = gld<NT>(s + (int64_t)p * a.ips);
                 }
 #pragma unroll
                 for (int u = 0; u < Regs<T>::unroll; ++u) {
@@ -463,7 +463,7 @@ __device__ __forceinline__ void stage_load(const StageArgs<T> &a, cpx<T> *data, 
 #pragma unroll
                 for (int u = 0; u < 2 * Regs<T>::unroll; ++u) {
                     const int p = base + u * step;
-                    if (p < a.nin) v[u] = s[(int64_t)p * a.ips];
+                    if (p < a.nin) v[u] = gld<NT>(s + (int64_t)p * a.ips);
                 }
 #pragma unroll
                 for (int u = 0; u < 2 * Regs<T>::unroll; ++u) {
@@ -477,7 +477,7 @@ __device__ __forceinline__ void stage_load(const StageArgs<T> &a, cpx<T> *data, 
 }
 
 // ---- post-processing + store: LDS -> global (output layout)
-template <typename T, int MG>
+template <typename T, int MG, bool NT>
 __device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> *data, const cpx<T> *tws,
                                             int64_t tile, int tid, int l, int tl)
 {
@@ -506,7 +506,7 @@ __device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> 
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         const int k = base + u * tpl;
-                        if (k < a.nout) d[(int64_t)k * a.ops] = L(Z, k);
+                        if (k < a.nout) gst<NT>(d + (int64_t)k * a.ops, L(Z, k));
                     }
                 }
             } else if (a.mode == M_COPY_R) {
@@ -515,7 +515,7 @@ __device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> 
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         const int k = base + u * tpl;
-                        if (k < a.nout) d[(int64_t)k * a.ops] = LR(Z, k);
+                        if (k < a.nout) gst<NT>(d + (int64_t)k * a.ops, LR(Z, k));
                     }
                 }
             }
@@ -532,7 +532,7 @@ __device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> 
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         const int k = base + u * tpl;
-                        if (k < a.nout) d[(int64_t)k * a.ops] = v[u];
+                        if (k < a.nout) gst<NT>(d + (int64_t)k * a.ops, v[u]);
                     }
                 }
             } else {
@@ -547,7 +547,7 @@ __device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> 
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         const int k = base + u * tpl;
-                        if (k < a.nout) d[(int64_t)k * a.ops] = v[u];
+                        if (k < a.nout) gst<NT>(d + (int64_t)k * a.ops, v[u]);
                     }
                 }
             }
@@ -596,9 +596,9 @@ __device__ __forceinline__ void stage_body(const StageArgs<T> &a, int64_t tile, 
     const int l = act ? tid / tpl : 0;
     const int tl = act ? tid % tpl : (1 << 28);
     cpx<T> *ln = data + l * a.ls;
-    stage_load(a, data, tile, tid, l, tl);
+    stage_load<T, false>(a, data, tile, tid, l, tl);
     if constexpr (MG == 0) {
-        stage_store<T, 0>(a, data, tws, tile, tid, l, tl);
+        stage_store<T, 0, false>(a, data, tws, tile, tid, l, tl);
         return;
     }
     // ---- pre-processing: raw line -> FFT input (in place through registers)
@@ -625,7 +625,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T> &a, int64_t tile, 
         run_pass<T, WIDE>(R, ln, tws + twoff[p], a.m, Ns, tl, tpl);
         Ns *= R;
     }
-    stage_store<T, MG>(a, data, tws, tile, tid, l, tl);
+    stage_store<T, MG, false>(a, data, tws, tile, tid, l, tl);
 }
 
 template <typename T, bool WIDE, int MG>
@@ -707,7 +707,7 @@ constexpr int fast_mb()
 }
 
 // first pass (Ns = 1) from global memory into LDS
-template <typename T, int R, int EM, int M>
+template <typename T, int R, int EM, int M, bool NT>
 __device__ __forceinline__ void fast_first(const StageArgs<T> &a, cpx<T> *ln, bool valid, int64_t io, int tl, int tpl,
                                            bool cplx, bool cj)
 {
@@ -722,7 +722,7 @@ __device__ __forceinline__ void fast_first(const StageArgs<T> &a, cpx<T> *ln, bo
                 const int j = tl + tpl * b;
                 if (j < NB) {
 #pragma unroll
-                    for (int r = 0; r < R; ++r) v[b][r] = s[(int64_t)(j + r * NB) * a.ips];
+                    for (int r = 0; r < R; ++r) v[b][r] = gld<NT>(s + (int64_t)(j + r * NB) * a.ips);
                 }
             }
         } else if (a.ips == 1) {
@@ -732,7 +732,7 @@ __device__ __forceinline__ void fast_first(const StageArgs<T> &a, cpx<T> *ln, bo
                 const int j = tl + tpl * b;
                 if (j < NB) {
 #pragma unroll
-                    for (int r = 0; r < R; ++r) v[b][r] = s[j + r * NB];
+                    for (int r = 0; r < R; ++r) v[b][r] = gld<NT>(s + j + r * NB);
                 }
             }
         } else {
@@ -744,7 +744,7 @@ __device__ __forceinline__ void fast_first(const StageArgs<T> &a, cpx<T> *ln, bo
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const int64_t q = 2 * (int64_t)(j + r * NB);
-                        v[b][r] = mk(s[q * a.ips], s[(q + 1) * a.ips]);
+                        v[b][r] = mk(gld<NT>(s + q * a.ips), gld<NT>(s + (q + 1) * a.ips));
                     }
                 }
             }
@@ -804,7 +804,7 @@ __device__ __forceinline__ void fast_mid(cpx<T> *ln, const cpx<T> *twp, int Ns, 
 // last pass (Ns = M / R) from LDS straight to global memory: complex output
 // (C2C, conjugation and scale applied) or the real pairs of a C2R line
 // (y_2k = Re conj Z_k, y_2k+1 = Im conj Z_k).
-template <typename T, int R, int EM, int M>
+template <typename T, int R, int EM, int M, bool NT>
 __device__ __forceinline__ void fast_last(const StageArgs<T> &a, const cpx<T> *ln, const cpx<T> *twp, bool valid,
                                           int64_t oo, int tl, int tpl, bool c2r)
 {
@@ -837,7 +837,7 @@ __device__ __forceinline__ void fast_last(const StageArgs<T> &a, const cpx<T> *l
                 for (int r = 0; r < R; ++r) {
                     cpx<T> w = v[b][r];
                     if (cj) w.y = -w.y;
-                    d[(int64_t)(j + r * NB) * a.ops] = scal(w, sc);
+                    gst<NT>(d + (int64_t)(j + r * NB) * a.ops, scal(w, sc));
                 }
             }
         }
@@ -848,7 +848,7 @@ __device__ __forceinline__ void fast_last(const StageArgs<T> &a, const cpx<T> *l
             const int j = tl + tpl * b;
             if (j < NB) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) d[j + r * NB] = mk(v[b][r].x * sc, -v[b][r].y * sc);
+                for (int r = 0; r < R; ++r) gst<NT>(d + j + r * NB, mk(v[b][r].x * sc, -v[b][r].y * sc));
             }
         }
     } else {
@@ -860,8 +860,8 @@ __device__ __forceinline__ void fast_last(const StageArgs<T> &a, const cpx<T> *l
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int64_t q = 2 * (int64_t)(j + r * NB);
-                    d[q * a.ops] = v[b][r].x * sc;
-                    d[(q + 1) * a.ops] = -v[b][r].y * sc;
+                    gst<NT>(d + q * a.ops, v[b][r].x * sc);
+                    gst<NT>(d + (q + 1) * a.ops, -v[b][r].y * sc);
                 }
             }
         }
@@ -896,7 +896,7 @@ __device__ __forceinline__ void fast_c2r_pre(const StageArgs<T> &a, cpx<T> *ln, 
 }
 
 // R2C split + store of X_0..X_M from the half-length transform in LDS.
-template <typename T>
+template <typename T, bool NT>
 __device__ __forceinline__ void fast_r2c_store(const StageArgs<T> &a, const cpx<T> *data, const cpx<T> *tws,
                                                int64_t tile, int tid, int l, int tl)
 {
@@ -933,7 +933,7 @@ __device__ __forceinline__ void fast_r2c_store(const StageArgs<T> &a, const cpx<
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int k = base + u * tpl;
-            if (k < a.nout) d[(int64_t)k * a.ops] = v[u];
+            if (k < a.nout) gst<NT>(d + (int64_t)k * a.ops, v[u]);
         }
     }
 }
@@ -941,8 +941,8 @@ __device__ __forceinline__ void fast_r2c_store(const StageArgs<T> &a, const cpx<
 // One tile of a fast stage: tws holds the staged tables, data the tile's
 // lines. Threads beyond the stage's lt x tpl shape (a batched launch sized
 // for its largest item) take indexes past every line and only join the
-// barriers.
-template <typename T, int R0, int R1, int R2, int EM>
+// barriers. NT: non-temporal global loads and stores.
+template <typename T, int R0, int R1, int R2, int EM, bool NT>
 __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, const cpx<T> *tws, cpx<T> *data)
 {
     constexpr int M = R0 * R1 * (R2 ? R2 : 1);
@@ -955,7 +955,7 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
     cpx<T> *ln = data + l * a.ls;
     const int mode = a.mode;
     if (mode == M_C2R) {
-        stage_load(a, data, tile, tid, l, tl);
+        stage_load<T, NT>(a, data, tile, tid, l, tl);
         fast_c2r_pre<T, EM, M>(a, ln, tws, tl, tpl);
         fast_mid<T, R0, EM, M>(ln, tws, 1, tl, tpl);
     } else {
@@ -966,13 +966,13 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
         int64_t io = 0, oo = 0;
         const bool valid = act && line < a.nlines;
         if (valid) line_offsets(a.g, line, io, oo);
-        fast_first<T, R0, EM, M>(a, data + fl * a.ls, valid, io, ft, tpl, mode == M_C2C,
-                                 (a.flags & F_CONJ_IN) != 0);
+        fast_first<T, R0, EM, M, NT>(a, data + fl * a.ls, valid, io, ft, tpl, mode == M_C2C,
+                                     (a.flags & F_CONJ_IN) != 0);
     }
     if constexpr (R2 != 0) fast_mid<T, R1, EM, M>(ln, tws + a.twoff[1], R0, tl, tpl);
     if (mode == M_R2C) {
         fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
-        if (act) fast_r2c_store(a, data, tws, tile, tid, l, tl);
+        if (act) fast_r2c_store<T, NT>(a, data, tws, tile, tid, l, tl);
         return;
     }
     const int fl = !act ? 0 : (a.out_col ? tid % lt : l);
@@ -981,10 +981,10 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
     int64_t io = 0, oo = 0;
     const bool valid = act && line < a.nlines;
     if (valid) line_offsets(a.g, line, io, oo);
-    fast_last<T, RL, EM, M>(a, data + fl * a.ls, tws + a.twoff[R2 ? 2 : 1], valid, oo, ft, tpl, mode == M_C2R);
+    fast_last<T, RL, EM, M, NT>(a, data + fl * a.ls, tws + a.twoff[R2 ? 2 : 1], valid, oo, ft, tpl, mode == M_C2R);
 }
 
-template <typename T, int R0, int R1, int R2, int EM>
+template <typename T, int R0, int R1, int R2, int EM, bool NT>
 __global__ __launch_bounds__(512) void k_line_fast(const StageArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -998,7 +998,7 @@ __global__ __launch_bounds__(512) void k_line_fast(const StageArgs<T> a)
         if (tile >= a.ntiles) return;
     }
     for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
-    fast_body<T, R0, R1, R2, EM>(a, tile, tws, tws + a.twn);
+    fast_body<T, R0, R1, R2, EM, NT>(a, tile, tws, tws + a.twn);
 }
 
 // Persistent batched fast stage: the work list of k_line_batch, every item
@@ -1028,7 +1028,7 @@ __global__ __launch_bounds__(512) void k_line_fast_batch(const StageArgs<T> *__r
         const int64_t tile = w - prefix[q];
         __syncthreads(); // the previous tile is done with LDS
         for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
-        fast_body<T, R0, R1, R2, EM>(a, tile, tws, tws + a.twn);
+        fast_body<T, R0, R1, R2, EM, false>(a, tile, tws, tws + a.twn);
     }
 }
 
@@ -1066,7 +1066,7 @@ __global__ __launch_bounds__(512) void k_line_multi_batch(const StageArgs<T> *__
 #define PFH_MULTI_CASE(TT, A, B, C, E)                                                                        \
     if constexpr (std::is_same<T, TT>::value && E != 32) {                                                   \
         if (r0 == A && r1 == B && r2 == C && em == E) {                                                      \
-            fast_body<T, A, B, C, E>(a, tile, tws, tws + a.twn);                                            \
+            fast_body<T, A, B, C, E, false>(a, tile, tws, tws + a.twn);                                     \
             continue;                                                                                        \
         }                                                                                                    \
     }
@@ -1082,7 +1082,12 @@ hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int e
     if (grid <= 0) return hipSuccess;
 #define PFH_FAST_CASE(TT, A, B, C, E)                                                                           \
     if constexpr (std::is_same<T, TT>::value) if (r0 == A && r1 == B && r2 == C && em == E) {                               \
-        hipLaunchKernelGGL((k_line_fast<T, A, B, C, E>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, a); \
+        if (a.flags & F_NT)                                                                                     \
+            hipLaunchKernelGGL((k_line_fast<T, A, B, C, E, true>), dim3((unsigned)grid), dim3(threads), lds_bytes, \
+                               s, a);                                                                           \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_line_fast<T, A, B, C, E, false>), dim3((unsigned)grid), dim3(threads),         \
+                               lds_bytes, s, a);                                                                \
         return hipGetLastError();                                                                               \
     }
     PFH_FAST_PLANS(PFH_FAST_CASE)
@@ -1119,7 +1124,9 @@ hipError_t set_fast_lds_limit(size_t bytes)
 {
 #define PFH_FAST_ATTR(TT, A, B, C, E)                                                                    \
     {                                                                                                     \
-        hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_line_fast<TT, A, B, C, E>), bytes); \
+        hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_line_fast<TT, A, B, C, E, false>), bytes); \
+        if (e == hipSuccess)                                                                              \
+            e = raise_lds_limit(reinterpret_cast<const void *>(&k_line_fast<TT, A, B, C, E, true>), bytes); \
         if (e == hipSuccess)                                                                              \
             e = raise_lds_limit(reinterpret_cast<const void *>(&k_line_fast_batch<TT, A, B, C, E>), bytes); \
         if (e != hipSuccess) return e;                                                                    \

@@ -61,6 +61,7 @@ enum StageFlags : int {
     F_XCD_REMAP = 512,  // map consecutive tiles onto one XCD
     F_DIRECT_FIRST = 1024, // first radix pass reads global memory into registers
     F_DIRECT_LAST = 2048,  // last radix pass writes global memory from registers (C2C)
+    F_NT = 4096,           // stage arrays exceed the last-level cache: non-temporal loads and stores
 };
 
 struct LineGeom {
