@@ -181,7 +181,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (this build has no "
-              "multi-GPU path; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "multi-GPU path: each route transition would become an exchange within a process-grid row "
+              "or column over RCCL)", file=sys.stderr)
         sys.exit(2)
 
     import torch
