@@ -332,6 +332,14 @@ pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *co
 pfh_status pfh_pointwise_scale_axes(pfh_plan plan, int side, void *data, size_t elems, const void *factors,
                                     pfh_stream_t stream);
 
+/* The same with complex per-axis factors ((re, im) pairs of the plan's real
+ * type, concatenated in logical axis order) on the plan's complex side: the
+ * spectral derivative d/dx_a is the factor i k_a on axis a and 1 elsewhere
+ * (LocalGrid wavenumbers, grid.rs:20-120, in a pointwise_many closure,
+ * pointwise.rs:421-490). Real data gives PFH_ERR_INVALID_ARGUMENT. */
+pfh_status pfh_pointwise_scale_axes_complex(pfh_plan plan, int side, void *data, size_t elems, const void *factors,
+                                            pfh_stream_t stream);
+
 /* u = w + a w^2 elementwise over n reals (pointwise2, pointwise.rs:76-88). */
 pfh_status pfh_pointwise_quadratic(int precision, const void *w, void *u, size_t n, double a, pfh_stream_t stream);
 

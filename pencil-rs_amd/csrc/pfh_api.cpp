@@ -2531,8 +2531,8 @@ pfh_status pfh_batch_execute(pfh_batch batch, int op, const void *const *srcs, v
 
 extern "C" {
 
-pfh_status pfh_pointwise_scale_axes(pfh_plan plan, int side, void *data, size_t elems, const void *factors,
-                                    pfh_stream_t stream)
+static pfh_status scale_axes_impl(pfh_plan plan, int side, void *data, size_t elems, const void *factors,
+                                  bool complex_factors, pfh_stream_t stream)
 {
     if (!plan || !data || !factors || (side != 0 && side != 1)) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
     const PlanImpl &P = plan->impl;
@@ -2548,6 +2548,9 @@ pfh_status pfh_pointwise_scale_axes(pfh_plan plan, int side, void *data, size_t 
     ScaleAxesArgs a{};
     a.nd = L.N;
     a.complex_data = L.elem == 2 * tb ? 1 : 0;
+    a.complex_factors = complex_factors ? 1 : 0;
+    if (complex_factors && !a.complex_data)
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "complex factors need the complex side of the plan");
     a.len = L.len();
     a.extra = L.extra;
     for (int p = 0; p < L.N; ++p) {
@@ -2560,6 +2563,18 @@ pfh_status pfh_pointwise_scale_axes(pfh_plan plan, int side, void *data, size_t 
     PFH_HIP_CHECK(P.precision == PFH_F32 ? launch_scale_axes<float>(a, data, factors, s)
                                          : launch_scale_axes<double>(a, data, factors, s));
     return ok();
+}
+
+pfh_status pfh_pointwise_scale_axes(pfh_plan plan, int side, void *data, size_t elems, const void *factors,
+                                    pfh_stream_t stream)
+{
+    return scale_axes_impl(plan, side, data, elems, factors, false, stream);
+}
+
+pfh_status pfh_pointwise_scale_axes_complex(pfh_plan plan, int side, void *data, size_t elems, const void *factors,
+                                            pfh_stream_t stream)
+{
+    return scale_axes_impl(plan, side, data, elems, factors, true, stream);
 }
 
 pfh_status pfh_pointwise_quadratic(int precision, const void *w, void *u, size_t n, double a, pfh_stream_t stream)

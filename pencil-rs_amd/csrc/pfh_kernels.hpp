@@ -90,6 +90,7 @@ hipError_t launch_permute_words(const PermuteArgs &p, const void *src, void *dst
 struct ScaleAxesArgs {
     int nd;              // spatial positions
     int complex_data;    // interleaved complex values (both parts scaled)
+    int complex_factors; // factors are complex (re, im) pairs; data must be complex
     int64_t len, extra;  // elements per extra block, extra blocks
     int64_t mext[8];     // extent of each memory position (slowest first)
     int64_t foff[8];     // offset of that position's axis in the factor array

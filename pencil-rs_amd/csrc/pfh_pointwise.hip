@@ -25,24 +25,36 @@ unsigned blocks_for(int64_t n)
 
 // data[e][memory index] *= prod_a f_a[i_a]: the memory index is split into
 // per-position indices (slowest first), position p holding logical axis
-// perm[p]; f holds the per-axis factors concatenated in logical axis order.
+// perm[p]; f holds the per-axis factors concatenated in logical axis order
+// (complex factors: (re, im) pairs, e.g. i k_a for a spectral derivative).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scale_axes(ScaleAxesArgs s, T *data, const T *f)
 {
     const int64_t total = s.len * s.extra;
     for (int64_t i = global_id(); i < total; i += grid_stride()) {
         int64_t rem = i % s.len;
-        T w = (T)1;
+        T wr = (T)1, wi = (T)0;
         for (int p = s.nd - 1; p >= 0; --p) {
             const int64_t idx = rem % s.mext[p];
             rem /= s.mext[p];
-            w *= f[s.foff[p] + idx];
+            if (s.complex_factors) {
+                const T fr = f[2 * (s.foff[p] + idx)], fi = f[2 * (s.foff[p] + idx) + 1];
+                const T r = wr * fr - wi * fi;
+                wi = wr * fi + wi * fr;
+                wr = r;
+            } else {
+                wr *= f[s.foff[p] + idx];
+            }
         }
-        if (s.complex_data) {
-            data[2 * i] *= w;
-            data[2 * i + 1] *= w;
+        if (s.complex_factors) {
+            const T x = data[2 * i], y = data[2 * i + 1];
+            data[2 * i] = x * wr - y * wi;
+            data[2 * i + 1] = x * wi + y * wr;
+        } else if (s.complex_data) {
+            data[2 * i] *= wr;
+            data[2 * i + 1] *= wr;
         } else {
-            data[i] *= w;
+            data[i] *= wr;
         }
     }
 }

@@ -184,7 +184,7 @@ EXPORTED_SYMBOLS = (
     "pfh_plan_workspace_bytes", "pfh_plan_layout", "pfh_execute", "pfh_execute_in_place", "pfh_local_execute",
     "pfh_local_execute_in_place", "pfh_execute_with_timing", "pfh_workspace_take_status", "pfh_batch_create",
     "pfh_batch_destroy", "pfh_batch_workspace_bytes", "pfh_batch_execute", "pfh_batch_tuning",
-    "pfh_pointwise_scale_axes", "pfh_pointwise_quadratic", "pfh_pointwise_quadratic_adjoint", "pfh_sum_squares",
+    "pfh_pointwise_scale_axes", "pfh_pointwise_scale_axes_complex", "pfh_pointwise_quadratic", "pfh_pointwise_quadratic_adjoint", "pfh_sum_squares",
     "pfh_reduce_workspace_bytes", "pfh_reduce", "pfh_pointwise", "pfh_io_workspace_bytes", "pfh_io_write",
     "pfh_io_read", "pfh_io_read_raw", "pfh_io_catalog", "pfh_io_write_named", "pfh_io_append_named",
     "pfh_io_read_named",
@@ -224,6 +224,7 @@ def load_library(path: str = LIB_PATH):
     lib.pfh_batch_tuning.argtypes = [vp, ip, ip, ip, ip]
     d = ctypes.c_double
     lib.pfh_pointwise_scale_axes.argtypes = [vp, i, vp, sz, vp, vp]
+    lib.pfh_pointwise_scale_axes_complex.argtypes = [vp, i, vp, sz, vp, vp]
     lib.pfh_pointwise_quadratic.argtypes = [i, vp, vp, sz, d, vp]
     lib.pfh_pointwise_quadratic_adjoint.argtypes = [i, vp, vp, vp, sz, d, vp]
     lib.pfh_sum_squares.argtypes = [i, vp, sz, vp, sz, vp, vp]
@@ -792,9 +793,11 @@ class BatchC2cPlan:
 def scale_axes(plan, side: int, data: torch.Tensor, factors: torch.Tensor) -> torch.Tensor:
     """data *= prod_a factors_a[i_a] in place over the plan's input (side 0)
     or output (side 1) array; `factors` is the per-axis arrays concatenated
-    in logical axis order, in the plan's real dtype (LocalGrid coordinates,
-    crates/pencil-array/src/grid.rs:20-120)."""
-    _check(_lib.pfh_pointwise_scale_axes(plan._h, side, _ptr(data), data.numel(), _ptr(factors), _stream()))
+    in logical axis order (LocalGrid coordinates,
+    crates/pencil-array/src/grid.rs:20-120), in the plan's real dtype, or in
+    its complex dtype for complex factors (i k_a: a spectral derivative)."""
+    fn = _lib.pfh_pointwise_scale_axes_complex if factors.is_complex() else _lib.pfh_pointwise_scale_axes
+    _check(fn(plan._h, side, _ptr(data), data.numel(), _ptr(factors), _stream()))
     return data
 
 

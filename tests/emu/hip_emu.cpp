@@ -541,7 +541,11 @@ hipError_t launch_scale_axes(const ScaleAxesArgs &s, void *data, const void *fac
     check_read("scale_axes", data, (size_t)total * eb);
     int64_t nf = 0;
     for (int p = 0; p < s.nd; ++p) nf = std::max(nf, s.foff[p] + s.mext[p]);
-    check_read("scale_axes factors", factors, (size_t)nf * sizeof(T));
+    check_read("scale_axes factors", factors, (size_t)nf * sizeof(T) * (s.complex_factors ? 2 : 1));
+    if (s.complex_factors) {
+        if (!s.complex_data) fail("scale_axes: complex factors on real data");
+        return hipSuccess; // bounds only
+    }
     T *d = static_cast<T *>(data);
     const T *f = static_cast<const T *>(factors);
     for (int64_t i = 0; i < total; ++i) {

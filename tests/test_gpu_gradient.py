@@ -119,3 +119,39 @@ def test_pointwise_kernels(gpu):
     got = pf.scale_axes(plan, 1, spec.clone(), torch.cat(f))
     assert torch.allclose(got.reshape(ref.shape), ref, rtol=1e-14, atol=0)
 
+
+
+def test_spectral_derivative_with_complex_axis_factors(gpu):
+    """d/dx of u = sin(2 x) cos(3 y) (1 + cos z) on a 32 x 24 x 16 grid
+    (x = 2 pi i / n): R2C, multiply by i k_x on axis 0 (complex per-axis
+    factors; 0 at the Nyquist wavenumber, which keeps the derivative real),
+    normalized C2R, against 2 cos(2 x) cos(3 y) (1 + cos z); and the factor
+    product against torch's complex arithmetic on the spectrum. (With a field
+    whose kz = 0 plane is zero the product would leave only rounding noise in
+    that plane, with imaginary parts as large as its real parts, and the C2R
+    endpoint check would rightly reject it.)"""
+    import math
+
+    n0, n1, n2 = 32, 24, 16
+    for dt, tol in ((torch.float64, 1e-12), (torch.float32, 2e-6)):
+        plan = pf.R2cPlan((n0, n1, n2), dtype=dt)
+        x = torch.arange(n0, dtype=torch.float64) * 2 * math.pi / n0
+        y = torch.arange(n1, dtype=torch.float64) * 2 * math.pi / n1
+        z = torch.arange(n2, dtype=torch.float64) * 2 * math.pi / n2
+        X, Y, Z = torch.meshgrid(x, y, z, indexing="ij")
+        u = (torch.sin(2 * X) * torch.cos(3 * Y) * (1 + torch.cos(Z))).to(dt).cuda().reshape(-1)
+        du = (2 * torch.cos(2 * X) * torch.cos(3 * Y) * (1 + torch.cos(Z))).cuda().reshape(-1)
+        spec = plan.forward(u)
+        kx = [k if k < n0 // 2 else (0 if k == n0 // 2 else k - n0) for k in range(n0)]
+        cdt = torch.complex64 if dt == torch.float32 else torch.complex128
+        f = torch.cat([torch.tensor([1j * k for k in kx], dtype=torch.complex128),
+                       torch.ones(n1, dtype=torch.complex128), torch.ones(n2 // 2 + 1, dtype=torch.complex128)])
+        f = f.to(cdt).cuda()
+        ref = spec.reshape(n2 // 2 + 1, n1, n0) * f[:n0].reshape(1, 1, n0)
+        pf.scale_axes(plan, 1, spec, f)
+        assert torch.allclose(spec.reshape(ref.shape), ref, rtol=tol, atol=tol * float(ref.abs().max()))
+        got = plan.inverse(spec)
+        err = float((got.double() - du).abs().max())
+        assert err <= 10 * tol * 2, err
+    with pytest.raises(pf.PfhError):
+        pf.scale_axes(plan, 0, u, f)  # complex factors on the real side
