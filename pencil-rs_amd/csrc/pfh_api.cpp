@@ -1742,13 +1742,16 @@ static pfh_status execute_impl(PlanImpl &P, int op, const void *src, size_t src_
     if (is_tr && op != PFH_FORWARD && op != PFH_BACKWARD) return set_err(PFH_ERR_INVALID_ARGUMENT, "transpose op");
     const OpPlan &o = P.ops[op];
     if (!o.valid) return set_err(PFH_ERR_INVALID_ARGUMENT, "op not supported by this plan");
-    if (!src || !dst) return set_err(PFH_ERR_INVALID_ARGUMENT, "null buffer");
+    // an empty extra shape gives empty arrays: nothing to read or write
+    if ((!src && o.in.total() > 0) || (!dst && o.out.total() > 0))
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "null buffer");
     if ((int64_t)src_elems != o.in.total())
         return set_err(is_tr ? PFH_ERR_SOURCE_LAYOUT_MISMATCH : PFH_ERR_INPUT_LAYOUT_MISMATCH, "source length",
                        (size_t)o.in.total(), src_elems);
     if ((int64_t)dst_elems != o.out.total())
         return set_err(is_tr ? PFH_ERR_DESTINATION_LAYOUT_MISMATCH : PFH_ERR_OUTPUT_LAYOUT_MISMATCH, "destination length",
                        (size_t)o.out.total(), dst_elems);
+    if (o.in.total() == 0 && o.out.total() == 0) return ok();
     if (is_tr && o.steps.empty()) {
         // generic word permutation
         PermuteArgs pa{};
@@ -1964,9 +1967,10 @@ pfh_status pfh_execute_with_timing(pfh_plan plan, int op, const void *src, size_
 pfh_status pfh_execute_in_place(pfh_plan plan, int op, void *data, size_t capacity_bytes, void *workspace,
                                 size_t workspace_bytes, pfh_stream_t stream_, int flags)
 {
-    if (!plan || !data) return set_err(PFH_ERR_INVALID_ARGUMENT, "null argument");
+    if (!plan) return set_err(PFH_ERR_INVALID_ARGUMENT, "null argument");
     if (op < 0 || op > 5) return set_err(PFH_ERR_INVALID_ARGUMENT, "op");
     PlanImpl &P = plan->impl;
+    if (!data && P.extra != 0) return set_err(PFH_ERR_INVALID_ARGUMENT, "null argument");
     hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
     if (P.kind == PFH_LOCAL_TRANSPOSE || P.kind == PFH_ALLTOALLV_TRANSPOSE) {
         // stage the source into the workspace, then permute back (local_transpose.rs:159-243)
@@ -1987,6 +1991,7 @@ pfh_status pfh_execute_in_place(pfh_plan plan, int op, void *data, size_t capaci
     const int64_t need = std::max(o.in.bytes(), o.out.bytes());
     if ((int64_t)capacity_bytes < need)
         return set_err(PFH_ERR_WORKSPACE_TOO_SMALL, "in-place storage", (size_t)need, capacity_bytes);
+    if (need == 0) return ok();
     WsView wv;
     pfh_status st = view_workspace(P, o, workspace, workspace_bytes, wv);
     if (st) return st;

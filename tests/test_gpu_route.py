@@ -415,3 +415,45 @@ def test_in_place_matches_out_of_place(gpu, dt):
     d = _tdev(xc.reshape(-1))
     cplan.execute_in_place(O.FORWARD, d)
     assert np.array_equal(_host(d), refc)
+
+
+def test_empty_extra_shape_is_a_no_op(gpu):
+    """ExtraShape [0] (crates/pencil-fft/tests/distributed_c2c.rs:150-156,
+    distributed/r2c tests :873-894, distributed_mixed.rs:869-893): plans build,
+    and every operation on the empty arrays succeeds without device work."""
+    e = torch.empty(0, device="cuda")
+    ec = torch.empty(0, dtype=torch.complex64, device="cuda")
+    plans = [
+        (pf.C2cPlan((3, 1, 4), extra_shape=(0,)), ec, ec),
+        (pf.R2cPlan((2, 3, 2), dtype=torch.float32, extra_shape=(0,)), e, ec),
+        (pf.C2rPlan((3, 2, 4), dtype=torch.float32, extra_shape=(0,)), ec, e),
+        (pf.R2rPlan((3, 4, 5), [pf.R2rKind.DctII, pf.R2rKind.DstI, None], dtype=torch.float32, extra_shape=(0,)),
+         e, e),
+        (pf.MixedR2cPlan((2, 3), [pf.AxisTransform.FFT, pf.AxisTransform.RFFT], dtype=torch.float32,
+                         extra_shape=(0,)), e, ec),
+    ]
+    for plan, src, out in plans:
+        assert plan.input_elements() == 0 and plan.output_elements() == 0
+        y = plan.forward(src)
+        assert y.numel() == 0
+        back = plan.inverse(y)
+        assert back.numel() == 0
+        if not isinstance(plan, pf.C2rPlan):
+            plan.execute_in_place(pf.FORWARD, torch.empty(0, dtype=out.dtype, device="cuda"))
+    tp = pf.LocalTransposePlan((2, 3, 4), (0, 1, 2), (2, 1, 0), element_bytes=8, extra_shape=(0,))
+    tp.execute(torch.empty(0, dtype=torch.float64, device="cuda"), torch.empty(0, dtype=torch.float64, device="cuda"))
+    torch.cuda.synchronize()
+
+
+def test_unit_extents_and_ragged_shapes(gpu):
+    """Extent-1 axes and odd lengths on every stage (tests/distributed_c2c.rs:130-166
+    use [3, 1, 4] and [3, 2, 3, 4]): a 4D C2C f64 plan against the oracle."""
+    rng = np.random.default_rng(21)
+    for shape, extra in (((3, 1, 4), (2,)), ((3, 2, 3, 4), ()), ((5, 1, 1, 7), (3,))):
+        x = uniform_complex(rng, tuple(extra) + shape, np.complex128).reshape(-1)
+        plan = pf.C2cPlan(shape, dtype=torch.complex128, extra_shape=extra)
+        got = _host(plan.forward(_tdev(x)))
+        ref = O.transform(O.KIND_C2C, x, O.FORWARD, shape=shape, extra=int(np.prod(extra, dtype=np.int64)))
+        assert rel_l2(got, ref) <= 1e-12
+        back = _host(plan.inverse(_tdev(got)))
+        assert rel_l2(back, x) <= 1e-12
