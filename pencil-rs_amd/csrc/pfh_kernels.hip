@@ -868,29 +868,78 @@ __device__ __forceinline__ void fast_last(const StageArgs<T> &a, const cpx<T> *l
     }
 }
 
-// C2R merge: half spectrum X_0..X_m in LDS -> conj of z_j = E_j + i O_j in
-// LDS (z_j = x_2j + i x_2j+1 after the positive-sign transform).
-template <typename T, int EM, int M>
-__device__ __forceinline__ void fast_c2r_pre(const StageArgs<T> &a, cpx<T> *ln, const cpx<T> *tws, int tl, int tpl)
+// C2R load + merge in one step. The half spectrum X_0..X_M of a 2M-point
+// real line becomes the M-point complex input conj(z_j), z_j = E_j + i O_j
+// with E_j = A + B, O_j = w_j (A - B), A = X_j, B = conj(X_{M-j}) (z_j =
+// x_2j + i x_2j+1 after the positive-sign transform). Each thread loads the
+// pair X_q, X_{M-q} (q = 0 .. M/2; every input point is loaded once, the
+// tile's lanes still reading 256-byte runs) and writes conj(z_q) and
+// conj(z_{M-q}) straight to LDS, so the raw half spectrum never goes
+// through LDS.
+template <typename T, int M, bool NT>
+__device__ __forceinline__ void fast_c2r_load(const StageArgs<T> &a, cpx<T> *data, const cpx<T> *tws, int64_t tile,
+                                              int tid, int l, int tl)
 {
-    constexpr int E = (M + 1 < EM) ? M + 1 : EM;
-    cpx<T> z[E];
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-        const int j = tl + tpl * i;
-        if (j < M) {
-            cpx<T> A = hin(a, ln, j, M);
-            cpx<T> B = conjg(hin(a, ln, M - j, M));
-            cpx<T> Ev = A + B, D = A - B;
-            cpx<T> O = cmul(tws[a.pre0 + j], D);
-            z[i] = conjg(Ev + mul_pi(O));
+    const int lt = a.lt, tpl = a.tpl;
+    if (tid < lt * tpl) {
+        int ll, p0;
+        if (a.in_col) {
+            ll = tid % lt;
+            p0 = tid / lt;
+        } else {
+            ll = l;
+            p0 = tl;
         }
-    }
-    __syncthreads();
+        const int64_t line = tile * lt + ll;
+        if (line < a.nlines) {
+            int64_t io, oo;
+            line_offsets(a.g, line, io, oo);
+            cpx<T> *lln = data + ll * a.ls;
+            const cpx<T> *s = reinterpret_cast<const cpx<T> *>(a.src) + io;
+            const bool cj = (a.flags & F_CONJ_IN) != 0;
+            const bool half = (a.flags & F_HERM_HALF) != 0;
+            constexpr int U = Regs<T>::unroll / 2;
+            constexpr int Q = M / 2; // last pair index (self-paired)
+            for (int base = p0; base <= Q; base += tpl * U) {
+                cpx<T> x[U], y[U];
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-        const int j = tl + tpl * i;
-        if (j < M) Lset(ln, j, z[i]);
+                for (int u = 0; u < U; ++u) {
+                    const int q = base + u * tpl;
+                    if (q <= Q) {
+                        x[u] = gld<NT>(s + (int64_t)q * a.ips);
+                        y[u] = gld<NT>(s + (int64_t)(M - q) * a.ips);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int q = base + u * tpl;
+                    if (q <= Q) {
+                        cpx<T> xq = x[u], yq = y[u];
+                        if (cj) {
+                            xq.y = -xq.y;
+                            yq.y = -yq.y;
+                        }
+                        if (q == 0) { // X_0 and X_M: imaginary parts dropped
+                            xq.y = (T)0;
+                            yq.y = (T)0;
+                        } else if (half) { // adjoint weights on interior bins
+                            xq = scal(xq, (T)0.5);
+                            yq = scal(yq, (T)0.5);
+                        }
+                        {
+                            const cpx<T> A = xq, B = conjg(yq);
+                            const cpx<T> E = A + B, D = A - B;
+                            Lset(lln, q, conjg(E + mul_pi(cmul(tws[a.pre0 + q], D))));
+                        }
+                        if (q > 0 && q < Q) {
+                            const cpx<T> A = yq, B = conjg(xq);
+                            const cpx<T> E = A + B, D = A - B;
+                            Lset(lln, M - q, conjg(E + mul_pi(cmul(tws[a.pre0 + M - q], D))));
+                        }
+                    }
+                }
+            }
+        }
     }
     __syncthreads();
 }
@@ -955,8 +1004,7 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
     cpx<T> *ln = data + l * a.ls;
     const int mode = a.mode;
     if (mode == M_C2R) {
-        stage_load<T, NT>(a, data, tile, tid, l, tl);
-        fast_c2r_pre<T, EM, M>(a, ln, tws, tl, tpl);
+        fast_c2r_load<T, M, NT>(a, data, tws, tile, tid, l, tl);
         fast_mid<T, R0, EM, M>(ln, tws, 1, tl, tpl);
     } else {
         // column access: lanes across the tile's lines; row access: along a line
