@@ -167,6 +167,36 @@ def adjoint_vs_plain(torch, pf, plan, x, spec, back, ws, reps: int = 5):
     return {"plain_ms": tp * 1e3, "adjoint_ms": ta * 1e3, "adjoint_over_plain": ta / tp}
 
 
+def f64_round_trips(torch, pf, n: int, steps: int = 20):
+    """The same round trip in f64 (the reference's other precision at this
+    size): rate, route-model bytes and roofline fraction, accuracy."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2)
+    x = torch.randn(n ** 3, generator=g, device="cuda", dtype=torch.float64)
+    plan = pf.R2cPlan((n, n, n), dtype=torch.float64)
+    spec = torch.empty(plan.output_elements(), dtype=torch.complex128, device="cuda")
+    back = torch.empty_like(x)
+    ws = plan.allocate_workspace()
+
+    def step():
+        plan.forward(x, spec, ws)
+        plan.inverse(spec, back, ws, flags=pf.EXEC_DEFER_STATUS)
+
+    step()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / steps
+    plan.take_status(ws)
+    rm = route_model_bytes(n, 8)
+    err = float(torch.linalg.norm(back - x) / torch.linalg.norm(x))
+    del x, spec, back, ws
+    return {"round_trips_per_s": 1.0 / dt, "ms_per_round_trip": dt * 1e3, "route_model_bytes": rm,
+            "route_model_frac_of_peak": rm / dt / 1e9 / PEAK_GBPS, "round_trip_rel_l2": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,6 +322,7 @@ def main():
         out["adjoint"] = adjoint_vs_plain(torch, pf, plan, x, spec, back, ws)
         plan.take_status(ws)
         out["batched"] = batched_vs_plain(torch, pf)
+        out["f64"] = f64_round_trips(torch, pf, n)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(n, threads)
