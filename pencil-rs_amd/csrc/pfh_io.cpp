@@ -246,7 +246,8 @@ pfh_status array_info(const pfh_array *a, ArrayInfo &I)
 {
     if (!a) return err(PFH_ERR_IO_INVALID_INPUT, "array");
     const pfh_pencil_layout &p = a->pencil;
-    if (p.ndim < 1 || p.ndim > PFH_MAX_DIMS || a->extra_ndim < 0 || a->extra_ndim > PFH_MAX_DIMS)
+    if (p.ndim < 1 || p.ndim > PFH_MAX_DIMS || a->extra_ndim < 0 || a->extra_ndim > PFH_MAX_DIMS ||
+        p.topology_dims < 0 || p.topology_dims > p.ndim)
         return err(PFH_ERR_IO_INVALID_INPUT, "rank");
     I.width = type_width((uint64_t)a->scalar_type);
     if (I.width == 0) return err(PFH_ERR_IO_INVALID_INPUT, "scalar type");
