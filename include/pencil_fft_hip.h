@@ -312,10 +312,12 @@ pfh_status pfh_batch_workspace_bytes(pfh_batch batch, size_t *bytes);
 pfh_status pfh_batch_execute(pfh_batch batch, int op, const void *const *srcs, void *const *dsts, void *workspace,
                              size_t workspace_bytes, pfh_stream_t stream, int use_graph);
 /* The tuning the batch runs with (threads per block, largest radix, lines
- * per tile on column-access stages, and whether the items of all
- * compile-time radix plans of a stage share one launch), as chosen by the
- * autotuner. */
-pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines, int *one_launch);
+ * per tile on column-access stages, whether the items of all compile-time
+ * radix plans of a stage share one launch, and whether a launch keeps one
+ * resident set of workgroups looping over the tiles or takes one workgroup
+ * per tile), as chosen by the autotuner. */
+pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines, int *one_launch,
+                            int *persistent);
 
 /* ---------------------------------------------------------------------- */
 /* Pointwise operations between transforms (pseudo-spectral steps).        */

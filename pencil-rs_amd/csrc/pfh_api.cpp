@@ -2307,7 +2307,9 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
                                : own.fast ? line_fast_batch_blocks_per_cu<T>(own.fr[0], own.fr[1], own.fr[2],
                                                                              own.fem, own.threads, own.lds)
                                           : line_batch_blocks_per_cu<T>(own.wide, own.threads, own.lds);
-            own.grid = std::min<int64_t>(own.total, (int64_t)per_cu * cus);
+            static const int persistent_env = env_int("PFH_BATCH_PERSISTENT", -1);
+            const bool persistent = persistent_env < 0 ? B.tuning.batch_persistent : persistent_env != 0;
+            own.grid = persistent ? std::min<int64_t>(own.total, (int64_t)per_cu * cus) : own.total;
         }
     }
     (void)stream;
@@ -2390,9 +2392,11 @@ static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int coun
     for (int block : {128, 256, 512})
         for (int maxr : {8, 16})
             for (int col : {0, 32})
-                for (bool multi : {true, false}) {
+                for (bool multi : {true, false})
+                    for (bool pers : {true, false}) {
                 Tuning t;
                 t.batch_multi = multi;
+                t.batch_persistent = pers;
                 t.block = block;
                 t.max_radix = maxr;
                 t.col_lines = col;
@@ -2502,13 +2506,15 @@ pfh_status pfh_batch_workspace_bytes(pfh_batch batch, size_t *bytes)
     return ok();
 }
 
-pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines, int *one_launch)
+pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines, int *one_launch,
+                            int *persistent)
 {
     if (!batch) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
     if (block) *block = batch->tuning.block;
     if (max_radix) *max_radix = batch->tuning.max_radix;
     if (col_lines) *col_lines = batch->tuning.col_lines;
     if (one_launch) *one_launch = batch->tuning.batch_multi ? 1 : 0;
+    if (persistent) *persistent = batch->tuning.batch_persistent ? 1 : 0;
     return ok();
 }
 

@@ -134,6 +134,8 @@ struct Tuning {
     int slab_mb = -1;    // slab intermediate target in MiB (0: no slab pipelining, -1: default)
     bool fill_block = false; // column-access tiles also take `block` threads (batched launches)
     bool batch_multi = true; // batched stages: one launch over every compile-time plan
+    bool batch_persistent = true; // batched launches: persistent workgroups (resident count) looping
+                                  // over the tiles, else one workgroup per tile
     int batch_group_mb = 0;  // batched problems run group by group (all stages of a group before
                              // the next; groups share one workspace region); 0: one group
 };

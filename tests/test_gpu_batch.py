@@ -50,8 +50,9 @@ def test_small_batch_matches_oracle(dt, autotune):
     torch_dt = torch.complex64 if dt == np.complex64 else torch.complex128
     plan = pf.BatchC2cPlan(shapes, dtype=torch_dt, autotune=autotune, extra_shapes=[(3,)] * len(shapes))
     if autotune:
-        block, maxr, col, one = plan.tuning()
+        block, maxr, col, one, persistent = plan.tuning()
         assert block in (128, 256, 512) and maxr in (8, 16) and col in (0, 32) and one in (True, False)
+        assert persistent in (True, False)
     srcs = [_tdev(x) for x in xs]
     for op in (O.FORWARD, O.INVERSE, O.BACKWARD):
         for use_graph in (False, True):
