@@ -1160,10 +1160,14 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
             ep.outer = ep.len / (ep.clen * ep.stride);
             ep.planes = (real_len % 2 == 0) ? 2 : 1;
             {
+                // the plane elements are one per cache line when the reduction
+                // axis is the fastest (the C2R input): many small chunks keep
+                // enough loads in flight
+                // (the partial kernel holds two elements per thread: chunks
+                // of at most 512 elements for its 256 threads)
                 const int64_t count = ep.outer * ep.stride;
-                int64_t nb = (count + 4095) / 4096;
+                int64_t nb = (count + 511) / 512;
                 if (nb < 1) nb = 1;
-                if (nb > 256) nb = 256;
                 ep.nblk = (int)nb;
                 P.partials = std::max<int64_t>(P.partials, ep.extra * ep.planes * nb);
             }

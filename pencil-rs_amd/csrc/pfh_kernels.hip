@@ -1238,14 +1238,30 @@ __global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const 
     const int64_t chunk = (count + gridDim.x - 1) / gridDim.x;
     const int64_t begin = (int64_t)blockIdx.x * chunk;
     const int64_t end = begin + chunk < count ? begin + chunk : count;
+    // a chunk is at most two elements per thread (the planner sizes the grid
+    // for it): both stay in registers for the two passes (maximum, then sums
+    // scaled by it)
+    constexpr int K = 2;
+    cpx<T> v[K];
+    bool have[K];
     double ma = 0.0, mi = 0.0;
     int bad = 0;
-    for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-        const int64_t o = i / e.stride, q = i - o * e.stride;
-        const cpx<T> v = base[o * e.clen * e.stride + q];
-        const double re = (double)v.x, im = (double)v.y;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int64_t i = begin + threadIdx.x + (int64_t)k * blockDim.x;
+        have[k] = i < end;
+        if (have[k]) {
+            const int64_t o = e.stride == 1 ? i : i / e.stride, q = i - o * e.stride;
+            v[k] = base[o * e.clen * e.stride + q];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (!have[k]) continue;
+        const double re = (double)v[k].x, im = (double)v[k].y;
         if (!isfinite(re) || !isfinite(im)) {
             bad = 1;
+            have[k] = false;
             continue;
         }
         ma = fmax(ma, fmax(fabs(re), fabs(im)));
@@ -1256,12 +1272,10 @@ __global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const 
     const double anybad = block_reduce_max((double)bad, sh);
     double sr = 0.0, si = 0.0;
     if (ma != 0.0) {
-        for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-            const int64_t o = i / e.stride, q = i - o * e.stride;
-            const cpx<T> v = base[o * e.clen * e.stride + q];
-            const double re = (double)v.x, im = (double)v.y;
-            if (!isfinite(re) || !isfinite(im)) continue;
-            const double nr = re / ma, ni = im / ma;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (!have[k]) continue;
+            const double nr = (double)v[k].x / ma, ni = (double)v[k].y / ma;
             sr += nr * nr;
             si += ni * ni;
         }

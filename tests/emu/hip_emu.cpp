@@ -469,6 +469,8 @@ hipError_t launch_endpoint_check(const EndpointArgs &e, const void *data, Endpoi
     ++g_launches;
     begin_launch();
     const int64_t count = e.outer * e.stride;
+    // the partial kernel covers at most 2 x 256 elements of a chunk
+    if (e.nblk < 1 || (count + e.nblk - 1) / e.nblk > 512) fail("endpoint chunk of more than 512 elements");
     for (int64_t b = 0; b < e.extra; ++b)
         for (int p = 0; p < e.planes; ++p) {
             const int64_t k = p == 0 ? 0 : e.clen - 1;
