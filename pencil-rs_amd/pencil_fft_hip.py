@@ -553,6 +553,32 @@ class _RoutePlan(_Plan):
     def adjoint_backward(self, src, dst=None, workspace=None):
         return self.execute(ADJOINT_BACKWARD, src, dst, workspace)
 
+    # In-place, timed and overlapped forms under the reference's method names
+    # (C2cPlan::{forward,inverse,backward}_{in_place,with_timing,with_overlap},
+    # distributed.rs:1507-1700). At one rank there is no exchange to overlap:
+    # the overlapped forms are the plain ones.
+    def forward_in_place(self, data, workspace=None, flags=0):
+        return self.execute_in_place(FORWARD, data, workspace, flags)
+
+    def inverse_in_place(self, data, workspace=None, flags=0):
+        return self.execute_in_place(INVERSE, data, workspace, flags)
+
+    def backward_in_place(self, data, workspace=None, flags=0):
+        return self.execute_in_place(BACKWARD, data, workspace, flags)
+
+    def forward_with_timing(self, src, dst=None, workspace=None):
+        return self.execute_with_timing(FORWARD, src, dst, workspace)
+
+    def inverse_with_timing(self, src, dst=None, workspace=None):
+        return self.execute_with_timing(INVERSE, src, dst, workspace)
+
+    def backward_with_timing(self, src, dst=None, workspace=None):
+        return self.execute_with_timing(BACKWARD, src, dst, workspace)
+
+    forward_with_overlap = forward
+    inverse_with_overlap = inverse
+    backward_with_overlap = backward
+
     def input_elements(self) -> int:
         return self.layout(0)[2]
 

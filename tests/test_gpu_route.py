@@ -457,3 +457,22 @@ def test_unit_extents_and_ragged_shapes(gpu):
         assert rel_l2(got, ref) <= 1e-12
         back = _host(plan.inverse(_tdev(got)))
         assert rel_l2(back, x) <= 1e-12
+
+
+def test_reference_method_names(gpu):
+    """forward / inverse _in_place, _with_timing and _with_overlap under the
+    reference's names give the plain operations' results."""
+    rng = np.random.default_rng(31)
+    shape = (6, 10, 8)
+    x = _tdev(uniform_complex(rng, shape, np.complex128).reshape(-1))
+    plan = pf.C2cPlan(shape, dtype=torch.complex128)
+    y = plan.forward(x)
+    assert torch.equal(plan.forward_with_overlap(x), y)
+    yt, timing = plan.forward_with_timing(x)
+    assert torch.equal(yt, y) and len(timing.entries()) == 3
+    d = x.clone()
+    plan.forward_in_place(d)
+    assert torch.equal(d, y)
+    plan.inverse_in_place(d)
+    assert rel_l2(_host(d), _host(x)) <= 1e-12
+    assert torch.equal(plan.inverse_with_overlap(y), plan.inverse(y))
