@@ -679,9 +679,22 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
     // cache (PFH_NT_MB overrides the threshold, 0 turns them off): measured
     // on MI355X, +5% on 512^3 f32 stages, but -25% on batches of ~20 MB
     // problems, whose tiles share cache lines; batched launches never use them.
+    // They also need whole cache lines per tile: rows of both arrays (the
+    // fastest memory axis) and column-tile runs (lines per tile x element)
+    // in multiples of 128 bytes; otherwise neighbouring tiles share lines and
+    // the hint evicts what the neighbour reads (permute_dims = false at
+    // 512^3, 257-element complex rows: 450 against 640 round trips/s).
     static const int nt_mb = env_int("PFH_NT_MB", 256);
-    if (nt_mb > 0 && st.in.bytes() + st.out.bytes() > ((int64_t)nt_mb << 20)) st.flags |= F_NT;
-    else st.flags &= ~F_NT;
+    auto rows_aligned = [](const Layout &L) {
+        return L.N > 0 && (L.shape[L.perm[L.N - 1]] * (int64_t)L.elem) % 128 == 0;
+    };
+    const bool runs_aligned = (!st.in_col || ((int64_t)st.lt * st.in.elem) % 128 == 0) &&
+                              (!st.out_col || ((int64_t)st.lt * st.out.elem) % 128 == 0);
+    if (nt_mb > 0 && st.in.bytes() + st.out.bytes() > ((int64_t)nt_mb << 20) && rows_aligned(st.in) &&
+        rows_aligned(st.out) && runs_aligned)
+        st.flags |= F_NT;
+    else
+        st.flags &= ~F_NT;
     return PFH_OK;
 }
 
