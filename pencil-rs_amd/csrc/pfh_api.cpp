@@ -643,7 +643,7 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
             need = std::max(need, ceil_div(st.m, emax));
             continue;
         }
-        const int MB = std::max(1, emax / R);
+        const int MB = st.fast ? std::max(1, emax / R) : generic_mb(R, wide_plan);
         need = std::max(need, ceil_div(st.m / R, MB));
     }
     if (mode_needs_pre(st.mode)) need = std::max(need, ceil_div(st.m, emax));
@@ -655,8 +655,19 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
     while (lt > 1 && (int64_t)(lt / 2) >= nlines) lt /= 2;
     while (lt * tpl < 64) tpl *= 2;
     const int cap = tu.block_cap > 0 ? std::min(tu.block_cap, max_threads) : max_threads;
-    while (lt * tpl > cap && lt > 1) lt /= 2;
-    if (lt * tpl > cap) return set_err(PFH_ERR_UNSUPPORTED, "line too long for one workgroup");
+    // A generic stage with a column side whose tile of lt lines x tpl
+    // threads exceeds the workgroup keeps the tile (runs of lt elements on
+    // the column side) and computes it in rounds of lc lines. With lc = lt
+    // instead, the 385-point DST-I lines of a 384-long axis (128 threads
+    // each) got 4-line tiles: 16-byte column runs, 3.5x slower on MI355X.
+    static const int rounds_env = env_int("PFH_ROUNDS", 1);
+    const bool rounds = rounds_env && !st.fast && (st.in_col || st.out_col) && (lt & (lt - 1)) == 0;
+    int lc = lt;
+    while (lc * tpl > cap && lc > 1) lc /= 2;
+    if (lc * tpl > cap) return set_err(PFH_ERR_UNSUPPORTED, "line too long for one workgroup");
+    if (!rounds) lt = lc;
+    // the column mapping spreads one point's lt lines over lc * tpl threads
+    while (lt > lc * tpl) lt /= 2;
     int need_c = std::max(st.m, 1);
     need_c = std::max(need_c, st.in_cplx ? st.nin : (st.nin + 1) / 2);
     need_c = std::max(need_c, st.out_cplx ? st.nout : (st.nout + 1) / 2);
@@ -666,12 +677,14 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
     auto lds_of = [&](int l) { return (size_t)(st.twn + (int64_t)l * st.ls) * (size_t)cb; };
     while (lds_of(lt) > dev.lds_limit && lt > 1) {
         lt /= 2;
-        while (lt * tpl < 64) tpl *= 2;
+        if (lc > lt) lc = lt;
+        while (lc * tpl < 64) tpl *= 2;
     }
     if (lds_of(lt) > dev.lds_limit) return set_err(PFH_ERR_UNSUPPORTED, "line does not fit in LDS");
     st.lt = lt;
+    st.lc = lc;
     st.tpl = tpl;
-    st.threads = lt * tpl;
+    st.threads = lc * tpl;
     st.lds_bytes = lds_of(lt);
     set_tiles(st, tu);
     // Non-temporal global accesses (the compile-time radix plans only) once
@@ -1562,6 +1575,7 @@ static StageArgs<T> fill_args(const PlanImpl &P, const Step &s, void *const bufs
     a.twn = s.twn;
     a.fem = s.fast ? s.fem : 0;
     a.lt = s.lt;
+    a.lc = s.lc;
     a.tpl = s.tpl;
     a.ls = s.ls;
     a.in_col = s.in_col;

@@ -325,19 +325,14 @@ __device__ __forceinline__ cpx<T> post_complex(const StageArgs<T> &a, const cpx<
     return scal(v, a.scale);
 }
 
-template <typename T>
-constexpr int pass_mb(int R)
-{
-    return (Regs<T>::emax / R) > 0 ? Regs<T>::emax / R : 1;
-}
 
 // One Stockham radix-R pass in LDS: butterflies j = tl + tpl * b read
 // x[j + r m/R], twiddle by exp(-2 pi i r (j mod Ns) / (Ns R)) and write
 // y[(j - j mod Ns) R + j mod Ns + r Ns].
-template <typename T, int R>
+template <typename T, int R, bool WIDE>
 __device__ __forceinline__ void radix_pass(cpx<T> *ln, const cpx<T> *twp, int m, int Ns, int tl, int tpl)
 {
-    constexpr int MB = pass_mb<T>(R);
+    constexpr int MB = generic_mb(R, WIDE);
     const int nb = m / R;
     cpx<T> v[MB][R];
 #pragma unroll
@@ -373,12 +368,11 @@ __device__ __forceinline__ void radix_pass(cpx<T> *ln, const cpx<T> *twp, int m,
 // Generic radix-R pass for any R (used for prime factors above 13): each
 // thread computes whole outputs as O(R) sums from LDS. roots = exp(-2 pi i
 // e / R), e < R, followed by the usual Ns (R - 1) pass twiddles.
-template <typename T>
+template <typename T, int E>
 __device__ __forceinline__ void radix_pass_generic(cpx<T> *ln, const cpx<T> *roots, int m, int R, int Ns, int tl, int tpl)
 {
     const cpx<T> *twp = roots + R;
     const int nb = m / R;
-    constexpr int E = Regs<T>::emax;
     cpx<T> v[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) {
@@ -413,25 +407,38 @@ __device__ __forceinline__ void radix_pass_generic(cpx<T> *ln, const cpx<T> *roo
 
 
 // ---- load: global -> LDS (raw line values, all loads of a chunk in flight)
-template <typename T, bool NT>
-__device__ __forceinline__ void stage_load(const StageArgs<T> &a, cpx<T> *data, int64_t tile, int tid, int l, int tl)
+// Thread mapping of a tile's load or store: lines ll + i * lstep, i < cnt,
+// points p0, p0 + step, ... A column side spreads the tile over all lc *
+// tpl threads, consecutive threads on consecutive lines of one point (runs
+// of lt elements); a row side goes line by line, tpl threads per line, lc
+// lines per round. Threads beyond lc * tpl get nothing.
+template <typename T>
+__device__ __forceinline__ void tile_mapping(const StageArgs<T> &a, int col, int tid, int l, int tl, int &ll, int &p0,
+                                             int &step, int &lstep, int &cnt)
 {
-    const int lt = a.lt, tpl = a.tpl;
-    if (tid >= lt * tpl) { // threads beyond this stage's tile shape only join the barrier
-        __syncthreads();
-        return;
-    }
-    const bool cin = complex_input(a.mode);
-    int ll, p0;
-    if (a.in_col) {
+    const int lt = a.lt, lc = a.lc, tpl = a.tpl;
+    if (col) {
+        const int rows = lc * tpl / lt;
         ll = tid % lt;
         p0 = tid / lt;
+        step = rows;
+        lstep = 0;
+        cnt = tid < rows * lt ? 1 : 0;
     } else {
         ll = l;
         p0 = tl;
+        step = tpl;
+        lstep = lc;
+        cnt = tid < lc * tpl ? lt / lc : 0;
     }
-    const int step = tpl;
-    const int64_t line = tile * lt + ll;
+}
+
+// Line ll of the tile, points p0, p0 + step, ... : global -> LDS.
+template <typename T, bool NT>
+__device__ __forceinline__ void load_line(const StageArgs<T> &a, cpx<T> *data, int64_t tile, int ll, int p0, int step)
+{
+    const bool cin = complex_input(a.mode);
+    const int64_t line = tile * a.lt + ll;
     if (line < a.nlines) {
         int64_t io, oo;
         line_offsets(a.g, line, io, oo);
@@ -473,26 +480,35 @@ __device__ __forceinline__ void stage_load(const StageArgs<T> &a, cpx<T> *data, 
             }
         }
     }
+}
+
+// Tile load: global -> LDS. lc lines are computed at a time (lc * tpl
+// threads) and the tile holds lt = lc * rounds lines. Threads beyond
+// lc * tpl only join the barrier.
+template <typename T, bool NT>
+__device__ __forceinline__ void stage_load(const StageArgs<T> &a, cpx<T> *data, int64_t tile, int tid, int l, int tl)
+{
+    int ll, p0, step, lstep, cnt;
+    tile_mapping(a, a.in_col, tid, l, tl, ll, p0, step, lstep, cnt);
+#pragma unroll 1
+    for (int i = 0; i < cnt; ++i) {
+        // an opaque copy of the first point per line keeps the compiler from
+        // hoisting per-point work out of the loop (and spilling it)
+        int p = p0;
+        asm volatile("" : "+v"(p));
+        load_line<T, NT>(a, data, tile, ll + i * lstep, p, step);
+    }
     __syncthreads();
 }
 
 // ---- post-processing + store: LDS -> global (output layout)
+// Line lo of the tile, points k0, k0 + tpl, ... : post-processing, LDS -> global.
 template <typename T, int MG, bool NT>
-__device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> *data, const cpx<T> *tws,
-                                            int64_t tile, int tid, int l, int tl)
+__device__ __forceinline__ void store_line(const StageArgs<T> &a, const cpx<T> *data, const cpx<T> *tws,
+                                           int64_t tile, int lo, int k0, int tpl)
 {
-    const int lt = a.lt, tpl = a.tpl;
-    if (tid >= lt * tpl) return;
     {
-        int lo, k0;
-        if (a.out_col) {
-            lo = tid % lt;
-            k0 = tid / lt;
-        } else {
-            lo = l;
-            k0 = tl;
-        }
-        const int64_t line = tile * lt + lo;
+        const int64_t line = tile * a.lt + lo;
         if (line < a.nlines) {
             int64_t io, oo;
             line_offsets(a.g, line, io, oo);
@@ -556,24 +572,40 @@ __device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> 
     }
 }
 
+// Tile store, LDS -> global: the thread mapping of stage_load on the
+// output side.
+template <typename T, int MG, bool NT>
+__device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> *data, const cpx<T> *tws,
+                                            int64_t tile, int tid, int l, int tl)
+{
+    int lo, k0, step, lstep, cnt;
+    tile_mapping(a, a.out_col, tid, l, tl, lo, k0, step, lstep, cnt);
+#pragma unroll 1
+    for (int i = 0; i < cnt; ++i) {
+        int k = k0;
+        asm volatile("" : "+v"(k));
+        store_line<T, MG, NT>(a, data, tws, tile, lo + i * lstep, k, step);
+    }
+}
+
 template <typename T, bool WIDE>
 __device__ __forceinline__ void run_pass(int R, cpx<T> *ln, const cpx<T> *twp, int m, int Ns, int tl, int tpl)
 {
     switch (R) {
-    case 2: radix_pass<T, 2>(ln, twp, m, Ns, tl, tpl); break;
-    case 3: radix_pass<T, 3>(ln, twp, m, Ns, tl, tpl); break;
-    case 4: radix_pass<T, 4>(ln, twp, m, Ns, tl, tpl); break;
-    case 5: radix_pass<T, 5>(ln, twp, m, Ns, tl, tpl); break;
-    case 8: radix_pass<T, 8>(ln, twp, m, Ns, tl, tpl); break;
+    case 2: radix_pass<T, 2, WIDE>(ln, twp, m, Ns, tl, tpl); break;
+    case 3: radix_pass<T, 3, WIDE>(ln, twp, m, Ns, tl, tpl); break;
+    case 4: radix_pass<T, 4, WIDE>(ln, twp, m, Ns, tl, tpl); break;
+    case 5: radix_pass<T, 5, WIDE>(ln, twp, m, Ns, tl, tpl); break;
+    case 8: radix_pass<T, 8, WIDE>(ln, twp, m, Ns, tl, tpl); break;
     case 16:
-        if constexpr (Regs<T>::max_radix >= 16) radix_pass<T, 16>(ln, twp, m, Ns, tl, tpl);
+        if constexpr (Regs<T>::max_radix >= 16) radix_pass<T, 16, WIDE>(ln, twp, m, Ns, tl, tpl);
         break;
     default:
         if constexpr (WIDE) {
-            if (R == 7) radix_pass<T, 7>(ln, twp, m, Ns, tl, tpl);
-            else if (R == 11) radix_pass<T, 11>(ln, twp, m, Ns, tl, tpl);
-            else if (R == 13) radix_pass<T, 13>(ln, twp, m, Ns, tl, tpl);
-            else radix_pass_generic<T>(ln, twp, m, R, Ns, tl, tpl);
+            if (R == 7) radix_pass<T, 7, WIDE>(ln, twp, m, Ns, tl, tpl);
+            else if (R == 11) radix_pass<T, 11, WIDE>(ln, twp, m, Ns, tl, tpl);
+            else if (R == 13) radix_pass<T, 13, WIDE>(ln, twp, m, Ns, tl, tpl);
+            else radix_pass_generic<T, Regs<T>::emax>(ln, twp, m, R, Ns, tl, tpl);
         }
         break;
     }
@@ -582,48 +614,58 @@ __device__ __forceinline__ void run_pass(int R, cpx<T> *ln, const cpx<T> *twp, i
 // WIDE instantiations also carry the radix 7 / 11 / 13 butterflies and the
 // generic pass; the narrow one keeps register pressure down for the
 // power-of-two and 3 / 5 lengths (and adds radix 32 for f32).
-// One tile of a generic stage. Threads beyond lt * tpl (a batched launch
-// sizes its blocks for the largest item) only take part in the barriers.
+// One tile of a generic stage: lt lines, computed lc at a time. Threads
+// beyond lc * tpl (a batched launch sizes its blocks for the largest item)
+// only take part in the barriers.
 template <typename T, bool WIDE, int MG>
 __device__ __forceinline__ void stage_body(const StageArgs<T> &a, int64_t tile, const cpx<T> *tws, cpx<T> *data,
                                            const int *radix, const int *twoff)
 {
     const int tid = threadIdx.x;
-    const int lt = a.lt, tpl = a.tpl;
-    const bool act = tid < lt * tpl;
-    // pass mapping: lt lines x tpl threads; inactive threads get an index
-    // beyond every line so that each pass skips them
+    const int lc = a.lc, tpl = a.tpl;
+    const bool act = tid < lc * tpl;
+    // pass mapping: lc lines x tpl threads per round; inactive threads get
+    // an index beyond every line so that each pass skips them
     const int l = act ? tid / tpl : 0;
     const int tl = act ? tid % tpl : (1 << 28);
-    cpx<T> *ln = data + l * a.ls;
     stage_load<T, false>(a, data, tile, tid, l, tl);
     if constexpr (MG == 0) {
         stage_store<T, 0, false>(a, data, tws, tile, tid, l, tl);
         return;
     }
-    // ---- pre-processing: raw line -> FFT input (in place through registers)
-    if (needs_pre(a.mode)) {
-        constexpr int E = Regs<T>::emax;
-        cpx<T> z[E];
+#pragma unroll 1
+    for (int r = 0; r < a.lt; r += lc) {
+        // Opaque per round: without it the compiler hoists the rounds'
+        // common tl-derived table reads and indexes out of the loop and keeps
+        // them live through every pass (hundreds of bytes of spills a lane).
+        int tlr = tl;
+        asm volatile("" : "+v"(tlr));
+        const int tl = tlr;
+        cpx<T> *ln = data + (l + r) * a.ls;
+        // ---- pre-processing: raw line -> FFT input (in place through registers)
+        if (needs_pre(a.mode)) {
+            constexpr int E = Regs<T>::emax;
+            cpx<T> z[E];
 #pragma unroll
-        for (int i = 0; i < E; ++i) {
-            const int j = tl + tpl * i;
-            if (j < a.m) z[i] = pre_value<T, MG>(a, ln, tws, j);
-        }
-        __syncthreads();
+            for (int i = 0; i < E; ++i) {
+                const int j = tl + tpl * i;
+                if (j < a.m) z[i] = pre_value<T, MG>(a, ln, tws, j);
+            }
+            __syncthreads();
 #pragma unroll
-        for (int i = 0; i < E; ++i) {
-            const int j = tl + tpl * i;
-            if (j < a.m) Lset(ln, j, z[i]);
+            for (int i = 0; i < E; ++i) {
+                const int j = tl + tpl * i;
+                if (j < a.m) Lset(ln, j, z[i]);
+            }
+            __syncthreads();
         }
-        __syncthreads();
-    }
-    // ---- Stockham passes in LDS (decimation in time, natural-order output)
-    int Ns = 1;
-    for (int p = 0; p < a.npass; ++p) {
-        const int R = radix[p];
-        run_pass<T, WIDE>(R, ln, tws + twoff[p], a.m, Ns, tl, tpl);
-        Ns *= R;
+        // ---- Stockham passes in LDS (decimation in time, natural-order output)
+        int Ns = 1;
+        for (int p = 0; p < a.npass; ++p) {
+            const int R = radix[p];
+            run_pass<T, WIDE>(R, ln, tws + twoff[p], a.m, Ns, tl, tpl);
+            Ns *= R;
+        }
     }
     stage_store<T, MG, false>(a, data, tws, tile, tid, l, tl);
 }

@@ -71,6 +71,7 @@ struct Step {
     int64_t nlines = 0, ips = 1, ops = 1;
     int in_col = 0, out_col = 0;
     int lt = 1, tpl = 1, ls = 1, threads = 64;
+    int lc = 1; // lines computed at a time: generic stages may hold lt = lc * rounds lines
     int64_t tiles = 0, grid = 0;
     int xcd_per = 0;
     size_t lds_bytes = 0;

@@ -23,6 +23,20 @@ struct Regs {
     static constexpr int max_radix = sizeof(T) == 4 ? 32 : 8;
 };
 constexpr int kEmax = 8;         // generic kernel values per thread and pass
+#ifndef PFH_WIDE_MB11
+#define PFH_WIDE_MB11 1
+#endif
+// Butterflies per thread and pass of the generic kernel: kEmax values per
+// thread; in the instantiation with the odd-prime passes the radix 5 and 7
+// passes take 3 and 2 butterflies (a 385-point line needs 64 threads instead
+// of 128, so twice the lines are computed at a time).
+constexpr int generic_mb(int R, bool wide)
+{
+    if (wide && R == 5) return 3;
+    if (wide && R == 7) return 2;
+    if (wide && (R == 11 || R == 13)) return PFH_WIDE_MB11;
+    return kEmax / R > 0 ? kEmax / R : 1;
+}
 constexpr int kMaxThreadsWide = 512; // generic kernel with the odd-prime passes
 constexpr int kMaxThreads = 1024;
 
@@ -92,6 +106,7 @@ struct StageArgs {
     int twn;                // complex entries of tw staged into LDS
     int fem;                // values per thread and pass of the compile-time plan (0: generic kernel)
     int lt, tpl;            // lines per tile, threads per line
+    int lc;                 // lines computed at a time (lc * tpl threads; lt is a multiple of lc)
     int ls;                 // LDS stride of one line (complex units)
     int in_col, out_col;    // column (lines adjacent in memory) access on load / store
     int xcd_per;            // tiles per XCD when F_XCD_REMAP

@@ -222,10 +222,40 @@ static void offsets(const LineGeom &g, int64_t line, int64_t &io, int64_t &oo)
 // shape checks shared by the single-stage kernels and the batch items
 template <typename T>
 static void check_shape(const char *what, const StageArgs<T> &a, int threads, size_t lds, int emax, bool fast,
-                        int fem)
+                        int fem, bool wide = false)
 {
     const int cb = 2 * (int)sizeof(T);
-    if (a.lt * a.tpl > threads) fail("%s: lt*tpl %d > threads %d", what, a.lt * a.tpl, threads);
+    if (a.lc < 1 || a.lt % a.lc != 0) fail("%s: lt %d not a multiple of lc %d", what, a.lt, a.lc);
+    if (fast && a.lc != a.lt) fail("%s: fast stage computes in rounds", what);
+    if (a.lc * a.tpl > threads) fail("%s: lc*tpl %d > threads %d", what, a.lc * a.tpl, threads);
+    if ((a.in_col || a.out_col) && a.lt > a.lc * a.tpl) fail("%s: column tile %d > threads", what, a.lt);
+    if (!fast) {
+        // stage_load / stage_store thread mapping: each (line, point) of a
+        // tile exactly once on both sides
+        const int nthr = a.lc * a.tpl;
+        for (int side = 0; side < 2; ++side) {
+            const bool col = side == 0 ? a.in_col : a.out_col;
+            const int np = side == 0 ? a.nin : a.nout;
+            std::vector<int> hits((size_t)a.lt * np, 0);
+            auto visit = [&](int ll, int p0, int step) {
+                for (int p = p0; p < np; p += step) ++hits[(size_t)ll * np + p];
+            };
+            for (int tid = 0; tid < nthr; ++tid) {
+                if (col) {
+                    const int rows = nthr / a.lt;
+                    if (tid < rows * a.lt) visit(tid % a.lt, tid / a.lt, rows);
+                } else {
+                    for (int r = 0; r < a.lt; r += a.lc) visit(tid / a.tpl + r, tid % a.tpl, a.tpl);
+                }
+            }
+            for (size_t i = 0; i < hits.size(); ++i)
+                if (hits[i] != 1) {
+                    fail("%s: %s mapping touches (%zu, %zu) %d times", what, side ? "store" : "load", i / np, i % np,
+                         hits[i]);
+                    break;
+                }
+        }
+    }
     if ((size_t)(a.twn + (int64_t)a.lt * a.ls) * cb > lds) fail("%s: LDS %zu too small", what, lds);
     if (lds > kLdsLimit - 256) fail("%s: LDS %zu above the budget", what, lds);
     if (a.ntiles * a.lt < a.nlines) fail("%s: tiles do not cover the lines", what);
@@ -244,7 +274,7 @@ static void check_shape(const char *what, const StageArgs<T> &a, int threads, si
         if (generic) {
             if ((int64_t)a.tpl * em < a.m) fail("%s: pass %d (generic radix %d) not covered", what, p, R);
         } else {
-            const int MB = std::max(1, em / R);
+            const int MB = fast ? std::max(1, em / R) : generic_mb(R, wide);
             if ((int64_t)a.tpl * MB < a.m / R) fail("%s: pass %d (radix %d) not covered", what, p, R);
         }
     }
@@ -299,9 +329,9 @@ static void trace(const char *what, const StageArgs<T> &a, int threads, int64_t 
 {
     if (!g_trace) return;
     std::printf("  %s mode %d n %d m %d nlines %" PRId64 " lt %d tpl %d threads %d tiles %" PRId64 " grid %" PRId64
-                " ips %" PRId64 " ops %" PRId64 " col %d/%d flags %d digits",
+                " ips %" PRId64 " ops %" PRId64 " col %d/%d flags %d lc %d digits",
                 what, a.mode, a.n, a.m, a.nlines, a.lt, a.tpl, threads, a.ntiles, grid, a.ips, a.ops, a.in_col,
-                a.out_col, a.flags);
+                a.out_col, a.flags, a.lc);
     for (int d = 0; d < a.g.nd; ++d)
         std::printf(" [%" PRId64 ":%" PRId64 "/%" PRId64 "]", a.g.ext[d], a.g.is[d], a.g.os[d]);
     std::printf("\n");
@@ -315,7 +345,7 @@ hipError_t launch_line_stage(const StageArgs<T> &a, bool wide, int threads, size
     begin_launch();
     if (threads > (wide ? kMaxThreadsWide : kMaxThreads)) fail("line_stage: %d threads", threads);
     trace("line_stage", a, threads, grid);
-    check_shape("line_stage", a, threads, lds_bytes, Regs<T>::emax, false, 0);
+    check_shape("line_stage", a, threads, lds_bytes, Regs<T>::emax, false, 0, wide);
     check_grid("line_stage", a, grid);
     if (a.twn > 0) check_read("line_stage tables", a.tw, (size_t)a.twn * 2 * sizeof(T));
     emulate("line_stage", a);
@@ -358,7 +388,7 @@ bool fast_plan_available(int tbytes, int r0, int r1, int r2, int em)
 template <typename T>
 static void emulate_batch(const char *what, const StageArgs<T> *items, const int64_t *prefix, int nitems,
                           int64_t total, int threads, size_t lds_bytes, int64_t grid, bool fast, int em,
-                          const int *fr)
+                          const int *fr, bool wide = false)
 {
     ++g_launches;
     begin_launch();
@@ -373,7 +403,11 @@ static void emulate_batch(const char *what, const StageArgs<T> *items, const int
         if (a.mode != M_C2C) fail("%s: item %d mode", what, q);
         if (fast && (a.radix[0] != fr[0] || a.radix[1] != fr[1] || (fr[2] && a.radix[2] != fr[2])))
             fail("%s: item %d radices differ from the launch's plan", what, q);
-        check_shape(what, a, threads, lds_bytes, Regs<T>::emax, fast, em);
+        for (int p = 0; p < a.npass && !fast; ++p)
+            if (!wide && !(a.radix[p] == 2 || a.radix[p] == 3 || a.radix[p] == 4 || a.radix[p] == 5 ||
+                           a.radix[p] == 8 || a.radix[p] == 16 || a.radix[p] == 32))
+                fail("%s: item %d radix %d in a narrow launch", what, q, a.radix[p]);
+        check_shape(what, a, threads, lds_bytes, Regs<T>::emax, fast, em, wide);
         emulate(what, a);
     }
 }
@@ -383,8 +417,7 @@ hipError_t launch_line_batch(const StageArgs<T> *items, const int64_t *prefix, i
                              int threads, size_t lds_bytes, int64_t grid, hipStream_t)
 {
     const int none[3] = {0, 0, 0};
-    (void)wide;
-    emulate_batch("line_batch", items, prefix, nitems, total, threads, lds_bytes, grid, false, 0, none);
+    emulate_batch("line_batch", items, prefix, nitems, total, threads, lds_bytes, grid, false, 0, none, wide);
     return hipSuccess;
 }
 template hipError_t launch_line_batch<float>(const StageArgs<float> *, const int64_t *, int, int64_t, bool, int,
