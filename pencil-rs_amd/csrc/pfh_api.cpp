@@ -621,6 +621,8 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
         ++nd;
     }
     g.nd = nd;
+    // the kernels divide line indexes in 32 bits (make_fastdiv)
+    if (nlines > INT32_MAX) return set_err(PFH_ERR_UNSUPPORTED, "more than 2^31 - 1 lines in one stage");
     st.g = g;
     st.nlines = nlines;
 
@@ -1564,9 +1566,19 @@ static StageArgs<T> fill_args(const PlanImpl &P, const Step &s, void *const bufs
     a.nout = s.nout;
     a.m = s.m;
     a.npass = (int)s.radices.size();
+    int Ns = 1;
     for (int p = 0; p < a.npass; ++p) {
         a.radix[p] = s.radices[p];
         a.twoff[p] = s.twoff[p];
+        const FastDiv f = make_fastdiv((uint32_t)Ns);
+        a.nsmag[p] = f.mag;
+        a.nssh[p] = f.sh;
+        Ns *= s.radices[p];
+    }
+    for (int d = 0; d < a.g.nd; ++d) {
+        const FastDiv f = make_fastdiv((uint32_t)a.g.ext[d]);
+        a.g.dmag[d] = f.mag;
+        a.g.dsh[d] = f.sh;
     }
     a.pre0 = s.pre0;
     a.pre1 = s.pre1;

@@ -26,22 +26,23 @@ namespace pfh {
 
 namespace {
 
-// Offsets of a line's first input and output element. The digit loop is
-// unrolled over the compile-time maximum so that the argument struct is never
-// indexed dynamically (which would force a private copy of it).
+// Offsets of a line's first input and output element (line < 2^31, digit
+// quotients by multiply-high with the planner's constants). The digit loop
+// is unrolled over the compile-time maximum so that the argument struct is
+// never indexed dynamically (which would force a private copy of it).
 __device__ __forceinline__ void line_offsets(const LineGeom &g, int64_t line, int64_t &io, int64_t &oo)
 {
     io = 0;
     oo = 0;
+    uint32_t x = (uint32_t)line;
 #pragma unroll
     for (int d = 0; d < kMaxDigits; ++d) {
         if (d < g.nd) {
-            const int64_t e = g.ext[d];
-            const int64_t q = line / e;
-            const int64_t r = line - q * e;
-            io += r * g.is[d];
-            oo += r * g.os[d];
-            line = q;
+            const uint32_t q = fastdiv(x, g.dmag[d], g.dsh[d]);
+            const uint32_t r = x - q * (uint32_t)g.ext[d];
+            io += (int64_t)r * g.is[d];
+            oo += (int64_t)r * g.os[d];
+            x = q;
         }
     }
 }
@@ -330,19 +331,22 @@ __device__ __forceinline__ cpx<T> post_complex(const StageArgs<T> &a, const cpx<
 // x[j + r m/R], twiddle by exp(-2 pi i r (j mod Ns) / (Ns R)) and write
 // y[(j - j mod Ns) R + j mod Ns + r Ns].
 template <typename T, int R, bool WIDE>
-__device__ __forceinline__ void radix_pass(cpx<T> *ln, const cpx<T> *twp, int m, int Ns, int tl, int tpl)
+__device__ __forceinline__ void radix_pass(cpx<T> *ln, const cpx<T> *twp, int m, int Ns, FastDiv nsd, int tl,
+                                           int tpl)
 {
     constexpr int MB = generic_mb(R, WIDE);
     const int nb = m / R;
     cpx<T> v[MB][R];
+    int kk[MB];
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
         const int j = tl + tpl * b;
+        kk[b] = j - (int)fastdiv((uint32_t)j, nsd.mag, nsd.sh) * Ns; // j mod Ns
         if (j < nb) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b][r] = L(ln, j + r * nb);
             if (Ns > 1) {
-                const int k = j % Ns;
+                const int k = kk[b];
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], twp[k * (R - 1) + (r - 1)]);
             }
@@ -354,7 +358,7 @@ __device__ __forceinline__ void radix_pass(cpx<T> *ln, const cpx<T> *twp, int m,
     for (int b = 0; b < MB; ++b) {
         const int j = tl + tpl * b;
         if (j < nb) {
-            const int k = j % Ns;
+            const int k = kk[b];
             const int base = (j - k) * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) Lset(ln, base + r * Ns, v[b][r]);
@@ -589,22 +593,23 @@ __device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> 
 }
 
 template <typename T, bool WIDE>
-__device__ __forceinline__ void run_pass(int R, cpx<T> *ln, const cpx<T> *twp, int m, int Ns, int tl, int tpl)
+__device__ __forceinline__ void run_pass(int R, cpx<T> *ln, const cpx<T> *twp, int m, int Ns, FastDiv nsd, int tl,
+                                         int tpl)
 {
     switch (R) {
-    case 2: radix_pass<T, 2, WIDE>(ln, twp, m, Ns, tl, tpl); break;
-    case 3: radix_pass<T, 3, WIDE>(ln, twp, m, Ns, tl, tpl); break;
-    case 4: radix_pass<T, 4, WIDE>(ln, twp, m, Ns, tl, tpl); break;
-    case 5: radix_pass<T, 5, WIDE>(ln, twp, m, Ns, tl, tpl); break;
-    case 8: radix_pass<T, 8, WIDE>(ln, twp, m, Ns, tl, tpl); break;
+    case 2: radix_pass<T, 2, WIDE>(ln, twp, m, Ns, nsd, tl, tpl); break;
+    case 3: radix_pass<T, 3, WIDE>(ln, twp, m, Ns, nsd, tl, tpl); break;
+    case 4: radix_pass<T, 4, WIDE>(ln, twp, m, Ns, nsd, tl, tpl); break;
+    case 5: radix_pass<T, 5, WIDE>(ln, twp, m, Ns, nsd, tl, tpl); break;
+    case 8: radix_pass<T, 8, WIDE>(ln, twp, m, Ns, nsd, tl, tpl); break;
     case 16:
-        if constexpr (Regs<T>::max_radix >= 16) radix_pass<T, 16, WIDE>(ln, twp, m, Ns, tl, tpl);
+        if constexpr (Regs<T>::max_radix >= 16) radix_pass<T, 16, WIDE>(ln, twp, m, Ns, nsd, tl, tpl);
         break;
     default:
         if constexpr (WIDE) {
-            if (R == 7) radix_pass<T, 7, WIDE>(ln, twp, m, Ns, tl, tpl);
-            else if (R == 11) radix_pass<T, 11, WIDE>(ln, twp, m, Ns, tl, tpl);
-            else if (R == 13) radix_pass<T, 13, WIDE>(ln, twp, m, Ns, tl, tpl);
+            if (R == 7) radix_pass<T, 7, WIDE>(ln, twp, m, Ns, nsd, tl, tpl);
+            else if (R == 11) radix_pass<T, 11, WIDE>(ln, twp, m, Ns, nsd, tl, tpl);
+            else if (R == 13) radix_pass<T, 13, WIDE>(ln, twp, m, Ns, nsd, tl, tpl);
             else radix_pass_generic<T, Regs<T>::emax>(ln, twp, m, R, Ns, tl, tpl);
         }
         break;
@@ -619,7 +624,8 @@ __device__ __forceinline__ void run_pass(int R, cpx<T> *ln, const cpx<T> *twp, i
 // only take part in the barriers.
 template <typename T, bool WIDE, int MG>
 __device__ __forceinline__ void stage_body(const StageArgs<T> &a, int64_t tile, const cpx<T> *tws, cpx<T> *data,
-                                           const int *radix, const int *twoff)
+                                           const int *radix, const int *twoff, const uint32_t *nsmag,
+                                           const int *nssh)
 {
     const int tid = threadIdx.x;
     const int lc = a.lc, tpl = a.tpl;
@@ -663,7 +669,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T> &a, int64_t tile, 
         int Ns = 1;
         for (int p = 0; p < a.npass; ++p) {
             const int R = radix[p];
-            run_pass<T, WIDE>(R, ln, tws + twoff[p], a.m, Ns, tl, tpl);
+            run_pass<T, WIDE>(R, ln, tws + twoff[p], a.m, Ns, FastDiv{nsmag[p], nssh[p]}, tl, tpl);
             Ns *= R;
         }
     }
@@ -687,15 +693,18 @@ __global__ __launch_bounds__(WIDE ? kMaxThreadsWide : kMaxThreads) void k_line_s
     for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
     // pass plan in LDS (read with a runtime pass index; indexing the argument
     // struct dynamically would force a private copy of it)
-    __shared__ int s_radix[kMaxPasses], s_twoff[kMaxPasses];
+    __shared__ int s_radix[kMaxPasses], s_twoff[kMaxPasses], s_nssh[kMaxPasses];
+    __shared__ uint32_t s_nsmag[kMaxPasses];
     if (tid == 0) {
 #pragma unroll
         for (int p = 0; p < kMaxPasses; ++p) {
             s_radix[p] = a.radix[p];
             s_twoff[p] = a.twoff[p];
+            s_nsmag[p] = a.nsmag[p];
+            s_nssh[p] = a.nssh[p];
         }
     }
-    stage_body<T, WIDE, MG>(a, tile, tws, data, s_radix, s_twoff);
+    stage_body<T, WIDE, MG>(a, tile, tws, data, s_radix, s_twoff, s_nsmag, s_nssh);
 }
 
 // Persistent batched stage: the blocks of one launch walk a work list of
@@ -730,7 +739,7 @@ __global__ __launch_bounds__(WIDE ? kMaxThreadsWide : kMaxThreads) void k_line_b
         const int64_t tile = w - prefix[q];
         __syncthreads(); // the previous tile is done with LDS
         for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
-        stage_body<T, WIDE, 1>(a, tile, tws, tws + a.twn, a.radix, a.twoff);
+        stage_body<T, WIDE, 1>(a, tile, tws, tws + a.twn, a.radix, a.twoff, a.nsmag, a.nssh);
     }
 }
 

@@ -83,7 +83,34 @@ struct LineGeom {
     int64_t ext[kMaxDigits];      // digit extents, digit 0 fastest
     int64_t is[kMaxDigits];       // input element stride of each digit
     int64_t os[kMaxDigits];       // output element stride of each digit
+    uint32_t dmag[kMaxDigits];    // invariant-divisor constants of ext (make_fastdiv)
+    int dsh[kMaxDigits];
 };
+
+// Exact quotient x / d for 0 <= x < 2^31 and an invariant 1 <= d < 2^31
+// without a division: q = (x + mulhi(x, mag)) >> sh with sh = ceil(log2 d)
+// and mag = floor(2^32 (2^sh - d) / d) + 1, the round-up method of
+// Granlund and Montgomery (PLDI 1994, section 4). x + mulhi(x, mag) < 2^32
+// because mag < 2^32 and x < 2^31.
+struct FastDiv {
+    uint32_t mag;
+    int sh;
+};
+
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    int sh = 0;
+    while ((1ull << sh) < d) ++sh;
+    FastDiv f;
+    f.mag = (uint32_t)(((1ull << 32) * ((1ull << sh) - d)) / d + 1);
+    f.sh = sh;
+    return f;
+}
+
+__host__ __device__ __forceinline__ uint32_t fastdiv(uint32_t x, uint32_t mag, int sh)
+{
+    return (x + (uint32_t)(((uint64_t)x * mag) >> 32)) >> sh;
+}
 
 template <typename T>
 struct StageArgs {
@@ -102,6 +129,8 @@ struct StageArgs {
     int npass;
     int radix[kMaxPasses];
     int twoff[kMaxPasses];  // pass twiddle offsets (complex units) into tw
+    uint32_t nsmag[kMaxPasses]; // make_fastdiv constants of each pass's Ns (product of the earlier radices)
+    int nssh[kMaxPasses];
     int pre0, pre1, post0, post1; // table offsets into tw
     int twn;                // complex entries of tw staged into LDS
     int fem;                // values per thread and pass of the compile-time plan (0: generic kernel)

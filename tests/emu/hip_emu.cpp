@@ -213,10 +213,46 @@ static void offsets(const LineGeom &g, int64_t line, int64_t &io, int64_t &oo)
     io = oo = 0;
     for (int d = 0; d < g.nd; ++d) {
         const int64_t q = line / g.ext[d], r = line - q * g.ext[d];
+        // the kernels' quotient (multiply-high with the planner's constants)
+        if (line >= (1LL << 31) || (int64_t)fastdiv((uint32_t)line, g.dmag[d], g.dsh[d]) != q)
+            fail("line %" PRId64 " digit %d: fast quotient differs", line, d);
         io += r * g.is[d];
         oo += r * g.os[d];
         line = q;
     }
+}
+
+// make_fastdiv / fastdiv against the exact quotient: every divisor below
+// 2^12 with numerators around its multiples, then random pairs and the
+// extremes of the 31-bit range.
+static void fastdiv_selftest()
+{
+    const int before = g_fail;
+    auto check = [&](uint32_t x, uint32_t d) {
+        const FastDiv f = make_fastdiv(d);
+        if (fastdiv(x, f.mag, f.sh) != x / d) fail("fastdiv %u / %u", x, d);
+    };
+    for (uint32_t d = 1; d < 4096 && g_fail == before; ++d)
+        for (uint32_t q : {0u, 1u, 2u, 3u, 1000u, (0x7fffffffu / d) - 1, 0x7fffffffu / d})
+            for (int e = -1; e <= 1; ++e) {
+                const int64_t x = (int64_t)q * d + e;
+                if (x >= 0 && x < (1LL << 31)) check((uint32_t)x, d);
+            }
+    uint64_t st = 0x9e3779b97f4a7c15ull;
+    auto rnd = [&]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return st;
+    };
+    for (int i = 0; i < 2000000 && g_fail == before; ++i) {
+        const uint32_t d = (uint32_t)(rnd() % 0x7fffffffu) + 1;
+        const uint32_t dd = (i & 1) ? d : (d >> (rnd() % 31)) + 1;
+        check((uint32_t)(rnd() & 0x7fffffffu), dd);
+        check(0x7fffffffu, dd);
+        check(0x7fffffffu - (0x7fffffffu % dd), dd);
+    }
+    std::printf("fastdiv: %s\n", g_fail == before ? "ok" : "FAILED");
 }
 
 // shape checks shared by the single-stage kernels and the batch items
@@ -279,6 +315,19 @@ static void check_shape(const char *what, const StageArgs<T> &a, int threads, si
         }
     }
     if (a.m > 0 && prod != a.m) fail("%s: radix product %d != m %d", what, prod, a.m);
+    if (!fast) {
+        // each pass's j mod Ns through the planner's constants, every j of the pass
+        int Ns = 1;
+        for (int p = 0; p < a.npass; ++p) {
+            const int nb = a.m / a.radix[p];
+            for (int j = 0; j < nb; ++j)
+                if ((int)fastdiv((uint32_t)j, a.nsmag[p], a.nssh[p]) != j / Ns) {
+                    fail("%s: pass %d Ns %d constants", what, p, Ns);
+                    break;
+                }
+            Ns *= a.radix[p];
+        }
+    }
     if (mode_pre(a.mode) && (int64_t)a.tpl * (fast ? fem : emax) < a.m) fail("%s: pre-processing not covered", what);
 }
 
@@ -1053,6 +1102,7 @@ int main(int argc, char **argv)
     }
     // 1 MiB slab intermediates so that the small shapes here are pipelined
     setenv("PFH_SLAB_MB", "1", 0);
+    pfh::fastdiv_selftest();
     route_cases<double>(PFH_F64);
     route_cases<float>(PFH_F32);
     batch_cases();
