@@ -197,6 +197,47 @@ def f64_round_trips(torch, pf, n: int, steps: int = 20):
             "route_model_frac_of_peak": rm / dt / 1e9 / PEAK_GBPS, "round_trip_rel_l2": err}
 
 
+def r2r_dht_round_trips(torch, pf, shape=(256, 384, 240), steps: int = 20):
+    """Mixed-kind R2R and DHT on a 256 x 384 x 240 real grid (lengths with
+    factors 2, 3, 5; the DST-I axis runs a 385 = 5 x 7 x 11 point FFT):
+    forward + normalized inverse per dtype, the route model's bytes (three
+    stages, each reading and writing the field once, per operation) and the
+    round-trip error."""
+    K = pf.R2rKind
+    n = shape[0] * shape[1] * shape[2]
+    out = {"workload": "256x384x240 real: R2R (DCT-II, DST-I, DCT-IV) and DHT, forward + normalized inverse"}
+    for dt, name in ((torch.float32, "f32"), (torch.float64, "f64")):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(3)
+        x = torch.rand(n, generator=g, device="cuda", dtype=dt) * 2 - 1
+        y = torch.empty_like(x)
+        back = torch.empty_like(x)
+        res = {}
+        for tag, plan in (("r2r", pf.R2rPlan(shape, [K.DctII, K.DstI, K.DctIV], dtype=dt)),
+                          ("dht", pf.DhtPlan(shape, dtype=dt))):
+            ws = plan.allocate_workspace()
+
+            def step():
+                plan.forward(x, y, ws)
+                plan.inverse(y, back, ws)
+
+            step()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(steps):
+                step()
+            torch.cuda.synchronize()
+            dt_s = (time.perf_counter() - t) / steps
+            rm = 2 * 6 * n * x.element_size()
+            res[tag + "_pair_ms"] = dt_s * 1e3
+            res[tag + "_route_model_frac_of_peak"] = rm / dt_s / 1e9 / PEAK_GBPS
+            res[tag + "_round_trip_rel_l2"] = float(torch.linalg.norm(back - x) / torch.linalg.norm(x))
+            del ws
+        out[name] = res
+        del x, y, back
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,6 +364,7 @@ def main():
         plan.take_status(ws)
         out["batched"] = batched_vs_plain(torch, pf)
         out["f64"] = f64_round_trips(torch, pf, n)
+        out["r2r_dht"] = r2r_dht_round_trips(torch, pf)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(n, threads)

@@ -645,10 +645,11 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
             need = std::max(need, ceil_div(st.m, emax));
             continue;
         }
-        const int MB = st.fast ? std::max(1, emax / R) : generic_mb(R, wide_plan);
+        const int MB = st.fast ? std::max(1, emax / R) : generic_mb(R, wide_plan, tbytes == 4);
         need = std::max(need, ceil_div(st.m / R, MB));
     }
-    if (mode_needs_pre(st.mode)) need = std::max(need, ceil_div(st.m, emax));
+    if (mode_needs_pre(st.mode))
+        need = std::max(need, ceil_div(st.m, st.fast ? emax : generic_pre_e(wide_plan, tbytes == 4)));
     int tpl = pow2ceil(need);
     static const int block_env = env_int("PFH_BLOCK", 256);
     const int target = tu.block > 0 ? tu.block : block_env;

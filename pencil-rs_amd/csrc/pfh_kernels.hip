@@ -334,7 +334,7 @@ template <typename T, int R, bool WIDE>
 __device__ __forceinline__ void radix_pass(cpx<T> *ln, const cpx<T> *twp, int m, int Ns, FastDiv nsd, int tl,
                                            int tpl)
 {
-    constexpr int MB = generic_mb(R, WIDE);
+    constexpr int MB = generic_mb(R, WIDE, sizeof(T) == 4);
     const int nb = m / R;
     cpx<T> v[MB][R];
     int kk[MB];
@@ -450,15 +450,15 @@ __device__ __forceinline__ void load_line(const StageArgs<T> &a, cpx<T> *data, i
         if (cin) {
             const cpx<T> *s = reinterpret_cast<const cpx<T> *>(a.src) + io;
             const bool cj = (a.flags & F_CONJ_IN) != 0;
-            for (int base = p0; base < a.nin; base += step * Regs<T>::unroll) {
-                cpx<T> v[Regs<T>::unroll];
+            for (int base = p0; base < a.nin; base += step * Regs<T>::gunroll) {
+                cpx<T> v[Regs<T>::gunroll];
 #pragma unroll
-                for (int u = 0; u < Regs<T>::unroll; ++u) {
+                for (int u = 0; u < Regs<T>::gunroll; ++u) {
                     const int p = base + u * step;
                     if (p < a.nin) v[u] = gld<NT>(s + (int64_t)p * a.ips);
                 }
 #pragma unroll
-                for (int u = 0; u < Regs<T>::unroll; ++u) {
+                for (int u = 0; u < Regs<T>::gunroll; ++u) {
                     const int p = base + u * step;
                     if (p < a.nin) {
                         cpx<T> w = v[u];
@@ -469,15 +469,15 @@ __device__ __forceinline__ void load_line(const StageArgs<T> &a, cpx<T> *data, i
             }
         } else {
             const T *s = reinterpret_cast<const T *>(a.src) + io;
-            for (int base = p0; base < a.nin; base += step * (2 * Regs<T>::unroll)) {
-                T v[2 * Regs<T>::unroll];
+            for (int base = p0; base < a.nin; base += step * (2 * Regs<T>::gunroll)) {
+                T v[2 * Regs<T>::gunroll];
 #pragma unroll
-                for (int u = 0; u < 2 * Regs<T>::unroll; ++u) {
+                for (int u = 0; u < 2 * Regs<T>::gunroll; ++u) {
                     const int p = base + u * step;
                     if (p < a.nin) v[u] = gld<NT>(s + (int64_t)p * a.ips);
                 }
 #pragma unroll
-                for (int u = 0; u < 2 * Regs<T>::unroll; ++u) {
+                for (int u = 0; u < 2 * Regs<T>::gunroll; ++u) {
                     const int p = base + u * step;
                     if (p < a.nin) LRset(lln, p, v[u]);
                 }
@@ -517,7 +517,7 @@ __device__ __forceinline__ void store_line(const StageArgs<T> &a, const cpx<T> *
             int64_t io, oo;
             line_offsets(a.g, line, io, oo);
             const cpx<T> *Z = data + lo * a.ls;
-            constexpr int U = Regs<T>::unroll;
+            constexpr int U = Regs<T>::gunroll;
             if constexpr (MG == 0) {
             if (a.mode == M_COPY_C) {
                 // identity stage: move the bits, no arithmetic (bit-exact permutation)
@@ -650,7 +650,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T> &a, int64_t tile, 
         cpx<T> *ln = data + (l + r) * a.ls;
         // ---- pre-processing: raw line -> FFT input (in place through registers)
         if (needs_pre(a.mode)) {
-            constexpr int E = Regs<T>::emax;
+            constexpr int E = generic_pre_e(WIDE, sizeof(T) == 4);
             cpx<T> z[E];
 #pragma unroll
             for (int i = 0; i < E; ++i) {

@@ -19,22 +19,30 @@ template <typename T>
 struct Regs {
     // generic kernel: 8 values per thread and pass (f32 and f64)
     static constexpr int emax = 8;
-    static constexpr int unroll = sizeof(T) == 4 ? 16 : 8; // loads / stores in flight per chunk
+    static constexpr int unroll = sizeof(T) == 4 ? 16 : 8; // fast kernels: loads / stores in flight per chunk
+#ifndef PFH_GUNROLL_F32
+#define PFH_GUNROLL_F32 4
+#endif
+#ifndef PFH_GUNROLL_F64
+#define PFH_GUNROLL_F64 4
+#endif
+    // generic kernel: loads / stores per unrolled chunk
+    static constexpr int gunroll = sizeof(T) == 4 ? PFH_GUNROLL_F32 : PFH_GUNROLL_F64;
     static constexpr int max_radix = sizeof(T) == 4 ? 32 : 8;
 };
 constexpr int kEmax = 8;         // generic kernel values per thread and pass
-#ifndef PFH_WIDE_MB11
-#define PFH_WIDE_MB11 1
-#endif
-// Butterflies per thread and pass of the generic kernel: kEmax values per
-// thread; in the instantiation with the odd-prime passes the radix 5 and 7
-// passes take 3 and 2 butterflies (a 385-point line needs 64 threads instead
-// of 128, so twice the lines are computed at a time).
-constexpr int generic_mb(int R, bool wide)
+// Generic kernel budgets. The instantiation with the odd-prime passes gives
+// the radix 5 and 7 passes 3 and 2 butterflies per thread and, in f32, the
+// radix 11 / 13 passes 2 and the pre-processing 16 values per thread, so a
+// 385-point line (5 x 7 x 11) takes 32 threads (f32) or 64 (f64) instead of
+// 128; f64 keeps the smaller budgets, whose registers would spill.
+constexpr int generic_pre_e(bool wide, bool f32) { return (wide && f32) ? 16 : kEmax; }
+
+constexpr int generic_mb(int R, bool wide, bool f32)
 {
     if (wide && R == 5) return 3;
     if (wide && R == 7) return 2;
-    if (wide && (R == 11 || R == 13)) return PFH_WIDE_MB11;
+    if (wide && f32 && (R == 11 || R == 13)) return 2;
     return kEmax / R > 0 ? kEmax / R : 1;
 }
 constexpr int kMaxThreadsWide = 512; // generic kernel with the odd-prime passes

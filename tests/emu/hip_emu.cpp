@@ -310,7 +310,7 @@ static void check_shape(const char *what, const StageArgs<T> &a, int threads, si
         if (generic) {
             if ((int64_t)a.tpl * em < a.m) fail("%s: pass %d (generic radix %d) not covered", what, p, R);
         } else {
-            const int MB = fast ? std::max(1, em / R) : generic_mb(R, wide);
+            const int MB = fast ? std::max(1, em / R) : generic_mb(R, wide, sizeof(T) == 4);
             if ((int64_t)a.tpl * MB < a.m / R) fail("%s: pass %d (radix %d) not covered", what, p, R);
         }
     }
@@ -328,7 +328,8 @@ static void check_shape(const char *what, const StageArgs<T> &a, int threads, si
             Ns *= a.radix[p];
         }
     }
-    if (mode_pre(a.mode) && (int64_t)a.tpl * (fast ? fem : emax) < a.m) fail("%s: pre-processing not covered", what);
+    if (mode_pre(a.mode) && (int64_t)a.tpl * (fast ? fem : generic_pre_e(wide, sizeof(T) == 4)) < a.m)
+        fail("%s: pre-processing not covered", what);
 }
 
 // identity-on-points emulation of one line stage
