@@ -340,7 +340,10 @@ static void choose_fast(Step &st, int tbytes)
     static const int enabled = env_int("PFH_FAST", 1);
     static const int v512 = env_int("PFH_FAST512", 0);
     if (!enabled) return;
-    if (!(st.mode == M_C2C || st.mode == M_R2C || st.mode == M_C2R)) return;
+    // even DHT lines run the R2C plan with a Hartley store (no input /
+    // output reversal, sign alternation or end weights)
+    const bool dht = st.mode == M_DHT && !(st.flags & (F_PRE_REV | F_PRE_ALT | F_POST_REV | F_POST_ALT | F_WEIGHTS));
+    if (!(st.mode == M_C2C || st.mode == M_R2C || st.mode == M_C2R || dht)) return;
     const int m = st.m;
     int r0 = 0, r1 = 0, r2 = 0, em = 0;
     if (tbytes == 4) {

@@ -1038,6 +1038,50 @@ __device__ __forceinline__ void fast_r2c_store(const StageArgs<T> &a, const cpx<
     }
 }
 
+// DHT store of an even line from the half-length transform of its real
+// pairs in LDS: H_k = Re X_k - Im X_k, with X_k = conj(X_{n-k}) above the
+// Nyquist index m (the generic kernel's post-processing of M_DHT).
+template <typename T, bool NT>
+__device__ __forceinline__ void fast_dht_store(const StageArgs<T> &a, const cpx<T> *data, const cpx<T> *tws,
+                                               int64_t tile, int tid, int l, int tl)
+{
+    const int lt = a.lt, tpl = a.tpl;
+    int lo, k0;
+    if (a.out_col) {
+        lo = tid % lt;
+        k0 = tid / lt;
+    } else {
+        lo = l;
+        k0 = tl;
+    }
+    const int64_t line = tile * lt + lo;
+    if (line >= a.nlines) return;
+    int64_t io, oo;
+    line_offsets(a.g, line, io, oo);
+    const cpx<T> *Z = data + lo * a.ls;
+    T *d = reinterpret_cast<T *>(a.dst) + oo;
+    const cpx<T> *w = tws + a.post0;
+    const int m = a.m;
+    constexpr int U = Regs<T>::unroll;
+    for (int base = k0; base < a.nout; base += tpl * U) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = base + u * tpl;
+            if (k < a.nout) {
+                const bool low = k <= m;
+                const cpx<T> x = split(Z, w, m, low ? k : 2 * m - k);
+                v[u] = (low ? x.x - x.y : x.x + x.y) * a.scale;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = base + u * tpl;
+            if (k < a.nout) gst<NT>(d + (int64_t)k * a.ops, v[u]);
+        }
+    }
+}
+
 // One tile of a fast stage: tws holds the staged tables, data the tile's
 // lines. Threads beyond the stage's lt x tpl shape (a batched launch sized
 // for its largest item) take indexes past every line and only join the
@@ -1069,9 +1113,12 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
                                      (a.flags & F_CONJ_IN) != 0);
     }
     if constexpr (R2 != 0) fast_mid<T, R1, EM, M>(ln, tws + a.twoff[1], R0, tl, tpl);
-    if (mode == M_R2C) {
+    if (mode == M_R2C || mode == M_DHT) {
         fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
-        if (act) fast_r2c_store<T, NT>(a, data, tws, tile, tid, l, tl);
+        if (act) {
+            if (mode == M_R2C) fast_r2c_store<T, NT>(a, data, tws, tile, tid, l, tl);
+            else fast_dht_store<T, NT>(a, data, tws, tile, tid, l, tl);
+        }
         return;
     }
     const int fl = !act ? 0 : (a.out_col ? tid % lt : l);

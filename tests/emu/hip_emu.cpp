@@ -413,7 +413,10 @@ hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int e
     if (threads > 512) fail("line_fast: %d threads", threads);
     if (!fast_plan_available((int)sizeof(T), r0, r1, r2, em)) fail("line_fast: plan not instantiated");
     if (a.radix[0] != r0 || a.radix[1] != r1 || (r2 && a.radix[2] != r2)) fail("line_fast: radices differ");
-    if (!(a.mode == M_C2C || a.mode == M_R2C || a.mode == M_C2R)) fail("line_fast: mode %d", a.mode);
+    if (!(a.mode == M_C2C || a.mode == M_R2C || a.mode == M_C2R || a.mode == M_DHT))
+        fail("line_fast: mode %d", a.mode);
+    if (a.mode == M_DHT && (a.flags & (F_PRE_REV | F_PRE_ALT | F_POST_REV | F_POST_ALT | F_WEIGHTS)))
+        fail("line_fast: DHT with pre / post flags %d", a.flags);
     trace("line_fast", a, threads, grid);
     check_shape("line_fast", a, threads, lds_bytes, Regs<T>::emax, true, em);
     if (a.twn > 0) check_read("line_fast tables", a.tw, (size_t)a.twn * 2 * sizeof(T));
@@ -974,6 +977,24 @@ void route_cases(int prec)
                 run_case<T>(name, p, nd, op, true, true, -1, 0);
             pfh_plan_destroy(p);
         }
+    }
+    // DHT on every axis, lengths whose half has a compile-time plan
+    const std::vector<C> dht_cases = {{{128, 96, 40}, {}}, {{6, 128}, {3}}, {{128, 5, 96}, {2}}};
+    for (const C &c : dht_cases) {
+        char name[128];
+        std::string sh;
+        for (size_t v : c.shape) sh += std::to_string(v) + "x";
+        pfh_desc d = route_desc(PFH_R2R, prec, c.shape, c.extra);
+        for (size_t a = 0; a < c.shape.size(); ++a) d.axis_kind[a] = PFH_DHT;
+        pfh_plan p = nullptr;
+        if (pfh_plan_create(&d, &p) != PFH_OK) {
+            fail("DHT %s %s: plan", pn, sh.c_str());
+            continue;
+        }
+        std::snprintf(name, sizeof(name), "DHT %s %s", pn, sh.c_str());
+        for (int op : {PFH_FORWARD, PFH_INVERSE, PFH_ADJOINT_FORWARD})
+            run_case<T>(name, p, (int)c.shape.size(), op, true, true, -1, 0);
+        pfh_plan_destroy(p);
     }
 }
 

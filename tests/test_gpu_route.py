@@ -317,6 +317,10 @@ R2R_CASES = [
     ((2, 3, 2, 3), 1, [pf.R2rKind.DstI, pf.R2rKind.DstII, pf.R2rKind.DstIII, pf.R2rKind.DstIV]),
     ((16, 24, 20), 1, [pf.R2rKind.DctII, pf.R2rKind.DstI, pf.R2rKind.DctIV]),
     ((12, 10, 9), 2, [pf.R2rKind.DstII, pf.R2rKind.DctIII, pf.R2rKind.DstIV]),
+    # even DHT lines whose half length has a compile-time plan (R2C kernels with a Hartley store)
+    ((128, 96, 40), 1, [pf.R2rKind.Dht, pf.R2rKind.Dht, pf.R2rKind.Dht]),
+    ((6, 128), 3, [pf.R2rKind.Dht, pf.R2rKind.Dht]),
+    ((128, 5, 96), 1, [pf.R2rKind.Dht, None, pf.R2rKind.Dht]),
 ]
 
 
