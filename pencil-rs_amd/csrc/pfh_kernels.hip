@@ -90,11 +90,26 @@ __device__ __forceinline__ void LRset(cpx<T> *ln, int p, T v)
 
 // One input value of a real line after the reversal / alternating sign /
 // adjoint edge weights requested by the stage.
+// Raw real line values for the pre-processing: from the tile's LDS line
+// (generic kernel) or straight from global memory (fast kernels' first pass).
 template <typename T>
-__device__ __forceinline__ T xin(const StageArgs<T> &a, const cpx<T> *ln, int i)
+struct LdsReal {
+    const cpx<T> *ln;
+    __device__ __forceinline__ T operator()(int p) const { return LR(ln, p); }
+};
+
+template <typename T, bool NT>
+struct GlobalReal {
+    const T *s;
+    int64_t ips;
+    __device__ __forceinline__ T operator()(int p) const { return gld<NT>(s + (int64_t)p * ips); }
+};
+
+template <typename T, typename Rd>
+__device__ __forceinline__ T xin(const StageArgs<T> &a, const Rd &rd, int i)
 {
     const int idx = (a.flags & F_PRE_REV) ? a.n - 1 - i : i;
-    T v = LR(ln, idx);
+    T v = rd(idx);
     if ((a.flags & F_PRE_ALT) && (idx & 1)) v = -v;
     if (a.flags & F_WEIGHTS) {
         if (idx == 0) v = v * a.prew0;
@@ -114,6 +129,68 @@ __device__ __forceinline__ cpx<T> hin(const StageArgs<T> &a, const cpx<T> *ln, i
     return v;
 }
 
+// FFT input value j of a real-input R2R mode, raw values read through rd.
+template <typename T, typename Rd>
+__device__ __forceinline__ cpx<T> pre_real(const StageArgs<T> &a, const Rd &rd, const cpx<T> *tws, int j)
+{
+    const int n = a.n;
+    switch (a.mode) {
+    case M_DHT_ODD:
+        return mk(xin(a, rd, j), (T)0);
+    case M_DCT2: {
+        const int h = n / 2;
+        const int i0 = 2 * j, i1 = 2 * j + 1;
+        T v0 = i0 < h ? xin(a, rd, 2 * i0) : xin(a, rd, 2 * (n - 1 - i0) + 1);
+        T v1 = i1 < h ? xin(a, rd, 2 * i1) : xin(a, rd, 2 * (n - 1 - i1) + 1);
+        return mk(v0, v1);
+    }
+    case M_DCT2_ODD: {
+        const int h = (n + 1) / 2;
+        T v = j < h ? xin(a, rd, 2 * j) : xin(a, rd, 2 * (n - 1 - j) + 1);
+        return mk(v, (T)0);
+    }
+    case M_DCT3: {
+        const int m = a.m;
+        // V_k = exp(i pi k / 2n) (y_k - i y_{n-k}), y_n = 0
+        const int ka = j, kb = m - j;
+        cpx<T> Va = cmul(tws[a.pre0 + ka], mk(xin(a, rd, ka), ka == 0 ? (T)0 : -xin(a, rd, n - ka)));
+        cpx<T> Vb = cmul(tws[a.pre0 + kb], mk(xin(a, rd, kb), kb == 0 ? (T)0 : -xin(a, rd, n - kb)));
+        if (ka == 0) Va.y = (T)0;
+        if (kb == m) Vb.y = (T)0;
+        cpx<T> B = conjg(Vb);
+        cpx<T> E = Va + B, D = Va - B;
+        cpx<T> O = cmul(tws[a.pre1 + j], D);
+        return conjg(E + mul_pi(O));
+    }
+    case M_DCT3_ODD: {
+        T y = xin(a, rd, j);
+        if (j == 0) y = y * (T)0.5;
+        return scal(tws[a.pre0 + j], y);
+    }
+    case M_DCT4:
+        return cmul(mk(xin(a, rd, 2 * j), xin(a, rd, n - 1 - 2 * j)), tws[a.pre0 + j]);
+    case M_DCT4_ODD:
+        return j < n ? scal(tws[a.pre0 + j], xin(a, rd, j)) : mk((T)0, (T)0);
+    case M_DCT1: {
+        const int Lx = 2 * (n - 1);
+        const int i0 = 2 * j, i1 = 2 * j + 1;
+        T v0 = i0 <= n - 1 ? xin(a, rd, i0) : xin(a, rd, Lx - i0);
+        T v1 = i1 <= n - 1 ? xin(a, rd, i1) : xin(a, rd, Lx - i1);
+        return mk(v0, v1);
+    }
+    case M_DST1: {
+        const int i0 = 2 * j, i1 = 2 * j + 1;
+        auto o = [&](int i) -> T {
+            if (i == 0 || i == n + 1) return (T)0;
+            return i <= n ? xin(a, rd, i - 1) : -xin(a, rd, 2 * n + 1 - i);
+        };
+        return mk(o(i0), o(i1));
+    }
+    default:
+        return mk(xin(a, rd, j), (T)0);
+    }
+}
+
 template <typename T, int MG>
 __device__ __forceinline__ cpx<T> pre_value(const StageArgs<T> &a, const cpx<T> *ln, const cpx<T> *tws, int j)
 {
@@ -121,7 +198,7 @@ __device__ __forceinline__ cpx<T> pre_value(const StageArgs<T> &a, const cpx<T> 
     if constexpr (MG == 1) {
     switch (a.mode) {
     case M_R2C_ODD:
-        return mk(xin(a, ln, j), (T)0);
+        return mk(xin(a, LdsReal<T>{ln}, j), (T)0);
     case M_C2R: {
         const int m = a.m; // n / 2; Nyquist index m
         cpx<T> A = hin(a, ln, j, m);
@@ -140,61 +217,7 @@ __device__ __forceinline__ cpx<T> pre_value(const StageArgs<T> &a, const cpx<T> 
         return L(ln, j);
     }
     } else {
-    switch (a.mode) {
-    case M_DHT_ODD:
-        return mk(xin(a, ln, j), (T)0);
-    case M_DCT2: {
-        const int h = n / 2;
-        const int i0 = 2 * j, i1 = 2 * j + 1;
-        T v0 = i0 < h ? xin(a, ln, 2 * i0) : xin(a, ln, 2 * (n - 1 - i0) + 1);
-        T v1 = i1 < h ? xin(a, ln, 2 * i1) : xin(a, ln, 2 * (n - 1 - i1) + 1);
-        return mk(v0, v1);
-    }
-    case M_DCT2_ODD: {
-        const int h = (n + 1) / 2;
-        T v = j < h ? xin(a, ln, 2 * j) : xin(a, ln, 2 * (n - 1 - j) + 1);
-        return mk(v, (T)0);
-    }
-    case M_DCT3: {
-        const int m = a.m;
-        // V_k = exp(i pi k / 2n) (y_k - i y_{n-k}), y_n = 0
-        const int ka = j, kb = m - j;
-        cpx<T> Va = cmul(tws[a.pre0 + ka], mk(xin(a, ln, ka), ka == 0 ? (T)0 : -xin(a, ln, n - ka)));
-        cpx<T> Vb = cmul(tws[a.pre0 + kb], mk(xin(a, ln, kb), kb == 0 ? (T)0 : -xin(a, ln, n - kb)));
-        if (ka == 0) Va.y = (T)0;
-        if (kb == m) Vb.y = (T)0;
-        cpx<T> B = conjg(Vb);
-        cpx<T> E = Va + B, D = Va - B;
-        cpx<T> O = cmul(tws[a.pre1 + j], D);
-        return conjg(E + mul_pi(O));
-    }
-    case M_DCT3_ODD: {
-        T y = xin(a, ln, j);
-        if (j == 0) y = y * (T)0.5;
-        return scal(tws[a.pre0 + j], y);
-    }
-    case M_DCT4:
-        return cmul(mk(xin(a, ln, 2 * j), xin(a, ln, n - 1 - 2 * j)), tws[a.pre0 + j]);
-    case M_DCT4_ODD:
-        return j < n ? scal(tws[a.pre0 + j], xin(a, ln, j)) : mk((T)0, (T)0);
-    case M_DCT1: {
-        const int Lx = 2 * (n - 1);
-        const int i0 = 2 * j, i1 = 2 * j + 1;
-        T v0 = i0 <= n - 1 ? xin(a, ln, i0) : xin(a, ln, Lx - i0);
-        T v1 = i1 <= n - 1 ? xin(a, ln, i1) : xin(a, ln, Lx - i1);
-        return mk(v0, v1);
-    }
-    case M_DST1: {
-        const int i0 = 2 * j, i1 = 2 * j + 1;
-        auto o = [&](int i) -> T {
-            if (i == 0 || i == n + 1) return (T)0;
-            return i <= n ? xin(a, ln, i - 1) : -xin(a, ln, 2 * n + 1 - i);
-        };
-        return mk(o(i0), o(i1));
-    }
-    default:
-        return L(ln, j);
-    }
+        return pre_real<T>(a, LdsReal<T>{ln}, tws, j);
     }
 }
 
@@ -817,6 +840,39 @@ __device__ __forceinline__ void fast_first(const StageArgs<T> &a, cpx<T> *ln, bo
     __syncthreads();
 }
 
+// First pass of a real-input R2R line: the pre-processing gathers each FFT
+// input value from global memory (pre_real through GlobalReal: reordering,
+// odd / even extension, pre-twiddles from the staged tables) into registers.
+template <typename T, int R, int EM, int M, bool NT>
+__device__ __forceinline__ void fast_first_pre(const StageArgs<T> &a, cpx<T> *ln, const cpx<T> *tws, bool valid,
+                                               int64_t io, int tl, int tpl)
+{
+    constexpr int MB = fast_mb<EM, R>();
+    constexpr int NB = M / R;
+    cpx<T> v[MB][R];
+    if (valid) {
+        const GlobalReal<T, NT> rd{reinterpret_cast<const T *>(a.src) + io, a.ips};
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            const int j = tl + tpl * b;
+            if (j < NB) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b][r] = pre_real<T>(a, rd, tws, j + r * NB);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            const int j = tl + tpl * b;
+            if (j < NB) {
+                Dft<T, R>::run(v[b]);
+#pragma unroll
+                for (int r = 0; r < R; ++r) Lset(ln, j * R + r, v[b][r]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // a pass in LDS with compile-time geometry
 template <typename T, int R, int EM, int M>
 __device__ __forceinline__ void fast_mid(cpx<T> *ln, const cpx<T> *twp, int Ns, int tl, int tpl)
@@ -1098,9 +1154,20 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
     const int tl = act ? tid % tpl : (1 << 28);
     cpx<T> *ln = data + l * a.ls;
     const int mode = a.mode;
+    // real-input R2R modes (every mode but these four)
+    const bool r2r = !(mode == M_C2C || mode == M_R2C || mode == M_C2R || mode == M_DHT);
     if (mode == M_C2R) {
         fast_c2r_load<T, M, NT>(a, data, tws, tile, tid, l, tl);
         fast_mid<T, R0, EM, M>(ln, tws, 1, tl, tpl);
+    } else if (r2r) {
+        __syncthreads(); // the pre-processing reads the staged tables
+        const int fl = !act ? 0 : (a.in_col ? tid % lt : l);
+        const int ft = !act ? (1 << 28) : (a.in_col ? tid / lt : tl);
+        const int64_t line = tile * lt + fl;
+        int64_t io = 0, oo = 0;
+        const bool valid = act && line < a.nlines;
+        if (valid) line_offsets(a.g, line, io, oo);
+        fast_first_pre<T, R0, EM, M, NT>(a, data + fl * a.ls, tws, valid, io, ft, tpl);
     } else {
         // column access: lanes across the tile's lines; row access: along a line
         const int fl = !act ? 0 : (a.in_col ? tid % lt : l);
@@ -1113,11 +1180,19 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
                                      (a.flags & F_CONJ_IN) != 0);
     }
     if constexpr (R2 != 0) fast_mid<T, R1, EM, M>(ln, tws + a.twoff[1], R0, tl, tpl);
-    if (mode == M_R2C || mode == M_DHT) {
+    if (mode == M_R2C || mode == M_DHT || r2r) {
         fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
         if (act) {
-            if (mode == M_R2C) fast_r2c_store<T, NT>(a, data, tws, tile, tid, l, tl);
-            else fast_dht_store<T, NT>(a, data, tws, tile, tid, l, tl);
+            if (mode == M_R2C) {
+                fast_r2c_store<T, NT>(a, data, tws, tile, tid, l, tl);
+            } else if (mode == M_DHT) {
+                fast_dht_store<T, NT>(a, data, tws, tile, tid, l, tl);
+            } else {
+                // the generic kernel's post-processing and store (column
+                // side: consecutive threads on consecutive lines)
+                const int lo = a.out_col ? tid % lt : l, k0 = a.out_col ? tid / lt : tl;
+                store_line<T, 2, NT>(a, data, tws, tile, lo, k0, tpl);
+            }
         }
         return;
     }

@@ -413,7 +413,9 @@ hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int e
     if (threads > 512) fail("line_fast: %d threads", threads);
     if (!fast_plan_available((int)sizeof(T), r0, r1, r2, em)) fail("line_fast: plan not instantiated");
     if (a.radix[0] != r0 || a.radix[1] != r1 || (r2 && a.radix[2] != r2)) fail("line_fast: radices differ");
-    if (!(a.mode == M_C2C || a.mode == M_R2C || a.mode == M_C2R || a.mode == M_DHT))
+    if (!(a.mode == M_C2C || a.mode == M_R2C || a.mode == M_C2R || a.mode == M_DHT || a.mode == M_DCT1 ||
+          a.mode == M_DST1 || a.mode == M_DCT2 || a.mode == M_DCT2_ODD || a.mode == M_DCT3 || a.mode == M_DCT3_ODD ||
+          a.mode == M_DCT4 || a.mode == M_DCT4_ODD || a.mode == M_DHT_ODD))
         fail("line_fast: mode %d", a.mode);
     if (a.mode == M_DHT && (a.flags & (F_PRE_REV | F_PRE_ALT | F_POST_REV | F_POST_ALT | F_WEIGHTS)))
         fail("line_fast: DHT with pre / post flags %d", a.flags);
@@ -977,6 +979,32 @@ void route_cases(int prec)
                 run_case<T>(name, p, nd, op, true, true, -1, 0);
             pfh_plan_destroy(p);
         }
+    }
+    // R2R kinds whose FFT length has a compile-time plan (fast first pass
+    // gathers the pre-processed values)
+    struct K {
+        std::vector<size_t> shape, extra;
+        std::vector<int> kinds;
+    };
+    const std::vector<K> fast_r2r = {{{128, 120, 63}, {}, {PFH_DCT2, PFH_DCT4, PFH_DST1}},
+                                     {{120, 64, 128}, {2}, {PFH_DST2, PFH_DCT3, PFH_DST4}},
+                                     {{65, 6, 128}, {}, {PFH_DCT1, 0, PFH_DCT3}},
+                                     {{128, 127, 2}, {}, {PFH_DST3, PFH_DST1, 0}}};
+    for (const K &c : fast_r2r) {
+        char name[128];
+        std::string sh;
+        for (size_t v : c.shape) sh += std::to_string(v) + "x";
+        pfh_desc d = route_desc(PFH_R2R, prec, c.shape, c.extra);
+        for (size_t a = 0; a < c.shape.size(); ++a) d.axis_kind[a] = c.kinds[a];
+        pfh_plan p = nullptr;
+        if (pfh_plan_create(&d, &p) != PFH_OK) {
+            fail("fast R2R %s %s: plan", pn, sh.c_str());
+            continue;
+        }
+        std::snprintf(name, sizeof(name), "fast R2R %s %s", pn, sh.c_str());
+        for (int op : {PFH_FORWARD, PFH_INVERSE, PFH_ADJOINT_FORWARD, PFH_ADJOINT_INVERSE})
+            run_case<T>(name, p, (int)c.shape.size(), op, true, true, -1, 0);
+        pfh_plan_destroy(p);
     }
     // DHT on every axis, lengths whose half has a compile-time plan
     const std::vector<C> dht_cases = {{{128, 96, 40}, {}}, {{6, 128}, {3}}, {{128, 5, 96}, {2}}};

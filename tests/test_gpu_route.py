@@ -321,6 +321,11 @@ R2R_CASES = [
     ((128, 96, 40), 1, [pf.R2rKind.Dht, pf.R2rKind.Dht, pf.R2rKind.Dht]),
     ((6, 128), 3, [pf.R2rKind.Dht, pf.R2rKind.Dht]),
     ((128, 5, 96), 1, [pf.R2rKind.Dht, None, pf.R2rKind.Dht]),
+    # R2R kinds whose FFT length has a compile-time plan (first pass gathers the pre-processed values)
+    ((128, 120, 63), 1, [pf.R2rKind.DctII, pf.R2rKind.DctIV, pf.R2rKind.DstI]),
+    ((120, 64, 128), 2, [pf.R2rKind.DstII, pf.R2rKind.DctIII, pf.R2rKind.DstIV]),
+    ((65, 6, 128), 1, [pf.R2rKind.DctI, None, pf.R2rKind.DctIII]),
+    ((128, 127, 2), 1, [pf.R2rKind.DstIII, pf.R2rKind.DstI, None]),
 ]
 
 
