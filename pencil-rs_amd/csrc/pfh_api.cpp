@@ -370,6 +370,9 @@ static void choose_fast(Step &st, int tbytes)
         case 240: r0 = 16; r1 = 3; r2 = 5; em = 16; break;
         case 320: r0 = 16; r1 = 4; r2 = 5; em = 16; break;
         case 384: r0 = 16; r1 = 8; r2 = 3; em = 16; break;
+        case 385: // route-only plan (DST-I of length 384)
+            if (r2r) { r0 = 11; r1 = 7; r2 = 5; em = 22; }
+            break;
         default: break;
         }
     }

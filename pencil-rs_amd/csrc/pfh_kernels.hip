@@ -1312,6 +1312,7 @@ hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int e
         return hipGetLastError();                                                                               \
     }
     PFH_FAST_PLANS(PFH_FAST_CASE)
+    PFH_FAST_ROUTE_PLANS(PFH_FAST_CASE)
 #undef PFH_FAST_CASE
     return hipErrorInvalidValue;
 }
@@ -1326,6 +1327,7 @@ bool fast_plan_available(int tbytes, int r0, int r1, int r2, int em)
 #define PFH_FAST_HAS(TT, A, B, C, E) \
     if ((int)sizeof(TT) == tbytes && r0 == A && r1 == B && r2 == C && em == E) return true;
     PFH_FAST_PLANS(PFH_FAST_HAS)
+    PFH_FAST_ROUTE_PLANS(PFH_FAST_HAS)
 #undef PFH_FAST_HAS
     return false;
 }
@@ -1354,6 +1356,15 @@ hipError_t set_fast_lds_limit(size_t bytes)
     }
     PFH_FAST_PLANS(PFH_FAST_ATTR)
 #undef PFH_FAST_ATTR
+#define PFH_FAST_ATTR_ROUTE(TT, A, B, C, E)                                                              \
+    {                                                                                                     \
+        hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_line_fast<TT, A, B, C, E, false>), bytes); \
+        if (e == hipSuccess)                                                                              \
+            e = raise_lds_limit(reinterpret_cast<const void *>(&k_line_fast<TT, A, B, C, E, true>), bytes); \
+        if (e != hipSuccess) return e;                                                                    \
+    }
+    PFH_FAST_ROUTE_PLANS(PFH_FAST_ATTR_ROUTE)
+#undef PFH_FAST_ATTR_ROUTE
     return hipSuccess;
 }
 

@@ -436,6 +436,7 @@ bool fast_plan_available(int tbytes, int r0, int r1, int r2, int em)
 #define PFH_FAST_HAS(TT, A, B, C, E) \
     if ((int)sizeof(TT) == tbytes && r0 == A && r1 == B && r2 == C && em == E) return true;
     PFH_FAST_PLANS(PFH_FAST_HAS)
+    PFH_FAST_ROUTE_PLANS(PFH_FAST_HAS)
 #undef PFH_FAST_HAS
     return false;
 }
