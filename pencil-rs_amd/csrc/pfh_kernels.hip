@@ -1141,8 +1141,12 @@ __device__ __forceinline__ void fast_dht_store(const StageArgs<T> &a, const cpx<
 // One tile of a fast stage: tws holds the staged tables, data the tile's
 // lines. Threads beyond the stage's lt x tpl shape (a batched launch sized
 // for its largest item) take indexes past every line and only join the
-// barriers. NT: non-temporal global loads and stores.
-template <typename T, int R0, int R1, int R2, int EM, bool NT>
+// barriers. NT: non-temporal global loads and stores. KIND: 0 = C2C lines
+// only (the batched kernels), 1 = route stages of the C2C / R2C / C2R / DHT
+// modes, 2 = route stages of the real-input R2R modes (their gathers and
+// post-processing live in an instantiation of their own, so the other
+// route kernels keep their smaller code).
+template <typename T, int R0, int R1, int R2, int EM, bool NT, int KIND = 1>
 __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, const cpx<T> *tws, cpx<T> *data)
 {
     constexpr int M = R0 * R1 * (R2 ? R2 : 1);
@@ -1154,12 +1158,8 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
     const int tl = act ? tid % tpl : (1 << 28);
     cpx<T> *ln = data + l * a.ls;
     const int mode = a.mode;
-    // real-input R2R modes (every mode but these four)
-    const bool r2r = !(mode == M_C2C || mode == M_R2C || mode == M_C2R || mode == M_DHT);
-    if (mode == M_C2R) {
-        fast_c2r_load<T, M, NT>(a, data, tws, tile, tid, l, tl);
-        fast_mid<T, R0, EM, M>(ln, tws, 1, tl, tpl);
-    } else if (r2r) {
+    if constexpr (KIND == 2) {
+        // real-input R2R modes: gathered first pass, generic post-processing
         __syncthreads(); // the pre-processing reads the staged tables
         const int fl = !act ? 0 : (a.in_col ? tid % lt : l);
         const int ft = !act ? (1 << 28) : (a.in_col ? tid / lt : tl);
@@ -1168,44 +1168,52 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
         const bool valid = act && line < a.nlines;
         if (valid) line_offsets(a.g, line, io, oo);
         fast_first_pre<T, R0, EM, M, NT>(a, data + fl * a.ls, tws, valid, io, ft, tpl);
+        if constexpr (R2 != 0) fast_mid<T, R1, EM, M>(ln, tws + a.twoff[1], R0, tl, tpl);
+        fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
+        if (act) {
+            // the generic kernel's store (column side: consecutive threads
+            // on consecutive lines)
+            const int lo = a.out_col ? tid % lt : l, k0 = a.out_col ? tid / lt : tl;
+            store_line<T, 2, NT>(a, data, tws, tile, lo, k0, tpl);
+        }
+        return;
     } else {
-        // column access: lanes across the tile's lines; row access: along a line
-        const int fl = !act ? 0 : (a.in_col ? tid % lt : l);
-        const int ft = !act ? (1 << 28) : (a.in_col ? tid / lt : tl);
+        const bool dht = KIND == 1 && mode == M_DHT;
+        if (mode == M_C2R) {
+            fast_c2r_load<T, M, NT>(a, data, tws, tile, tid, l, tl);
+            fast_mid<T, R0, EM, M>(ln, tws, 1, tl, tpl);
+        } else {
+            // column access: lanes across the tile's lines; row access: along a line
+            const int fl = !act ? 0 : (a.in_col ? tid % lt : l);
+            const int ft = !act ? (1 << 28) : (a.in_col ? tid / lt : tl);
+            const int64_t line = tile * lt + fl;
+            int64_t io = 0, oo = 0;
+            const bool valid = act && line < a.nlines;
+            if (valid) line_offsets(a.g, line, io, oo);
+            fast_first<T, R0, EM, M, NT>(a, data + fl * a.ls, valid, io, ft, tpl, mode == M_C2C,
+                                         (a.flags & F_CONJ_IN) != 0);
+        }
+        if constexpr (R2 != 0) fast_mid<T, R1, EM, M>(ln, tws + a.twoff[1], R0, tl, tpl);
+        if (mode == M_R2C || dht) {
+            fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
+            if (act) {
+                if (mode == M_R2C) fast_r2c_store<T, NT>(a, data, tws, tile, tid, l, tl);
+                else fast_dht_store<T, NT>(a, data, tws, tile, tid, l, tl);
+            }
+            return;
+        }
+        const int fl = !act ? 0 : (a.out_col ? tid % lt : l);
+        const int ft = !act ? (1 << 28) : (a.out_col ? tid / lt : tl);
         const int64_t line = tile * lt + fl;
         int64_t io = 0, oo = 0;
         const bool valid = act && line < a.nlines;
         if (valid) line_offsets(a.g, line, io, oo);
-        fast_first<T, R0, EM, M, NT>(a, data + fl * a.ls, valid, io, ft, tpl, mode == M_C2C,
-                                     (a.flags & F_CONJ_IN) != 0);
+        fast_last<T, RL, EM, M, NT>(a, data + fl * a.ls, tws + a.twoff[R2 ? 2 : 1], valid, oo, ft, tpl,
+                                    mode == M_C2R);
     }
-    if constexpr (R2 != 0) fast_mid<T, R1, EM, M>(ln, tws + a.twoff[1], R0, tl, tpl);
-    if (mode == M_R2C || mode == M_DHT || r2r) {
-        fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
-        if (act) {
-            if (mode == M_R2C) {
-                fast_r2c_store<T, NT>(a, data, tws, tile, tid, l, tl);
-            } else if (mode == M_DHT) {
-                fast_dht_store<T, NT>(a, data, tws, tile, tid, l, tl);
-            } else {
-                // the generic kernel's post-processing and store (column
-                // side: consecutive threads on consecutive lines)
-                const int lo = a.out_col ? tid % lt : l, k0 = a.out_col ? tid / lt : tl;
-                store_line<T, 2, NT>(a, data, tws, tile, lo, k0, tpl);
-            }
-        }
-        return;
-    }
-    const int fl = !act ? 0 : (a.out_col ? tid % lt : l);
-    const int ft = !act ? (1 << 28) : (a.out_col ? tid / lt : tl);
-    const int64_t line = tile * lt + fl;
-    int64_t io = 0, oo = 0;
-    const bool valid = act && line < a.nlines;
-    if (valid) line_offsets(a.g, line, io, oo);
-    fast_last<T, RL, EM, M, NT>(a, data + fl * a.ls, tws + a.twoff[R2 ? 2 : 1], valid, oo, ft, tpl, mode == M_C2R);
 }
 
-template <typename T, int R0, int R1, int R2, int EM, bool NT>
+template <typename T, int R0, int R1, int R2, int EM, bool NT, int KIND = 1>
 __global__ __launch_bounds__(512) void k_line_fast(const StageArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1219,7 +1227,7 @@ __global__ __launch_bounds__(512) void k_line_fast(const StageArgs<T> a)
         if (tile >= a.ntiles) return;
     }
     for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
-    fast_body<T, R0, R1, R2, EM, NT>(a, tile, tws, tws + a.twn);
+    fast_body<T, R0, R1, R2, EM, NT, KIND>(a, tile, tws, tws + a.twn);
 }
 
 // Persistent batched fast stage: the work list of k_line_batch, every item
@@ -1249,7 +1257,7 @@ __global__ __launch_bounds__(512) void k_line_fast_batch(const StageArgs<T> *__r
         const int64_t tile = w - prefix[q];
         __syncthreads(); // the previous tile is done with LDS
         for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
-        fast_body<T, R0, R1, R2, EM, false>(a, tile, tws, tws + a.twn);
+        fast_body<T, R0, R1, R2, EM, false, 0>(a, tile, tws, tws + a.twn);
     }
 }
 
@@ -1287,7 +1295,7 @@ __global__ __launch_bounds__(512) void k_line_multi_batch(const StageArgs<T> *__
 #define PFH_MULTI_CASE(TT, A, B, C, E)                                                                        \
     if constexpr (std::is_same<T, TT>::value && E != 32) {                                                   \
         if (r0 == A && r1 == B && r2 == C && em == E) {                                                      \
-            fast_body<T, A, B, C, E, false>(a, tile, tws, tws + a.twn);                                     \
+            fast_body<T, A, B, C, E, false, 0>(a, tile, tws, tws + a.twn);                                  \
             continue;                                                                                        \
         }                                                                                                    \
     }
@@ -1301,8 +1309,18 @@ hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int e
                             int64_t grid, hipStream_t s)
 {
     if (grid <= 0) return hipSuccess;
+    // real-input R2R modes (f32): their own instantiation, ordinary accesses
+    const bool r2r = !(a.mode == M_C2C || a.mode == M_R2C || a.mode == M_C2R || a.mode == M_DHT);
 #define PFH_FAST_CASE(TT, A, B, C, E)                                                                           \
     if constexpr (std::is_same<T, TT>::value) if (r0 == A && r1 == B && r2 == C && em == E) {                               \
+        if constexpr (std::is_same<TT, float>::value) {                                                         \
+            if (r2r) {                                                                                          \
+                hipLaunchKernelGGL((k_line_fast<T, A, B, C, E, false, 2>), dim3((unsigned)grid), dim3(threads), \
+                                   lds_bytes, s, a);                                                            \
+                return hipGetLastError();                                                                       \
+            }                                                                                                   \
+        }                                                                                                       \
+        if (r2r) return hipErrorInvalidValue;                                                                   \
         if (a.flags & F_NT)                                                                                     \
             hipLaunchKernelGGL((k_line_fast<T, A, B, C, E, true>), dim3((unsigned)grid), dim3(threads), lds_bytes, \
                                s, a);                                                                           \
@@ -1365,6 +1383,14 @@ hipError_t set_fast_lds_limit(size_t bytes)
     }
     PFH_FAST_ROUTE_PLANS(PFH_FAST_ATTR_ROUTE)
 #undef PFH_FAST_ATTR_ROUTE
+#define PFH_FAST_ATTR_R2R(TT, A, B, C, E)                                                                \
+    if constexpr (std::is_same<TT, float>::value) {                                                       \
+        hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_line_fast<TT, A, B, C, E, false, 2>), bytes); \
+        if (e != hipSuccess) return e;                                                                    \
+    }
+    PFH_FAST_PLANS(PFH_FAST_ATTR_R2R)
+    PFH_FAST_ROUTE_PLANS(PFH_FAST_ATTR_R2R)
+#undef PFH_FAST_ATTR_R2R
     return hipSuccess;
 }
 
