@@ -255,6 +255,7 @@ static void fastdiv_selftest()
     std::printf("fastdiv: %s\n", g_fail == before ? "ok" : "FAILED");
 }
 
+
 // shape checks shared by the single-stage kernels and the batch items
 template <typename T>
 static void check_shape(const char *what, const StageArgs<T> &a, int threads, size_t lds, int emax, bool fast,
@@ -873,6 +874,27 @@ pfh_desc route_desc(int kind, int prec, std::vector<size_t> shape, std::vector<s
     return d;
 }
 
+// Plans whose stages would index more than 2^31 - 1 lines (the kernels'
+// 32-bit line division) are refused at creation; 2^30 lines still plan.
+static void line_count_guard()
+{
+    const int before = g_fail;
+    for (size_t ex : {256, 512}) {
+        pfh_desc d = route_desc(PFH_C2C, PFH_F32, {2048, 2048, 64}, {ex});
+        pfh_plan p = nullptr;
+        const pfh_status st = pfh_plan_create(&d, &p);
+        if (ex == 256 && st != PFH_OK) fail("line-count guard: 2^30 lines refused (%d)", (int)st);
+        if (ex == 512) {
+            pfh_error_info e;
+            pfh_last_error(&e);
+            if (st != PFH_ERR_UNSUPPORTED || std::strstr(e.message, "2^31") == nullptr)
+                fail("line-count guard: 2^31 lines gave status %d (%s)", (int)st, e.message);
+        }
+        if (p) pfh_plan_destroy(p);
+    }
+    std::printf("line-count guard: %s\n", g_fail == before ? "ok" : "FAILED");
+}
+
 template <typename T>
 void route_cases(int prec)
 {
@@ -1154,6 +1176,7 @@ int main(int argc, char **argv)
     // 1 MiB slab intermediates so that the small shapes here are pipelined
     setenv("PFH_SLAB_MB", "1", 0);
     pfh::fastdiv_selftest();
+    line_count_guard();
     route_cases<double>(PFH_F64);
     route_cases<float>(PFH_F32);
     batch_cases();
