@@ -375,6 +375,10 @@ static void choose_fast(Step &st, int tbytes)
             break;
         default: break;
         }
+    } else if (st.mode == M_R2C || st.mode == M_C2R || dht) {
+        // route-only f64 plans (never C2C, so never a batched line)
+        if (m == 120) { r0 = 8; r1 = 3; r2 = 5; em = 8; }
+        if (m == 192) { r0 = 8; r1 = 8; r2 = 3; em = 8; }
     }
     if (r0 == 0 && (m < 64 || (m & (m - 1)) != 0)) return;
     if (r0 != 0) {

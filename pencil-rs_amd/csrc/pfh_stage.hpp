@@ -154,9 +154,12 @@ struct StageArgs {
 // Instantiated fast plans: {R0, R1, R2, EM}.
 // Compile-time plans for route stages only (the batched kernels, which
 // carry C2C lines, do not instantiate them): the 385-point FFT of a DST-I
-// line of length 384 (5 x 7 x 11, 22 values per thread: 32 threads a line).
+// line of length 384 (5 x 7 x 11, 22 values per thread: 32 threads a line)
+// and the f64 half-length FFTs of 240- and 384-point R2C / C2R / DHT lines.
 #define PFH_FAST_ROUTE_PLANS(X) \
-    X(float, 11, 7, 5, 22)
+    X(float, 11, 7, 5, 22)      \
+    X(double, 8, 3, 5, 8)       \
+    X(double, 8, 8, 3, 8)
 
 #define PFH_FAST_PLANS(X)      \
     X(float, 8, 8, 0, 16)      \
