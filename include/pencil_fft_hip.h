@@ -94,7 +94,10 @@ typedef enum pfh_status {
     PFH_ERR_IO_INVALID_NAME = 40,         /* NamedIoError::InvalidName */
     PFH_ERR_IO_DUPLICATE_NAME = 41,       /* NamedIoError::DuplicateName */
     PFH_ERR_IO_NOT_FOUND = 42,            /* NamedIoError::NotFound */
-    PFH_ERR_IO_INVALID_TAIL = 43          /* NamedIoError::InvalidTail */
+    PFH_ERR_IO_INVALID_TAIL = 43,         /* NamedIoError::InvalidTail */
+    PFH_ERR_POISONED = 44,                /* ArrayError::Poisoned (pencil-array error.rs:158, many.rs:234) */
+    PFH_ERR_ALIASED_BUFFERS = 45          /* overlapping source / destination / workspace (no Rust analogue:
+                                             the borrow rules forbid it) */
 } pfh_status;
 
 /* Payload of the last error on the calling thread (Rust error fields). */
@@ -226,11 +229,26 @@ void pfh_plan_destroy(pfh_plan plan);
 /* Workspace bytes needed by op (TransposeWorkspaceRequirements,
  * transpose.rs:226-254, scratch_len / embedding_len of the local plans). The
  * workspace also holds the plan's small device status area: fill a new
- * workspace with zeros once before its first use (the library keeps the
- * status area consistent afterwards). Besides the route's intermediate
- * arrays it holds the slab intermediate of slab-pipelined stage pairs, which
- * are off unless PFH_SLAB_MB in the environment sets a slab size in MiB. */
+ * workspace with zeros once before its first use, or call
+ * pfh_workspace_init (the library keeps the status area consistent
+ * afterwards). Besides the route's intermediate arrays it holds the slab
+ * intermediate of slab-pipelined stage pairs, which are off unless
+ * PFH_SLAB_MB in the environment sets a slab size in MiB.
+ *
+ * Binding (the reference's workspaces belong to the plan that made them,
+ * distributed.rs:2188-2191, :4015-4017): a distributed plan's workspace is
+ * bound to the first plan that uses it (or to the plan given to
+ * pfh_workspace_init); any other plan then gets PFH_ERR_WORKSPACE_MISMATCH
+ * before any device work. The binding is recorded in the workspace itself, so
+ * a workspace refilled with zeros (a new allocation) is unbound again. */
 pfh_status pfh_plan_workspace_bytes(pfh_plan plan, int op, size_t *bytes);
+
+/* Binds a workspace to plan and resets its status area: clears the sticky
+ * status and the in-place poisoned state (the reference's equivalent is a new
+ * workspace and a new in-place array from plan.workspace() / the array
+ * constructors). workspace_bytes must cover the status area (256 bytes);
+ * stream ordered, no synchronization. Distributed plans only. */
+pfh_status pfh_workspace_init(pfh_plan plan, void *workspace, size_t workspace_bytes, pfh_stream_t stream);
 
 /* Layout of the plan's input (side 0) or output (side 1) array for the
  * forward direction: input_pencil() / output_pencil()
@@ -240,7 +258,8 @@ pfh_status pfh_plan_workspace_bytes(pfh_plan plan, int op, size_t *bytes);
 pfh_status pfh_plan_layout(pfh_plan plan, int side, int perm[PFH_MAX_DIMS], size_t shape[PFH_MAX_DIMS],
                            size_t *elements, size_t *element_bytes);
 
-/* Distributed-plan execution, out of place, source preserved:
+/* Distributed-plan execution, out of place, source preserved (src, dst and
+ * the workspace must not overlap: PFH_ERR_ALIASED_BUFFERS):
  * C2cPlan::{forward, inverse, backward} (distributed.rs:1448-1500),
  * R2cPlan::{forward, inverse, backward} (distributed/r2c.rs:856-1000),
  * R2rPlan/DhtPlan::{forward, inverse, backward} (distributed/r2r.rs:1271+),
@@ -253,7 +272,18 @@ pfh_status pfh_execute(pfh_plan plan, int op, const void *src, size_t src_elems,
 
 /* In-place execution in one buffer of capacity_bytes (C2cPlan::*_in_place,
  * distributed.rs:3672-3774; R2cPlan in-place arrays; local *_in_place
- * methods; LocalTransposePlan::execute_in_place, local_transpose.rs:159-243). */
+ * methods; LocalTransposePlan::execute_in_place, local_transpose.rs:159-243).
+ *
+ * Failure state (run_in_place_transaction, distributed.rs:3776-3794): the
+ * reference's in-place array state lives here in the workspace. A call that
+ * fails after it started writing `data` (an inverse whose spectrum the C2R
+ * boundary check rejects: the complex stages before the check have already
+ * overwritten data) poisons the workspace; every later in-place call with it
+ * returns PFH_ERR_POISONED and writes nothing, until pfh_workspace_init
+ * resets it. With PFH_EXEC_DEFER_STATUS the failure is reported by
+ * pfh_workspace_take_status (or by the next synchronous in-place call), and
+ * in-place calls issued before that write nothing. Out-of-place calls are
+ * not affected by the poisoned state. */
 pfh_status pfh_execute_in_place(pfh_plan plan, int op, void *data, size_t capacity_bytes, void *workspace,
                                 size_t workspace_bytes, pfh_stream_t stream, int flags);
 

@@ -175,6 +175,94 @@ static void pfo_unpack_region(int N, const size_t *shape, size_t extra, size_t e
 }
 
 /* ---------------------------------------------------------------------- */
+/* C2R boundary acceptance thresholds                                       */
+/* ---------------------------------------------------------------------- */
+
+/* ceil(log2(n)) for n >= 1; 0 for n <= 1 (ceil_log2_len, mixed.rs:4076-4082;
+ * usize::BITS - (n - 1).leading_zeros(), r2c.rs:4848-4850). */
+static int pfo_ceil_log2(size_t n)
+{
+    int bits = 0;
+    if (n <= 1) return 0;
+    for (size_t v = n - 1; v; v >>= 1) ++bits;
+    return bits;
+}
+
+/* Error-model length E_a, raw composition factor and allowance of one
+ * complex-suffix axis of a MixedR2cPlan (mixed.rs:408-417, :4093-4108):
+ * FFT (n, n, 0); R2R: E_a = 2(n-1) DCT-I, 2(n+1) DST-I, 4n II/III, 8n IV,
+ * factor = the R2R normalization factor (2(n-1), 2(n+1), 2n: r2r.rs:1222-1228),
+ * allowance 2; DHT (n, n, 2). Returns 0 for the identity. */
+int pfo_mixed_axis_model(int code, size_t n, size_t *E, size_t *factor, int *allowance)
+{
+    *allowance = 2;
+    switch (code) {
+    case PFO_AXIS_FFT: *E = n; *factor = n; *allowance = 0; return 1;
+    case PFO_DCT1: *E = 2 * (n - 1); *factor = 2 * (n - 1); return 1;
+    case PFO_DST1: *E = 2 * (n + 1); *factor = 2 * (n + 1); return 1;
+    case PFO_DCT2: case PFO_DCT3: case PFO_DST2: case PFO_DST3: *E = 4 * n; *factor = 2 * n; return 1;
+    case PFO_DCT4: case PFO_DST4: *E = 8 * n; *factor = 2 * n; return 1;
+    case PFO_DHT: *E = n; *factor = n; return 1;
+    default: *E = 0; *factor = 1; *allowance = 0; return 0;
+    }
+}
+
+/* Thresholds of the C2R boundary check of plan D for route operation op
+ * (PFO_OP_INVERSE: normalized; PFO_OP_BACKWARD: raw), precision 4 (f32) or
+ * 8 (f64):
+ *  - R2C / C2R orientation: depth = 1 + sum ceil_log2(n_a) over the selected
+ *    axes other than the reduction axis with n_a > 1 (normalization_depth,
+ *    r2c.rs:4841-4853); raw absolute = inverse absolute * prod n_a over the
+ *    same axes, extents of 1 included (raw_absolute_threshold_for_shape,
+ *    r2c.rs:4855-4878);
+ *  - MixedR2cPlan: depth = 1 + sum (ceil_log2(E_a) + allowance_a), raw
+ *    absolute scaled by prod factor_a, over the non-identity axes before the
+ *    RFFT axis (mixed_depth_normalization, mixed.rs:4085-4117);
+ *  - relative = 128 * epsilon * depth, absolute = 128 * min_subnormal * depth
+ *    with epsilon and the smallest positive subnormal of the real type taken
+ *    to f64 (r2c.rs:4737-4742; lib.rs:255-281: f32::EPSILON, f32::from_bits(1)).
+ * Returns PFO_ERR_ARG for a plan without a real boundary. */
+int pfo_endpoint_thresholds(const pfo_desc *D, int precision, int op, double *relative, double *absolute,
+                            double *depth_out)
+{
+    const int N = D->ndim;
+    int raxis = -1;
+    if (N < 1 || N > PFO_MAXD) return PFO_ERR_ARG;
+    if (D->kind == PFO_KIND_R2C) {
+        for (int a = N - 1; a >= 0; --a) if (D->selected & (1u << a)) { raxis = a; break; }
+    } else if (D->kind == PFO_KIND_C2R) {
+        for (int a = 0; a < N; ++a) if (D->selected & (1u << a)) { raxis = a; break; }
+    } else if (D->kind == PFO_KIND_MIXED_R2C) {
+        for (int a = 0; a < N; ++a) if (D->r2r[a] == PFO_AXIS_RFFT) { raxis = a; break; }
+    }
+    if (raxis < 0) return PFO_ERR_ARG;
+    double depth = 1.0, transverse = 1.0;
+    if (D->kind == PFO_KIND_MIXED_R2C) {
+        for (int a = 0; a < raxis; ++a) {
+            size_t E, f;
+            int allow;
+            if (!pfo_mixed_axis_model(D->r2r[a], D->shape[a], &E, &f, &allow)) continue;
+            depth += pfo_ceil_log2(E) + allow;
+            transverse *= (double)f;
+        }
+    } else {
+        for (int a = 0; a < N; ++a) {
+            if (a == raxis || !(D->selected & (1u << a))) continue;
+            depth += pfo_ceil_log2(D->shape[a]);
+            transverse *= (double)D->shape[a];
+        }
+    }
+    /* f32::EPSILON = 2^-23, f32::from_bits(1) = 2^-149; f64: 2^-52, 2^-1074 */
+    const double eps = precision == 4 ? ldexp(1.0, -23) : ldexp(1.0, -52);
+    const double minsub = precision == 4 ? ldexp(1.0, -149) : ldexp(1.0, -1074);
+    *relative = 128.0 * eps * depth;
+    *absolute = 128.0 * minsub * depth;
+    if (op == PFO_OP_BACKWARD) *absolute *= transverse;
+    if (depth_out) *depth_out = depth;
+    return PFO_OK;
+}
+
+/* ---------------------------------------------------------------------- */
 /* Typed instantiations                                                      */
 /* ---------------------------------------------------------------------- */
 

@@ -129,6 +129,17 @@ int pfo_endpoint_accept_f64(int ndim, const int *perm, const size_t *rshape, siz
 int pfo_endpoint_accept_f32(int ndim, const int *perm, const size_t *rshape, size_t extra, int axis,
                             size_t real_len, const float *data, double relative, double absolute);
 
+/* C2R boundary thresholds of plan d for op (PFO_OP_INVERSE normalized,
+ * PFO_OP_BACKWARD raw) in precision 4 (f32) or 8 (f64): relative,
+ * absolute and the error-model depth (normalization_depth /
+ * raw_absolute_threshold_for_shape, r2c.rs:4841-4878; MixedR2cPlan,
+ * mixed.rs:400-427, :4085-4117). */
+int pfo_endpoint_thresholds(const pfo_desc *d, int precision, int op, double *relative, double *absolute,
+                            double *depth);
+/* Error model of one MixedR2cPlan suffix axis: E_a, factor_a, allowance_a
+ * (mixed.rs:408-417); 0 for the identity. */
+int pfo_mixed_axis_model(int code, size_t n, size_t *E, size_t *factor, int *allowance);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,10 @@ def lib():
             getattr(_lib, name).argtypes = [ctypes.c_int, vp, vp, sz, ctypes.c_int, sz, vp,
                                             ctypes.c_double, ctypes.c_double]
         _lib.pfo_build_route.argtypes = [ctypes.c_int, ctypes.c_int, vp]
+        dp = ctypes.POINTER(ctypes.c_double)
+        _lib.pfo_endpoint_thresholds.argtypes = [ctypes.POINTER(_Desc), ctypes.c_int, ctypes.c_int, dp, dp, dp]
+        _lib.pfo_mixed_axis_model.argtypes = [ctypes.c_int, sz, ctypes.POINTER(sz), ctypes.POINTER(sz),
+                                              ctypes.POINTER(ctypes.c_int)]
     return _lib
 
 
@@ -275,18 +279,76 @@ def endpoint_accept(data: np.ndarray, perm, rshape, extra, axis, real_len, relat
                    real_len, _ptr(data), relative, absolute))
 
 
-def endpoint_thresholds(dtype, shape, selected, reduction_axis, normalize):
-    """(relative, absolute) acceptance thresholds, distributed/r2c.rs:4841-4878."""
-    eps = float(np.finfo(np.float32).eps) if _real_dtype(dtype) == np.float32 else float(np.finfo(np.float64).eps)
-    minsub = float(np.float32(1.4e-45)) if _real_dtype(dtype) == np.float32 else 5e-324
+def _min_subnormal(dtype) -> float:
+    """The smallest positive subnormal of the real type, as f64
+    (f32::from_bits(1) / f64::from_bits(1), lib.rs:264-280)."""
+    bits = np.array([1], dtype=np.uint32 if _real_dtype(dtype) == np.float32 else np.uint64)
+    return float(bits.view(np.float32 if _real_dtype(dtype) == np.float32 else np.float64)[0])
+
+
+def mixed_axis_model(code: int, n: int):
+    """(E_a, factor_a, allowance_a) of one MixedR2cPlan complex-suffix axis
+    (mixed.rs:408-417): FFT (n, n, 0); DCT-I 2(n-1), DST-I 2(n+1), types II/III
+    4n, type IV 8n with the R2R normalization factor (2(n-1), 2(n+1), 2n) and
+    allowance 2; DHT (n, n, 2). None for the identity."""
+    if code == AXIS_FFT:
+        return n, n, 0
+    if code == DCT1:
+        return 2 * (n - 1), 2 * (n - 1), 2
+    if code == DST1:
+        return 2 * (n + 1), 2 * (n + 1), 2
+    if code in (DCT2, DCT3, DST2, DST3):
+        return 4 * n, 2 * n, 2
+    if code in (DCT4, DST4):
+        return 8 * n, 2 * n, 2
+    if code == DHT:
+        return n, n, 2
+    return None
+
+
+def endpoint_thresholds(dtype, shape, selected, reduction_axis, normalize, mixed=None):
+    """(relative, absolute) acceptance thresholds of the C2R boundary check
+    (normalization_depth / raw_absolute_threshold_for_shape,
+    distributed/r2c.rs:4841-4878; for a MixedR2cPlan pass its per-axis codes
+    as mixed: mixed_depth_normalization, mixed.rs:4085-4117)."""
+    eps = float(np.finfo(_real_dtype(dtype)).eps)
+    minsub = _min_subnormal(dtype)
     depth, transverse = 1.0, 1.0
-    for a, n in enumerate(shape):
-        if a != reduction_axis and a in selected:
-            if n > 1:
-                depth += (n - 1).bit_length()
-            transverse *= float(n)
+    if mixed is not None:
+        for a in range(reduction_axis):
+            m = mixed_axis_model(mixed[a], shape[a])
+            if m is None:
+                continue
+            E, factor, allowance = m
+            depth += (E - 1).bit_length() + allowance
+            transverse *= float(factor)
+    else:
+        for a, n in enumerate(shape):
+            if a != reduction_axis and a in selected:
+                if n > 1:
+                    depth += (n - 1).bit_length()
+                transverse *= float(n)
     relative = 128.0 * eps * depth
     absolute = 128.0 * minsub * depth
     if not normalize:
         absolute *= transverse
     return relative, absolute
+
+
+def endpoint_thresholds_c(kind, dtype, shape, op, selected=None, r2r=None):
+    """The C restatement's thresholds for plan kind and route op (INVERSE or
+    BACKWARD): (relative, absolute, depth)."""
+    desc = make_desc(kind, shape, 1, selected, r2r, None, True)
+    rel, ab, dep = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    st = lib().pfo_endpoint_thresholds(ctypes.byref(desc), 4 if _real_dtype(dtype) == np.float32 else 8, op,
+                                       ctypes.byref(rel), ctypes.byref(ab), ctypes.byref(dep))
+    if st != OK:
+        raise ValueError("oracle thresholds: no real boundary")
+    return rel.value, ab.value, dep.value
+
+
+def mixed_axis_model_c(code: int, n: int):
+    E, f, al = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_int()
+    if not lib().pfo_mixed_axis_model(code, n, ctypes.byref(E), ctypes.byref(f), ctypes.byref(al)):
+        return None
+    return E.value, f.value, al.value

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -18,6 +19,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
 #include "pfh_collectives.hpp"
@@ -1376,8 +1378,11 @@ static bool valid_perm(int N, const int *p)
     return true;
 }
 
+static std::atomic<uint64_t> g_next_tag{1};
+
 static pfh_status create_impl(const pfh_desc *d, PlanImpl &P)
 {
+    P.tag = g_next_tag.fetch_add(1);
     P.desc = *d;
     P.kind = d->kind;
     P.precision = d->precision;
@@ -1544,7 +1549,102 @@ struct WsView {
     EndpointPartial *ep = nullptr;
     void *w[2] = {nullptr, nullptr};
     void *t = nullptr; // slab intermediate
+    // in-place calls: every stage reads the gate (set from the poison word by
+    // the call's first kernel), and a rejected spectrum poisons the workspace
+    int *poison = nullptr;
+    bool gate_all = false;
 };
+
+static void status_view(void *ws, WsView &v)
+{
+    char *b = static_cast<char *>(ws);
+    v.status = reinterpret_cast<int *>(b + kStatusOffStatus);
+    v.gate = reinterpret_cast<int *>(b + kStatusOffGate);
+    v.first_code = reinterpret_cast<unsigned long long *>(b + kStatusOffFirst);
+}
+
+// ---------------------------------------------------------------------------
+// Workspace binding and in-place failure state.
+//
+// The reference binds a workspace to the plan that made it and rejects any
+// other (Arc::ptr_eq, distributed.rs:2188-2191, :4015-4017), and marks an
+// in-place array Poisoned once an operation fails after its first write
+// (run_in_place_transaction, distributed.rs:3776-3794; many.rs:25). Here the
+// workspace is caller memory: its status area holds the tag of the plan it
+// is bound to (written at its first use, or by pfh_workspace_init) and the
+// poison word. The host keeps what it last saw per workspace address, so the
+// device tag is read (one synchronous 16-byte copy) only when a workspace is
+// used with a plan other than the one it was last seen with.
+// ---------------------------------------------------------------------------
+
+struct WsEntry {
+    uint64_t tag = 0;
+    bool poisoned = false;
+    bool pending = false; // an in-place call with a device check ran with deferred status
+};
+static std::mutex g_ws_mu;
+static std::unordered_map<uintptr_t, WsEntry> g_ws;
+
+static pfh_status bind_workspace(const PlanImpl &P, void *ws, hipStream_t stream)
+{
+    const uintptr_t key = reinterpret_cast<uintptr_t>(ws);
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto it = g_ws.find(key);
+    if (it != g_ws.end() && it->second.tag == P.tag) return PFH_OK;
+    PFH_HIP_CHECK(hipStreamSynchronize(stream));
+    unsigned char area[32];
+    PFH_HIP_CHECK(hipMemcpy(area, ws, sizeof(area), hipMemcpyDeviceToHost));
+    uint64_t tag;
+    int poison;
+    std::memcpy(&tag, area + kStatusOffTag, sizeof(tag));
+    std::memcpy(&poison, area + kStatusOffPoison, sizeof(poison));
+    if (tag != 0 && tag != P.tag)
+        return set_err(PFH_ERR_WORKSPACE_MISMATCH, "workspace is bound to another plan");
+    if (tag == 0) {
+        PFH_HIP_CHECK(launch_fill_u64(reinterpret_cast<unsigned long long *>(static_cast<char *>(ws) + kStatusOffTag),
+                                      1, (unsigned long long)P.tag, stream));
+    }
+    WsEntry e;
+    e.tag = P.tag;
+    e.poisoned = poison != 0;
+    g_ws[key] = e;
+    return PFH_OK;
+}
+
+static bool ws_poisoned(const void *ws)
+{
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto it = g_ws.find(reinterpret_cast<uintptr_t>(ws));
+    return it != g_ws.end() && it->second.poisoned;
+}
+
+static void ws_set_poisoned(const void *ws, bool v)
+{
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    g_ws[reinterpret_cast<uintptr_t>(ws)].poisoned = v;
+}
+
+static bool ws_pending(const void *ws)
+{
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto it = g_ws.find(reinterpret_cast<uintptr_t>(ws));
+    return it != g_ws.end() && it->second.pending;
+}
+
+static void ws_set_pending(const void *ws, bool v)
+{
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto it = g_ws.find(reinterpret_cast<uintptr_t>(ws));
+    if (it != g_ws.end()) it->second.pending = v;
+}
+
+// [a, a + na) and [b, b + nb) share a byte
+static bool overlaps(const void *a, size_t na, const void *b, size_t nb)
+{
+    if (!a || !b || na == 0 || nb == 0) return false;
+    const uintptr_t x = reinterpret_cast<uintptr_t>(a), y = reinterpret_cast<uintptr_t>(b);
+    return x < y + nb && y < x + na;
+}
 
 // Workspace of one operation: status area, W0, W1, slab intermediate T.
 static int64_t op_ws_bytes(const PlanImpl &P, const OpPlan &o)
@@ -1558,9 +1658,7 @@ static pfh_status view_workspace(const PlanImpl &P, const OpPlan &o, void *ws, s
     if (ws == nullptr || (int64_t)ws_bytes < need)
         return set_err(PFH_ERR_WORKSPACE_TOO_SMALL, "workspace too small", (size_t)need, ws ? ws_bytes : 0);
     char *b = static_cast<char *>(ws);
-    v.status = reinterpret_cast<int *>(b);
-    v.gate = reinterpret_cast<int *>(b + 4);
-    v.first_code = reinterpret_cast<unsigned long long *>(b + 8);
+    status_view(ws, v);
     v.ep = reinterpret_cast<EndpointPartial *>(b + kStatusBytes);
     char *w = b + status_area_bytes(P);
     v.w[0] = w;
@@ -1624,12 +1722,23 @@ static StageArgs<T> fill_args(const PlanImpl &P, const Step &s, void *const bufs
     return a;
 }
 
+// The C2R boundary check of a step; in-place calls keep the gate that their
+// first kernel set and poison the workspace on a rejected spectrum.
+template <typename T>
+static hipError_t launch_check(const Step &s, const void *in, const WsView &wv, hipStream_t stream)
+{
+    EndpointArgs e = s.ep;
+    e.keep_gate = wv.gate_all ? 1 : 0;
+    e.poison = wv.poison;
+    return launch_endpoint_check<T>(e, in, wv.ep, wv.gate, wv.status, stream);
+}
+
 template <typename T>
 static pfh_status launch_step_ptrs(const PlanImpl &P, const Step &s, const void *in, void *out, const WsView &wv,
                                    hipStream_t stream)
 {
     if (s.endpoint_check) {
-        PFH_HIP_CHECK(launch_endpoint_check<T>(s.ep, in, wv.ep, wv.gate, wv.status, stream));
+        PFH_HIP_CHECK(launch_check<T>(s, in, wv, stream));
     }
     if (s.local_endpoint_check) {
         PFH_HIP_CHECK(launch_local_endpoint_check<T>(in, s.nlines, s.nin, (s.n % 2 == 0) ? 1 : 0, wv.first_code,
@@ -1639,8 +1748,8 @@ static pfh_status launch_step_ptrs(const PlanImpl &P, const Step &s, const void 
     Step sp = s;
     sp.in_buf = 0;
     sp.out_buf = 1;
-    const StageArgs<T> a =
-        fill_args<T>(P, sp, io, (s.endpoint_check || s.local_endpoint_check || s.gated) ? wv.gate : nullptr);
+    const bool gated = s.endpoint_check || s.local_endpoint_check || s.gated || wv.gate_all;
+    const StageArgs<T> a = fill_args<T>(P, sp, io, gated ? wv.gate : nullptr);
     if (s.fast) {
         PFH_HIP_CHECK(launch_line_fast<T>(a, s.fr[0], s.fr[1], s.fr[2], s.fem, s.threads, s.lds_bytes, s.grid, stream));
         return PFH_OK;
@@ -1739,11 +1848,9 @@ static pfh_status run_op(const PlanImpl &P, const OpPlan &o, void *const bufs[4]
             Step chk = s;
             chk.gated = true;
             if (P.precision == PFH_F32) {
-                if (s.endpoint_check)
-                    PFH_HIP_CHECK(launch_endpoint_check<float>(s.ep, bufs[s.in_buf], wv.ep, wv.gate, wv.status, stream));
+                if (s.endpoint_check) PFH_HIP_CHECK(launch_check<float>(s, bufs[s.in_buf], wv, stream));
             } else {
-                if (s.endpoint_check)
-                    PFH_HIP_CHECK(launch_endpoint_check<double>(s.ep, bufs[s.in_buf], wv.ep, wv.gate, wv.status, stream));
+                if (s.endpoint_check) PFH_HIP_CHECK(launch_check<double>(s, bufs[s.in_buf], wv, stream));
             }
             if (tr->mark(stream, idx)) return PFH_ERR_DEVICE;
             TimingTrace::Entry ec;
@@ -1777,26 +1884,36 @@ static bool op_has_device_check(const OpPlan &o)
     return false;
 }
 
-// Reads (and clears) the sticky status after a synchronous execution.
+// Reads (and clears) the sticky status after a synchronous execution, and
+// records a poisoned workspace (the poison word itself stays set).
 static pfh_status collect_status(const PlanImpl &P, const WsView &wv, hipStream_t stream)
 {
     PFH_HIP_CHECK(hipStreamSynchronize(stream));
-    int status = 0;
+    unsigned char area[32];
     unsigned long long code = ~0ull;
-    PFH_HIP_CHECK(hipMemcpy(&status, wv.status, sizeof(int), hipMemcpyDeviceToHost));
-    if (status & PFH_DEVICE_INVALID_ENDPOINT)
-        PFH_HIP_CHECK(hipMemcpy(&code, wv.first_code, sizeof(code), hipMemcpyDeviceToHost));
+    PFH_HIP_CHECK(hipMemcpy(area, wv.status, sizeof(area), hipMemcpyDeviceToHost));
+    int status, poison;
+    std::memcpy(&status, area + kStatusOffStatus, sizeof(status));
+    std::memcpy(&poison, area + kStatusOffPoison, sizeof(poison));
+    std::memcpy(&code, area + kStatusOffFirst, sizeof(code));
     if (status) {
         PFH_HIP_CHECK(launch_fill_i32(wv.status, 1, 0, stream));
         PFH_HIP_CHECK(hipStreamSynchronize(stream));
     }
+    if (poison) ws_set_poisoned(wv.status, true);
+    ws_set_pending(wv.status, false);
     if (status & PFH_DEVICE_INVALID_ENDPOINT) {
         const size_t batch = (size_t)(code / 2);
         const size_t index = (code & 1) ? (size_t)(P.line_len / 2) : 0;
         return set_err(PFH_ERR_INVALID_SPECTRUM_ENDPOINT, "non-zero imaginary endpoint", 0, 0, batch, index);
     }
     if (status & PFH_DEVICE_INVALID_SPECTRUM)
-        return set_err(PFH_ERR_INVALID_SPECTRUM, "inverse spectrum has an invalid constrained boundary plane");
+        return set_err(PFH_ERR_INVALID_SPECTRUM, (status & PFH_DEVICE_POISONED)
+                                                     ? "inverse spectrum has an invalid constrained boundary plane; "
+                                                       "later in-place calls on the poisoned workspace wrote nothing"
+                                                     : "inverse spectrum has an invalid constrained boundary plane");
+    if (status & PFH_DEVICE_POISONED)
+        return set_err(PFH_ERR_POISONED, "an earlier in-place call failed after it started: the call wrote nothing");
     return ok();
 }
 
@@ -1818,6 +1935,8 @@ static pfh_status execute_impl(PlanImpl &P, int op, const void *src, size_t src_
         return set_err(is_tr ? PFH_ERR_DESTINATION_LAYOUT_MISMATCH : PFH_ERR_OUTPUT_LAYOUT_MISMATCH, "destination length",
                        (size_t)o.out.total(), dst_elems);
     if (o.in.total() == 0 && o.out.total() == 0) return ok();
+    if (is_tr && overlaps(src, (size_t)o.in.bytes(), dst, (size_t)o.out.bytes()))
+        return set_err(PFH_ERR_ALIASED_BUFFERS, "source and destination overlap");
     if (is_tr && o.steps.empty()) {
         // generic word permutation
         PermuteArgs pa{};
@@ -1852,6 +1971,19 @@ static pfh_status execute_impl(PlanImpl &P, int op, const void *src, size_t src_
     const bool needs_ws = !is_tr;
     if (needs_ws) {
         pfh_status st = view_workspace(P, o, ws, ws_bytes, wv);
+        if (st) return st;
+    }
+    // Rust's borrow rules keep source, destination and workspace apart; here
+    // overlapping buffers are refused before any work
+    if (overlaps(src, (size_t)o.in.bytes(), dst, (size_t)o.out.bytes()))
+        return set_err(PFH_ERR_ALIASED_BUFFERS, "source and destination overlap");
+    if (needs_ws) {
+        const size_t wb = (size_t)op_ws_bytes(P, o);
+        if (overlaps(src, (size_t)o.in.bytes(), ws, wb))
+            return set_err(PFH_ERR_ALIASED_BUFFERS, "source and workspace overlap");
+        if (overlaps(dst, (size_t)o.out.bytes(), ws, wb))
+            return set_err(PFH_ERR_ALIASED_BUFFERS, "destination and workspace overlap");
+        pfh_status st = bind_workspace(P, ws, stream);
         if (st) return st;
     }
     void *bufs[4] = {const_cast<void *>(src), dst, wv.w[0], wv.w[1]};
@@ -1951,6 +2083,8 @@ const char *pfh_status_string(pfh_status s)
     case PFH_ERR_IO_DUPLICATE_NAME: return "dataset name already exists";
     case PFH_ERR_IO_NOT_FOUND: return "dataset name was not found";
     case PFH_ERR_IO_INVALID_TAIL: return "dataset container has an invalid tail";
+    case PFH_ERR_POISONED: return "array is poisoned by an incomplete in-place operation";
+    case PFH_ERR_ALIASED_BUFFERS: return "source, destination and workspace must not overlap";
     }
     return "unknown status";
 }
@@ -2046,6 +2180,8 @@ pfh_status pfh_execute_in_place(pfh_plan plan, int op, void *data, size_t capaci
             return set_err(PFH_ERR_SOURCE_LAYOUT_MISMATCH, "data too small", need, capacity_bytes);
         if (!workspace || workspace_bytes < need)
             return set_err(PFH_ERR_SCRATCH_TOO_SMALL, "scratch too small", need, workspace ? workspace_bytes : 0);
+        if (overlaps(workspace, need, data, need))
+            return set_err(PFH_ERR_ALIASED_BUFFERS, "data and scratch overlap");
         PFH_HIP_CHECK(hipMemcpyAsync(workspace, data, need, hipMemcpyDeviceToDevice, stream));
         return execute_impl(P, op == PFH_BACKWARD ? PFH_BACKWARD : PFH_FORWARD, workspace, (size_t)o.in.total(), data,
                             (size_t)o.out.total(), nullptr, 0, stream, flags, nullptr);
@@ -2061,10 +2197,55 @@ pfh_status pfh_execute_in_place(pfh_plan plan, int op, void *data, size_t capaci
     WsView wv;
     pfh_status st = view_workspace(P, o, workspace, workspace_bytes, wv);
     if (st) return st;
+    if (overlaps(data, (size_t)need, workspace, (size_t)op_ws_bytes(P, o)))
+        return set_err(PFH_ERR_ALIASED_BUFFERS, "data and workspace overlap");
+    st = bind_workspace(P, workspace, stream);
+    if (st) return st;
+    // ArrayError::Poisoned (many.rs:234): an earlier in-place call on this
+    // workspace failed after it started writing its array
+    if (ws_poisoned(workspace))
+        return set_err(PFH_ERR_POISONED, "an earlier in-place call on this workspace failed after it started");
+    // a failure found on the device by an earlier deferred call is not known
+    // here yet: the call's first kernel gates every stage on the poison word
+    wv.poison = reinterpret_cast<int *>(static_cast<char *>(workspace) + kStatusOffPoison);
+    wv.gate_all = true;
+    PFH_HIP_CHECK(launch_inplace_begin(wv.gate, wv.poison, wv.status, stream));
     void *bufs[4] = {data, data, wv.w[0], wv.w[1]};
     st = run_op(P, o, bufs, wv, stream, nullptr);
-    if (st) return st;
-    if (op_has_device_check(o) && !(flags & PFH_EXEC_DEFER_STATUS)) return collect_status(P, wv, stream);
+    if (st) {
+        // a launch failed after the call started writing the array
+        (void)launch_fill_i32(wv.poison, 1, 1, stream);
+        ws_set_poisoned(workspace, true);
+        return st;
+    }
+    // the status is read when this call checks the spectrum on the device, or
+    // when an earlier deferred in-place call may have poisoned the workspace
+    // (this call's stages were then gated off)
+    const bool check = op_has_device_check(o);
+    if (flags & PFH_EXEC_DEFER_STATUS) {
+        if (check) ws_set_pending(workspace, true);
+        return ok();
+    }
+    if (check || ws_pending(workspace)) return collect_status(P, wv, stream);
+    return ok();
+}
+
+pfh_status pfh_workspace_init(pfh_plan plan, void *workspace, size_t workspace_bytes, pfh_stream_t stream_)
+{
+    if (!plan || !workspace) return set_err(PFH_ERR_INVALID_ARGUMENT, "null argument");
+    PlanImpl &P = plan->impl;
+    if (P.kind <= PFH_LOCAL_DHT || P.kind == PFH_LOCAL_TRANSPOSE || P.kind == PFH_ALLTOALLV_TRANSPOSE)
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "local and transpose plans have no plan-bound workspace");
+    if (workspace_bytes < (size_t)kStatusBytes)
+        return set_err(PFH_ERR_WORKSPACE_TOO_SMALL, "workspace too small", (size_t)kStatusBytes, workspace_bytes);
+    hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+    PFH_HIP_CHECK(launch_fill_i32(static_cast<int *>(workspace), kStatusBytes / 4, 0, stream));
+    PFH_HIP_CHECK(launch_fill_u64(reinterpret_cast<unsigned long long *>(static_cast<char *>(workspace) + kStatusOffTag),
+                                  1, (unsigned long long)P.tag, stream));
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    WsEntry e;
+    e.tag = P.tag;
+    g_ws[reinterpret_cast<uintptr_t>(workspace)] = e;
     return ok();
 }
 
@@ -2106,6 +2287,13 @@ pfh_status pfh_local_execute(pfh_plan plan, int op, const void *src, size_t src_
     }
     if (batch == 0) return ok();
     if (!src || !dst) return set_err(PFH_ERR_INVALID_ARGUMENT, "null buffer");
+    {
+        const size_t sb = (size_t)(batch * lin * tmpl.in.elem), db = (size_t)(batch * lout * tmpl.out.elem);
+        if (overlaps(src, sb, dst, db)) return set_err(PFH_ERR_ALIASED_BUFFERS, "source and destination overlap");
+        if (c2r_check && (overlaps(src, sb, workspace, (size_t)status_area_bytes(P)) ||
+                          overlaps(dst, db, workspace, (size_t)status_area_bytes(P))))
+            return set_err(PFH_ERR_ALIASED_BUFFERS, "buffers and workspace overlap");
+    }
     OpPlan o = tmpl;
     Step &s = o.steps[0];
     s.in.extra = s.out.extra = batch;
@@ -2156,10 +2344,7 @@ pfh_status pfh_workspace_take_status(pfh_plan plan, void *workspace, pfh_stream_
 {
     if (!plan || !workspace) return set_err(PFH_ERR_INVALID_ARGUMENT, "null argument");
     WsView wv;
-    char *b = static_cast<char *>(workspace);
-    wv.status = reinterpret_cast<int *>(b);
-    wv.gate = reinterpret_cast<int *>(b + 4);
-    wv.first_code = reinterpret_cast<unsigned long long *>(b + 8);
+    status_view(workspace, wv);
     return collect_status(plan->impl, wv, reinterpret_cast<hipStream_t>(stream));
 }
 
@@ -2327,7 +2512,8 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
             for (int R : st.radices) wide = wide || is_wide_radix(R);
             // every compile-time plan of the standard register budget shares
             // one launch; the others get a launch per plan / radix set
-            const bool multi = B.tuning.batch_multi && st.fast && st.fem == (sizeof(T) == 4 ? 16 : 8);
+            const bool multi = B.tuning.batch_multi && st.fast &&
+                               multi_batch_plan_available((int)sizeof(T), st.fr[0], st.fr[1], st.fr[2], st.fem);
             Group *g = nullptr;
             for (Group &c : groups) {
                 const BatchStage &k = c.bs;

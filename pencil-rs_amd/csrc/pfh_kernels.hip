@@ -1340,6 +1340,17 @@ template hipError_t launch_line_fast<float>(const StageArgs<float> &, int, int, 
 template hipError_t launch_line_fast<double>(const StageArgs<double> &, int, int, int, int, int, size_t, int64_t,
                                              hipStream_t);
 
+// The plans k_line_multi_batch dispatches on (its PFH_MULTI_CASE list): an
+// item of any other plan must not go to it, or its tiles would be skipped.
+bool multi_batch_plan_available(int tbytes, int r0, int r1, int r2, int em)
+{
+#define PFH_MULTI_HAS(TT, A, B, C, E) \
+    if (E != 32 && (int)sizeof(TT) == tbytes && r0 == A && r1 == B && r2 == C && em == E) return true;
+    PFH_FAST_PLANS(PFH_MULTI_HAS)
+#undef PFH_MULTI_HAS
+    return false;
+}
+
 bool fast_plan_available(int tbytes, int r0, int r1, int r2, int em)
 {
 #define PFH_FAST_HAS(TT, A, B, C, E) \
@@ -1439,7 +1450,7 @@ __global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const 
                                                           int *gate)
 {
     __shared__ double sh[4];
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *gate = 0;
+    if (!e.keep_gate && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *gate = 0;
     const int64_t bp = blockIdx.y;
     const int64_t batch = bp / e.planes;
     const int plane = (int)(bp % e.planes);
@@ -1506,7 +1517,11 @@ __global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const 
 // Combines the chunks of one batch (one block per batch) with a fixed-order
 // tree reduction: global max M, sums rescaled by (m_chunk / M)^2 (equal to the
 // reference's sums of (x/M)^2), then the acceptance rule
-// (distributed.rs:4330-4349). Block (0) of k_endpoint_partial cleared *gate.
+// (distributed.rs:4330-4349). Block (0) of k_endpoint_partial cleared *gate
+// (in-place calls: k_inplace_begin set it from the poison word). A rejected
+// spectrum of an in-place call also poisons the workspace: the stages before
+// the check have already overwritten the caller's array
+// (run_in_place_transaction, distributed.rs:3776-3794).
 __global__ __launch_bounds__(256) void k_endpoint_combine(EndpointArgs e, const EndpointPartial *part, int *gate,
                                                           int *status)
 {
@@ -1542,6 +1557,19 @@ __global__ __launch_bounds__(256) void k_endpoint_combine(EndpointArgs e, const 
     if (threadIdx.x == 0 && invalid) {
         atomicOr(gate, 1);
         atomicOr(status, PFH_DEVICE_INVALID_SPECTRUM);
+        if (e.poison != nullptr) atomicOr(e.poison, 1);
+    }
+}
+
+// First kernel of an in-place call: a workspace poisoned by an earlier
+// in-place failure gates every stage of this call (it writes nothing) and
+// flags the call in the status word.
+__global__ void k_inplace_begin(int *gate, const int *poison, int *status)
+{
+    if (threadIdx.x == 0) {
+        const int p = *poison;
+        *gate = p != 0 ? 1 : 0;
+        if (p != 0) atomicOr(status, PFH_DEVICE_POISONED);
     }
 }
 
@@ -1763,7 +1791,7 @@ hipError_t launch_endpoint_check(const EndpointArgs &e, const void *data, Endpoi
     if (by > 0)
         hipLaunchKernelGGL(k_endpoint_partial<T>, dim3((unsigned)e.nblk, (unsigned)by), dim3(256), 0, s, e, data,
                            part, gate);
-    else
+    else if (!e.keep_gate)
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, gate, (int64_t)1, 0);
     if (e.extra > 0)
         hipLaunchKernelGGL(k_endpoint_combine, dim3((unsigned)e.extra), dim3(256), 0, s, e, part, gate, status);
@@ -1793,6 +1821,21 @@ template hipError_t launch_local_endpoint_check<float>(const void *, int64_t, in
                                                        int *, hipStream_t);
 template hipError_t launch_local_endpoint_check<double>(const void *, int64_t, int64_t, int, unsigned long long *,
                                                         int *, int *, hipStream_t);
+
+hipError_t launch_inplace_begin(int *gate, const int *poison, int *status, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_inplace_begin, dim3(1), dim3(64), 0, s, gate, poison, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, int64_t n, unsigned long long v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)g), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_i32(int *p, int64_t n, int v, hipStream_t s)
 {

@@ -12,6 +12,14 @@ namespace pfh {
 // Sticky device status bits (kept in the caller's workspace).
 constexpr int PFH_DEVICE_INVALID_SPECTRUM = 1;
 constexpr int PFH_DEVICE_INVALID_ENDPOINT = 2;
+constexpr int PFH_DEVICE_POISONED = 4; // an in-place call found the workspace poisoned and wrote nothing
+
+// Workspace status area (the first 256 bytes of every plan workspace).
+constexpr int kStatusOffStatus = 0;  // int: sticky device status bits
+constexpr int kStatusOffGate = 4;    // int: nonzero = later stages of this call write nothing
+constexpr int kStatusOffFirst = 8;   // u64: first failing local endpoint code
+constexpr int kStatusOffTag = 16;    // u64: tag of the plan the workspace is bound to (0 = unbound)
+constexpr int kStatusOffPoison = 24; // int: an in-place call failed after it started writing
 
 // One chunk's endpoint statistics of an R2C boundary plane.
 struct EndpointPartial {
@@ -32,6 +40,8 @@ struct EndpointArgs {
     int planes;      // 2 (even real length) or 1
     int nblk;        // chunks per plane
     double relative, absolute;
+    int keep_gate;   // 1: the gate was set before the check (in-place calls) and is not cleared
+    int *poison;     // non-null (in-place calls): a rejected spectrum sets *poison = 1
 };
 
 struct PermuteArgs {
@@ -72,6 +82,7 @@ template <typename T>
 hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int em, int threads, size_t lds_bytes,
                             int64_t grid, hipStream_t s);
 bool fast_plan_available(int tbytes, int r0, int r1, int r2, int em);
+bool multi_batch_plan_available(int tbytes, int r0, int r1, int r2, int em);
 hipError_t set_fast_lds_limit(size_t bytes);
 
 template <typename T>
@@ -83,6 +94,11 @@ hipError_t launch_local_endpoint_check(const void *data, int64_t batch, int64_t 
                                        unsigned long long *first_code, int *gate, int *status, hipStream_t s);
 
 hipError_t launch_fill_i32(int *p, int64_t n, int v, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long *p, int64_t n, unsigned long long v, hipStream_t s);
+
+// First kernel of an in-place call: gate = poison word; a poisoned
+// workspace also sets PFH_DEVICE_POISONED in the status word.
+hipError_t launch_inplace_begin(int *gate, const int *poison, int *status, hipStream_t s);
 
 hipError_t launch_permute_words(const PermuteArgs &p, const void *src, void *dst, int word, hipStream_t s);
 

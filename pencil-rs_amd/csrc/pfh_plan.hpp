@@ -148,6 +148,7 @@ struct PlanImpl {
     int N = 0;
     int64_t extra = 1;
     int64_t partials = 1; // endpoint-statistics chunks kept in the workspace status area
+    uint64_t tag = 0;     // unique per plan: a workspace's status area records the plan it is bound to
     Tuning tuning;
     DeviceInfo dev;
     // op -> plan (out of place and in place)

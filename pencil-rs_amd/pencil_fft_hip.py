@@ -66,6 +66,7 @@ STATUS_NAMES = {
     29: "ExtraExtentMismatch", 30: "EmptyInputs", 31: "InvalidInput", 32: "InvalidPath", 33: "SizeLimit",
     34: "MetadataMismatch", 35: "InvalidFile", 36: "IncompleteFile", 37: "Native", 38: "WriteIncomplete",
     39: "CommitUncertain", 40: "InvalidName", 41: "DuplicateName", 42: "NotFound", 43: "InvalidTail",
+    44: "Poisoned", 45: "AliasedBuffers",
 }
 
 
@@ -182,7 +183,7 @@ _lib = None
 EXPORTED_SYMBOLS = (
     "pfh_version", "pfh_last_error", "pfh_status_string", "pfh_plan_create", "pfh_plan_destroy",
     "pfh_plan_workspace_bytes", "pfh_plan_layout", "pfh_execute", "pfh_execute_in_place", "pfh_local_execute",
-    "pfh_local_execute_in_place", "pfh_execute_with_timing", "pfh_workspace_take_status", "pfh_batch_create",
+    "pfh_local_execute_in_place", "pfh_execute_with_timing", "pfh_workspace_take_status", "pfh_workspace_init", "pfh_batch_create",
     "pfh_batch_destroy", "pfh_batch_workspace_bytes", "pfh_batch_execute", "pfh_batch_tuning",
     "pfh_pointwise_scale_axes", "pfh_pointwise_scale_axes_complex", "pfh_pointwise_quadratic", "pfh_pointwise_quadratic_adjoint", "pfh_sum_squares",
     "pfh_reduce_workspace_bytes", "pfh_reduce", "pfh_pointwise", "pfh_io_workspace_bytes", "pfh_io_write",
@@ -216,6 +217,7 @@ def load_library(path: str = LIB_PATH):
     lib.pfh_local_execute_in_place.argtypes = [vp, i, vp, sz, sz, vp, sz, vp, i]
     lib.pfh_execute_with_timing.argtypes = [vp, i, vp, sz, vp, sz, vp, sz, vp, ctypes.POINTER(TransformTiming)]
     lib.pfh_workspace_take_status.argtypes = [vp, vp, vp]
+    lib.pfh_workspace_init.argtypes = [vp, vp, sz, vp]
     lib.pfh_batch_create.argtypes = [ctypes.POINTER(_Desc), i, i, ctypes.POINTER(vp)]
     lib.pfh_batch_destroy.argtypes = [vp]
     lib.pfh_batch_workspace_bytes.argtypes = [vp, ctypes.POINTER(sz)]
@@ -315,6 +317,13 @@ class _Plan:
         (the reference's *Workspace types, bound to one plan)."""
         need = max(self.workspace_bytes(op) for op in range(6))
         return torch.zeros(max(need, 256), dtype=torch.uint8, device="cuda")
+
+    def init_workspace(self, ws: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Binds a workspace to this plan and resets its status area, which
+        also clears an in-place poisoned state (pfh_workspace_init)."""
+        w = self._workspace(ws)
+        _check(_lib.pfh_workspace_init(self._h, _ptr(w), w.numel() * w.element_size(), _stream()))
+        return w
 
     def _workspace(self, ws: Optional[torch.Tensor]) -> torch.Tensor:
         if ws is not None:

@@ -373,6 +373,8 @@ static void check_grid(const char *what, const StageArgs<T> &a, int64_t grid)
 }
 
 int g_launches = 0;
+int g_all_launches = 0;         // every device launch, fills included
+bool g_reject_endpoints = false; // the emulated C2R boundary check rejects every spectrum
 static const bool g_trace = std::getenv("PFH_EMU_TRACE") != nullptr;
 
 template <typename T>
@@ -393,6 +395,7 @@ hipError_t launch_line_stage(const StageArgs<T> &a, bool wide, int threads, size
                              hipStream_t)
 {
     ++g_launches;
+    ++g_all_launches;
     begin_launch();
     if (threads > (wide ? kMaxThreadsWide : kMaxThreads)) fail("line_stage: %d threads", threads);
     trace("line_stage", a, threads, grid);
@@ -410,6 +413,7 @@ hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int e
                             int64_t grid, hipStream_t)
 {
     ++g_launches;
+    ++g_all_launches;
     begin_launch();
     if (threads > 512) fail("line_fast: %d threads", threads);
     if (!fast_plan_available((int)sizeof(T), r0, r1, r2, em)) fail("line_fast: plan not instantiated");
@@ -432,6 +436,15 @@ template hipError_t launch_line_fast<float>(const StageArgs<float> &, int, int, 
 template hipError_t launch_line_fast<double>(const StageArgs<double> &, int, int, int, int, int, size_t, int64_t,
                                              hipStream_t);
 
+bool multi_batch_plan_available(int tbytes, int r0, int r1, int r2, int em)
+{
+#define PFH_MULTI_HAS(TT, A, B, C, E) \
+    if (E != 32 && (int)sizeof(TT) == tbytes && r0 == A && r1 == B && r2 == C && em == E) return true;
+    PFH_FAST_PLANS(PFH_MULTI_HAS)
+#undef PFH_MULTI_HAS
+    return false;
+}
+
 bool fast_plan_available(int tbytes, int r0, int r1, int r2, int em)
 {
 #define PFH_FAST_HAS(TT, A, B, C, E) \
@@ -448,6 +461,7 @@ static void emulate_batch(const char *what, const StageArgs<T> *items, const int
                           const int *fr, bool wide = false)
 {
     ++g_launches;
+    ++g_all_launches;
     begin_launch();
     if (threads > kMaxThreadsWide) fail("%s: %d threads", what, threads);
     if (grid <= 0 || grid > total) fail("%s: grid %" PRId64 " for %" PRId64 " tiles", what, grid, total);
@@ -508,6 +522,7 @@ hipError_t launch_line_multi_batch(const StageArgs<T> *items, const int64_t *pre
             fail("line_multi_batch: item %d has no plan in the kernel", q);
     }
     ++g_launches;
+    ++g_all_launches;
     begin_launch();
     if (threads > 512) fail("line_multi_batch: %d threads", threads);
     if (grid <= 0 || grid > total) fail("line_multi_batch: grid");
@@ -557,6 +572,7 @@ hipError_t launch_endpoint_check(const EndpointArgs &e, const void *data, Endpoi
                                  int *status, hipStream_t)
 {
     ++g_launches;
+    ++g_all_launches;
     begin_launch();
     const int64_t count = e.outer * e.stride;
     // the partial kernel covers at most 2 x 256 elements of a chunk
@@ -573,7 +589,13 @@ hipError_t launch_endpoint_check(const EndpointArgs &e, const void *data, Endpoi
         }
     check_write("endpoint_check partials", part, sizeof(EndpointPartial) * e.extra * e.planes * e.nblk);
     check_read("endpoint_check status", status, sizeof(int));
-    *gate = 0; // accept
+    if (g_reject_endpoints) {
+        *gate = 1;
+        *status |= PFH_DEVICE_INVALID_SPECTRUM;
+        if (e.poison) *e.poison = 1;
+    } else if (!e.keep_gate) {
+        *gate = 0; // accept
+    }
     return hipSuccess;
 }
 template hipError_t launch_endpoint_check<float>(const EndpointArgs &, const void *, EndpointPartial *, int *, int *,
@@ -586,6 +608,7 @@ hipError_t launch_local_endpoint_check(const void *data, int64_t lines, int64_t 
                                        unsigned long long *first, int *gate, int *status, hipStream_t)
 {
     ++g_launches;
+    ++g_all_launches;
     (void)nyq;
     check_read("local_endpoint_check", data, (size_t)(lines * clen) * 2 * sizeof(T));
     check_read("local_endpoint_check first", first, sizeof(*first));
@@ -600,13 +623,32 @@ template hipError_t launch_local_endpoint_check<double>(const void *, int64_t, i
 
 hipError_t launch_fill_i32(int *p, int64_t n, int v, hipStream_t)
 {
+    ++g_all_launches;
+    if (n > 0 && !check_read("fill_i32", p, (size_t)n * sizeof(int))) return hipSuccess;
     for (int64_t i = 0; i < n; ++i) p[i] = v;
+    return hipSuccess;
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, int64_t n, unsigned long long v, hipStream_t)
+{
+    ++g_all_launches;
+    if (n > 0 && !check_read("fill_u64", p, (size_t)n * sizeof(*p))) return hipSuccess;
+    for (int64_t i = 0; i < n; ++i) p[i] = v;
+    return hipSuccess;
+}
+
+hipError_t launch_inplace_begin(int *gate, const int *poison, int *status, hipStream_t)
+{
+    ++g_all_launches;
+    *gate = *poison != 0 ? 1 : 0;
+    if (*poison) *status |= PFH_DEVICE_POISONED;
     return hipSuccess;
 }
 
 hipError_t launch_permute_words(const PermuteArgs &p, const void *src, void *dst, int word, hipStream_t)
 {
     ++g_launches;
+    ++g_all_launches;
     begin_launch();
     const int64_t total = p.len * p.extra;
     for (int64_t i = 0; i < total; ++i) {
@@ -628,6 +670,7 @@ template <typename T>
 hipError_t launch_scale_axes(const ScaleAxesArgs &s, void *data, const void *factors, hipStream_t)
 {
     ++g_launches;
+    ++g_all_launches;
     const int64_t total = s.len * s.extra;
     const size_t eb = (s.complex_data ? 2 : 1) * sizeof(T);
     check_read("scale_axes", data, (size_t)total * eb);
@@ -1153,8 +1196,194 @@ void pointwise_cases()
     }
 }
 
+// The boundary contract (no device work before a refusal): workspace
+// binding (WorkspaceMismatch), overlapping buffers, and the in-place poisoned
+// state after a failure that follows the first write, synchronous and
+// deferred.
+static void contract_cases()
+{
+    auto expect = [](const char *what, pfh_status got, pfh_status want) {
+        if (got != want)
+            fail("contract: %s: got %d (%s), expected %d", what, (int)got, pfh_status_string(got), (int)want);
+    };
+    auto mk = [](pfh_desc d) {
+        pfh_plan p = nullptr;
+        if (pfh_plan_create(&d, &p) != PFH_OK) fail("contract: plan");
+        return p;
+    };
+    auto ws_bytes = [](pfh_plan p) {
+        size_t m = 0;
+        for (int op = 0; op < 6; ++op) {
+            size_t b = 0;
+            if (pfh_plan_workspace_bytes(p, op, &b) == PFH_OK) m = std::max(m, b);
+        }
+        return m;
+    };
+    auto same = [](const void *a, const std::vector<uint8_t> &b) { return std::memcmp(a, b.data(), b.size()) == 0; };
+    auto snap = [](const void *a, size_t n) {
+        std::vector<uint8_t> v(n);
+        std::memcpy(v.data(), a, n);
+        return v;
+    };
+    const int before_fail = g_fail;
+
+    // workspace binding
+    const pfh_desc dc = route_desc(PFH_C2C, PFH_F32, {8, 6, 4}, {2});
+    pfh_plan A = mk(dc), B = mk(dc);
+    const size_t n = 8 * 6 * 4 * 2, nb = n * 8;
+    const size_t wa = ws_bytes(A), wb = ws_bytes(B);
+    void *wsA = dmalloc(wa, true), *wsB = dmalloc(wb, true);
+    float *src = static_cast<float *>(dmalloc(nb));
+    for (size_t i = 0; i < 2 * n; ++i) src[i] = (float)i;
+    void *dst = dmalloc(nb, true);
+    expect("A with its workspace", pfh_execute(A, PFH_FORWARD, src, n, dst, n, wsA, wa, nullptr, 0), PFH_OK);
+    expect("B with its workspace", pfh_execute(B, PFH_FORWARD, src, n, dst, n, wsB, wb, nullptr, 0), PFH_OK);
+    {
+        std::memset(dst, 0x5A, nb);
+        const auto d0 = snap(dst, nb), w0 = snap(wsA, wa);
+        const int l0 = pfh::g_all_launches;
+        expect("B with A's workspace", pfh_execute(B, PFH_FORWARD, src, n, dst, n, wsA, wa, nullptr, 0),
+               PFH_ERR_WORKSPACE_MISMATCH);
+        expect("B in place with A's workspace", pfh_execute_in_place(B, PFH_FORWARD, dst, nb, wsA, wa, nullptr, 0),
+               PFH_ERR_WORKSPACE_MISMATCH);
+        expect("B timed with A's workspace",
+               [&] {
+                   pfh_transform_timing t;
+                   return pfh_execute_with_timing(B, PFH_FORWARD, src, n, dst, n, wsA, wa, nullptr, &t);
+               }(),
+               PFH_ERR_WORKSPACE_MISMATCH);
+        if (pfh::g_all_launches != l0) fail("contract: a refused call launched device work");
+        if (!same(dst, d0) || !same(wsA, w0)) fail("contract: a refused call wrote memory");
+    }
+    // pfh_workspace_init rebinds; a zero-filled workspace is unbound
+    expect("init A's workspace for B", pfh_workspace_init(B, wsA, wa, nullptr), PFH_OK);
+    expect("B with the rebound workspace", pfh_execute(B, PFH_FORWARD, src, n, dst, n, wsA, wa, nullptr, 0), PFH_OK);
+    expect("A with the workspace bound to B", pfh_execute(A, PFH_FORWARD, src, n, dst, n, wsA, wa, nullptr, 0),
+           PFH_ERR_WORKSPACE_MISMATCH);
+    std::memset(wsA, 0, wa);
+    expect("A with a zero-filled workspace", pfh_execute(A, PFH_FORWARD, src, n, dst, n, wsA, wa, nullptr, 0), PFH_OK);
+    expect("init of a local plan's workspace",
+           [&] {
+               pfh_desc d;
+               std::memset(&d, 0, sizeof(d));
+               d.kind = PFH_LOCAL_C2C;
+               d.precision = PFH_F32;
+               d.line_len = 8;
+               pfh_plan L = mk(d);
+               const pfh_status st = pfh_workspace_init(L, wsA, wa, nullptr);
+               pfh_plan_destroy(L);
+               return st;
+           }(),
+           PFH_ERR_INVALID_ARGUMENT);
+
+    // overlapping buffers: refused before any device work
+    {
+        char *big = static_cast<char *>(dmalloc(3 * nb + wa, true));
+        const int l0 = pfh::g_all_launches;
+        expect("src == dst", pfh_execute(A, PFH_FORWARD, big, n, big, n, wsA, wa, nullptr, 0), PFH_ERR_ALIASED_BUFFERS);
+        expect("dst = src + 8 bytes", pfh_execute(A, PFH_FORWARD, big, n, big + 8, n, wsA, wa, nullptr, 0),
+               PFH_ERR_ALIASED_BUFFERS);
+        expect("dst ends where src starts (no overlap)",
+               pfh_execute(A, PFH_FORWARD, big + nb, n, big, n, wsA, wa, nullptr, 0), PFH_OK);
+        const int l1 = pfh::g_all_launches;
+        expect("src inside the workspace", pfh_execute(A, PFH_FORWARD, big + nb, n, big, n, big + nb, wa, nullptr, 0),
+               PFH_ERR_ALIASED_BUFFERS);
+        expect("dst inside the workspace",
+               pfh_execute(A, PFH_FORWARD, src, n, big + 2 * nb, n, big + 2 * nb + 16, wa, nullptr, 0),
+               PFH_ERR_ALIASED_BUFFERS);
+        expect("in-place data inside the workspace",
+               pfh_execute_in_place(A, PFH_FORWARD, big + 64, nb, big, wa + nb, nullptr, 0), PFH_ERR_ALIASED_BUFFERS);
+        pfh_desc d;
+        std::memset(&d, 0, sizeof(d));
+        d.kind = PFH_LOCAL_C2C;
+        d.precision = PFH_F32;
+        d.line_len = 8;
+        pfh_plan L = mk(d);
+        expect("local src == dst", pfh_local_execute(L, PFH_FORWARD, big, 16, big, 16, nullptr, 0, nullptr, 0),
+               PFH_ERR_ALIASED_BUFFERS);
+        pfh_plan_destroy(L);
+        pfh_desc t = route_desc(PFH_LOCAL_TRANSPOSE, PFH_F32, {4, 5, 6}, {});
+        t.element_bytes = 8;
+        t.src_perm[0] = 0, t.src_perm[1] = 1, t.src_perm[2] = 2;
+        t.dst_perm[0] = 2, t.dst_perm[1] = 0, t.dst_perm[2] = 1;
+        pfh_plan T = mk(t);
+        expect("transpose src == dst", pfh_execute(T, PFH_FORWARD, big, 120, big, 120, nullptr, 0, nullptr, 0),
+               PFH_ERR_ALIASED_BUFFERS);
+        pfh_plan_destroy(T);
+        if (pfh::g_all_launches != l1) fail("contract: an aliased call launched device work");
+        (void)l0;
+        (void)hipFree(big);
+    }
+
+    // in-place poisoned state (run_in_place_transaction)
+    pfh_plan R = mk(route_desc(PFH_R2C, PFH_F32, {8, 6, 4}, {}));
+    const size_t wr = ws_bytes(R);
+    void *wsR = dmalloc(wr, true);
+    const size_t cap = 8 * 6 * 3 * 8;
+    void *data = dmalloc(cap, true);
+    pfh::g_reject_endpoints = true;
+    expect("in-place inverse of a rejected spectrum",
+           pfh_execute_in_place(R, PFH_INVERSE, data, cap, wsR, wr, nullptr, 0), PFH_ERR_INVALID_SPECTRUM);
+    pfh::g_reject_endpoints = false;
+    {
+        const auto d0 = snap(data, cap);
+        const int l0 = pfh::g_all_launches;
+        expect("in-place forward on a poisoned workspace",
+               pfh_execute_in_place(R, PFH_FORWARD, data, cap, wsR, wr, nullptr, 0), PFH_ERR_POISONED);
+        expect("in-place inverse on a poisoned workspace",
+               pfh_execute_in_place(R, PFH_INVERSE, data, cap, wsR, wr, nullptr, 0), PFH_ERR_POISONED);
+        if (pfh::g_all_launches != l0) fail("contract: a poisoned in-place call launched device work");
+        if (!same(data, d0)) fail("contract: a poisoned in-place call wrote its array");
+    }
+    {
+        float *x = static_cast<float *>(dmalloc(8 * 6 * 4 * 4, true));
+        void *y = dmalloc(cap, true);
+        expect("out of place with a poisoned workspace",
+               pfh_execute(R, PFH_FORWARD, x, 8 * 6 * 4, y, 8 * 6 * 3, wsR, wr, nullptr, 0), PFH_OK);
+        (void)hipFree(x);
+        (void)hipFree(y);
+    }
+    expect("workspace init resets", pfh_workspace_init(R, wsR, wr, nullptr), PFH_OK);
+    expect("in-place forward after the reset", pfh_execute_in_place(R, PFH_FORWARD, data, cap, wsR, wr, nullptr, 0),
+           PFH_OK);
+    // deferred: the failure and the gated later call show at the status read
+    pfh::g_reject_endpoints = true;
+    expect("deferred in-place inverse",
+           pfh_execute_in_place(R, PFH_INVERSE, data, cap, wsR, wr, nullptr, PFH_EXEC_DEFER_STATUS), PFH_OK);
+    pfh::g_reject_endpoints = false;
+    {
+        const auto d0 = snap(data, cap);
+        expect("deferred in-place forward after it",
+               pfh_execute_in_place(R, PFH_FORWARD, data, cap, wsR, wr, nullptr, PFH_EXEC_DEFER_STATUS), PFH_OK);
+        if (!same(data, d0)) fail("contract: an in-place call after a deferred failure wrote its array");
+    }
+    expect("take status", pfh_workspace_take_status(R, wsR, nullptr), PFH_ERR_INVALID_SPECTRUM);
+    expect("in-place after the status read", pfh_execute_in_place(R, PFH_FORWARD, data, cap, wsR, wr, nullptr, 0),
+           PFH_ERR_POISONED);
+    // a synchronous in-place call after a deferred failure reads the status itself
+    expect("reset again", pfh_workspace_init(R, wsR, wr, nullptr), PFH_OK);
+    pfh::g_reject_endpoints = true;
+    expect("deferred in-place inverse (2)",
+           pfh_execute_in_place(R, PFH_INVERSE, data, cap, wsR, wr, nullptr, PFH_EXEC_DEFER_STATUS), PFH_OK);
+    pfh::g_reject_endpoints = false;
+    expect("synchronous in-place forward after it",
+           pfh_execute_in_place(R, PFH_FORWARD, data, cap, wsR, wr, nullptr, 0), PFH_ERR_INVALID_SPECTRUM);
+    expect("and the next one", pfh_execute_in_place(R, PFH_FORWARD, data, cap, wsR, wr, nullptr, 0), PFH_ERR_POISONED);
+
+    for (void *p : {(void *)wsA, wsB, (void *)src, dst, wsR, data}) (void)hipFree(p);
+    pfh_plan_destroy(A);
+    pfh_plan_destroy(B);
+    pfh_plan_destroy(R);
+    std::printf("contract: %s\n", g_fail == before_fail ? "ok" : "FAILED");
+}
+
 int main(int argc, char **argv)
 {
+    if (argc == 2 && std::strcmp(argv[1], "contract") == 0) {
+        contract_cases();
+        std::printf("launches %d, failures %d\n", pfh::g_launches, g_fail);
+        return g_fail ? 1 : 0;
+    }
     if (argc == 6) {
         // one plan: emu c2c|r2c n0 n1 n2 f32|f64 (planner defaults)
         const bool r2c = std::strcmp(argv[1], "r2c") == 0, f32 = std::strcmp(argv[5], "f32") == 0;

@@ -159,7 +159,7 @@ R2R_KINDS = [pf.R2rKind.DctI, pf.R2rKind.DctII, pf.R2rKind.DctIII, pf.R2rKind.Dc
 @pytest.mark.parametrize("kind", R2R_KINDS)
 def test_local_r2r_matches_oracle(gpu, kind, dt):
     rng = np.random.default_rng(kind)
-    for n in [1, 2, 3, 4, 5, 6, 7, 8, 12, 13, 16, 60, 96, 240, 256, 384]:
+    for n in [1, 2, 3, 4, 5, 6, 7, 8, 12, 13, 16, 60, 96, 240, 256, 384, 385, 386]:
         if kind == pf.R2rKind.DctI and n == 1:
             with pytest.raises(pf.PfhError):
                 pf.LocalR2rPlan(1, kind)
@@ -176,7 +176,7 @@ def test_local_r2r_matches_oracle(gpu, kind, dt):
 @pytest.mark.parametrize("dt", DTYPES)
 def test_local_dht_matches_oracle(gpu, dt):
     rng = np.random.default_rng(9)
-    for n in [2, 3, 4, 5, 8, 12, 13, 60, 240, 256, 384]:
+    for n in [2, 3, 4, 5, 8, 12, 13, 60, 240, 256, 384, 385]:
         x = uniform_real(rng, (3, n), dt)
         plan = pf.LocalDhtPlan(n, dtype=torch.from_numpy(x).dtype)
         xd = _tdev(x)

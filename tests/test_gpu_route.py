@@ -326,6 +326,13 @@ R2R_CASES = [
     ((120, 64, 128), 2, [pf.R2rKind.DstII, pf.R2rKind.DctIII, pf.R2rKind.DstIV]),
     ((65, 6, 128), 1, [pf.R2rKind.DctI, None, pf.R2rKind.DctIII]),
     ((128, 127, 2), 1, [pf.R2rKind.DstIII, pf.R2rKind.DstI, None]),
+    # 385-point FFTs (the 11 x 7 x 5 compile-time plan in f32): odd-length
+    # DCT-II / DCT-III / DST-II / DST-III / DHT lines of length 385, DST-I of
+    # length 384 and DCT-I of length 386
+    ((385, 4, 6), 1, [pf.R2rKind.DctII, None, pf.R2rKind.DstIII]),
+    ((6, 385, 4), 1, [pf.R2rKind.DctIII, pf.R2rKind.DstII, None]),
+    ((4, 6, 385), 1, [None, pf.R2rKind.Dht, pf.R2rKind.Dht]),
+    ((384, 3, 386), 1, [pf.R2rKind.DstI, None, pf.R2rKind.DctI]),
 ]
 
 

@@ -35,3 +35,15 @@ def test_routes_and_batch_in_bounds_with_layout_semantics(emu):
     # every scenario reported ok
     lines = [l for l in r.stdout.splitlines() if l.endswith(": ok") or l.endswith(": FAILED")]
     assert lines and all(l.endswith(": ok") for l in lines)
+
+
+def test_boundary_contract_refuses_before_device_work(emu):
+    """WorkspaceMismatch for a workspace bound to another plan
+    (distributed.rs:2188-2191, :4015-4017), overlapping src / dst / workspace
+    refused, and the in-place Poisoned state after a failure that follows
+    the first write (run_in_place_transaction, distributed.rs:3776-3794;
+    many.rs:234), synchronous and deferred: every refusal launches nothing
+    and writes nothing."""
+    r = subprocess.run([emu, "contract"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
+    assert "contract: ok" in r.stdout
