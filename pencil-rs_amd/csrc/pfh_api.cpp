@@ -2249,9 +2249,23 @@ pfh_status pfh_workspace_init(pfh_plan plan, void *workspace, size_t workspace_b
     return ok();
 }
 
+static pfh_status local_execute_impl(pfh_plan plan, int op, const void *src, size_t src_elems, void *dst,
+                                     size_t dst_elems, void *workspace, size_t workspace_bytes,
+                                     pfh_stream_t stream_, int flags, bool in_place);
+
 pfh_status pfh_local_execute(pfh_plan plan, int op, const void *src, size_t src_elems, void *dst,
                              size_t dst_elems, void *workspace, size_t workspace_bytes, pfh_stream_t stream_,
                              int flags)
+{
+    return local_execute_impl(plan, op, src, src_elems, dst, dst_elems, workspace, workspace_bytes, stream_, flags,
+                              false);
+}
+
+// in_place: src == dst is the caller's in-place array (every workgroup reads
+// its lines into LDS before it writes them); any other overlap is refused
+static pfh_status local_execute_impl(pfh_plan plan, int op, const void *src, size_t src_elems, void *dst,
+                                     size_t dst_elems, void *workspace, size_t workspace_bytes,
+                                     pfh_stream_t stream_, int flags, bool in_place)
 {
     if (!plan) return set_err(PFH_ERR_INVALID_ARGUMENT, "null plan");
     if (op < 0 || op > 5) return set_err(PFH_ERR_INVALID_ARGUMENT, "op");
@@ -2289,7 +2303,8 @@ pfh_status pfh_local_execute(pfh_plan plan, int op, const void *src, size_t src_
     if (!src || !dst) return set_err(PFH_ERR_INVALID_ARGUMENT, "null buffer");
     {
         const size_t sb = (size_t)(batch * lin * tmpl.in.elem), db = (size_t)(batch * lout * tmpl.out.elem);
-        if (overlaps(src, sb, dst, db)) return set_err(PFH_ERR_ALIASED_BUFFERS, "source and destination overlap");
+        if (!(in_place && src == dst) && overlaps(src, sb, dst, db))
+            return set_err(PFH_ERR_ALIASED_BUFFERS, "source and destination overlap");
         if (c2r_check && (overlaps(src, sb, workspace, (size_t)status_area_bytes(P)) ||
                           overlaps(dst, db, workspace, (size_t)status_area_bytes(P))))
             return set_err(PFH_ERR_ALIASED_BUFFERS, "buffers and workspace overlap");
@@ -2326,7 +2341,8 @@ pfh_status pfh_local_execute_in_place(pfh_plan plan, int op, void *data, size_t 
         return set_err(PFH_ERR_ALLOCATION_FAILED, "in-place storage too small", (size_t)need, capacity_bytes);
     if (P.kind != PFH_LOCAL_R2C) {
         // same layout on both sides: every workgroup reads its lines into LDS before writing them
-        return pfh_local_execute(plan, op, data, elems, data, elems, workspace, workspace_bytes, stream_, flags);
+        return local_execute_impl(plan, op, data, elems, data, elems, workspace, workspace_bytes, stream_, flags,
+                                  true);
     }
     // packed real / half-complex storage: stage the input through the workspace
     const int64_t ws_need = status_area_bytes(P) + align256(in_bytes);
@@ -2335,6 +2351,8 @@ pfh_status pfh_local_execute_in_place(pfh_plan plan, int op, void *data, size_t 
                        workspace ? workspace_bytes : 0);
     if (batch == 0) return ok();
     char *stage = static_cast<char *>(workspace) + status_area_bytes(P);
+    if (overlaps(data, (size_t)need, workspace, (size_t)ws_need))
+        return set_err(PFH_ERR_ALIASED_BUFFERS, "data and workspace overlap");
     PFH_HIP_CHECK(hipMemcpyAsync(stage, data, (size_t)in_bytes, hipMemcpyDeviceToDevice, stream));
     return pfh_local_execute(plan, op, stage, (size_t)(batch * lin), data, (size_t)(batch * lout), workspace,
                              workspace_bytes, stream_, flags);

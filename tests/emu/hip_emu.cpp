@@ -1285,7 +1285,7 @@ static void contract_cases()
                PFH_ERR_ALIASED_BUFFERS);
         expect("dst ends where src starts (no overlap)",
                pfh_execute(A, PFH_FORWARD, big + nb, n, big, n, wsA, wa, nullptr, 0), PFH_OK);
-        const int l1 = pfh::g_all_launches;
+        int l1 = pfh::g_all_launches;
         expect("src inside the workspace", pfh_execute(A, PFH_FORWARD, big + nb, n, big, n, big + nb, wa, nullptr, 0),
                PFH_ERR_ALIASED_BUFFERS);
         expect("dst inside the workspace",
@@ -1301,6 +1301,11 @@ static void contract_cases()
         pfh_plan L = mk(d);
         expect("local src == dst", pfh_local_execute(L, PFH_FORWARD, big, 16, big, 16, nullptr, 0, nullptr, 0),
                PFH_ERR_ALIASED_BUFFERS);
+        const int l2 = pfh::g_all_launches;
+        expect("local in place", pfh_local_execute_in_place(L, PFH_FORWARD, big, 16, 16 * 8, nullptr, 0, nullptr, 0),
+               PFH_OK);
+        if (pfh::g_all_launches == l2) fail("contract: local in place launched nothing");
+        l1 = pfh::g_all_launches;
         pfh_plan_destroy(L);
         pfh_desc t = route_desc(PFH_LOCAL_TRANSPOSE, PFH_F32, {4, 5, 6}, {});
         t.element_bytes = 8;
