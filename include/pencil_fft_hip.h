@@ -317,6 +317,20 @@ pfh_status pfh_execute_with_timing(pfh_plan plan, int op, const void *src, size_
  * PFH_EXEC_DEFER_STATUS executions (synchronizes the stream). */
 pfh_status pfh_workspace_take_status(pfh_plan plan, void *workspace, pfh_stream_t stream);
 
+/* Allocation-free pfh_execute. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The operation runs
+ * on a workspace inside the library, a static device array of 128 MiB per
+ * GPU (held until the process ends), instead of a caller workspace; it calls
+ * no hipMalloc / hipFree. Distributed plans only; an operation whose
+ * workspace (pfh_plan_workspace_bytes) exceeds 128 MiB gives
+ * PFH_ERR_WORKSPACE_TOO_SMALL {required, actual = 128 MiB}. flags must be 0:
+ * a rejected C2R spectrum is reported synchronously (PFH_ERR_INVALID_SPECTRUM,
+ * dst untouched). All calls on one GPU share its array: calls that overlap
+ * on one GPU (concurrent host threads, or unordered streams) race. */
+pfh_status pfh_execute_noalloc(pfh_plan plan, int op, const void *src, size_t src_elems, void *dst,
+                               size_t dst_elems, pfh_stream_t stream, int flags);
+
 /* ---------------------------------------------------------------------- */
 /* Batched uneven transforms (no reference counterpart; the reference runs */
 /* collections sequentially, distributed/collections.rs:229-311).          */

@@ -1918,7 +1918,8 @@ static pfh_status collect_status(const PlanImpl &P, const WsView &wv, hipStream_
 }
 
 static pfh_status execute_impl(PlanImpl &P, int op, const void *src, size_t src_elems, void *dst, size_t dst_elems,
-                               void *ws, size_t ws_bytes, hipStream_t stream, int flags, pfh_transform_timing *timing)
+                               void *ws, size_t ws_bytes, hipStream_t stream, int flags, pfh_transform_timing *timing,
+                               bool bind = true)
 {
     if (op < 0 || op > 5) return set_err(PFH_ERR_INVALID_ARGUMENT, "op");
     const bool is_tr = (P.kind == PFH_LOCAL_TRANSPOSE || P.kind == PFH_ALLTOALLV_TRANSPOSE);
@@ -1983,8 +1984,10 @@ static pfh_status execute_impl(PlanImpl &P, int op, const void *src, size_t src_
             return set_err(PFH_ERR_ALIASED_BUFFERS, "source and workspace overlap");
         if (overlaps(dst, (size_t)o.out.bytes(), ws, wb))
             return set_err(PFH_ERR_ALIASED_BUFFERS, "destination and workspace overlap");
-        pfh_status st = bind_workspace(P, ws, stream);
-        if (st) return st;
+        if (bind) {
+            pfh_status st = bind_workspace(P, ws, stream);
+            if (st) return st;
+        }
     }
     void *bufs[4] = {const_cast<void *>(src), dst, wv.w[0], wv.w[1]};
     TimingTrace tr;
@@ -2229,6 +2232,35 @@ pfh_status pfh_execute_in_place(pfh_plan plan, int op, void *data, size_t capaci
     if (check || ws_pending(workspace)) return collect_status(P, wv, stream);
     return ok();
 }
+
+} // extern "C"
+
+namespace pfh {
+// Out-of-place execution of a distributed plan on a workspace the library
+// owns (the allocation-free entry point's static scratch): no plan binding,
+// the status area is cleared by a fill kernel before the operation, device
+// errors are read synchronously. ws_bytes is the scratch capacity.
+pfh_status execute_on_scratch(pfh_plan plan, int op, const void *src, size_t src_elems, void *dst, size_t dst_elems,
+                              void *ws, size_t ws_bytes, hipStream_t stream)
+{
+    if (!plan) return set_err(PFH_ERR_INVALID_ARGUMENT, "null plan");
+    if (op < 0 || op > 5) return set_err(PFH_ERR_INVALID_ARGUMENT, "op");
+    PlanImpl &P = plan->impl;
+    if (P.kind <= PFH_LOCAL_DHT || P.kind == PFH_LOCAL_TRANSPOSE || P.kind == PFH_ALLTOALLV_TRANSPOSE)
+        return set_err(PFH_ERR_INVALID_ARGUMENT, "the static-scratch entry point runs distributed plans only");
+    const OpPlan &o = P.ops[op];
+    if (!o.valid) return set_err(PFH_ERR_INVALID_ARGUMENT, "op not supported by this plan");
+    const int64_t need = op_ws_bytes(P, o);
+    if (need > (int64_t)ws_bytes)
+        return set_err(PFH_ERR_WORKSPACE_TOO_SMALL,
+                       "the operation's workspace exceeds the library's static scratch (required > actual bytes)",
+                       (size_t)need, ws_bytes);
+    PFH_HIP_CHECK(launch_fill_i32(static_cast<int *>(ws), kStatusBytes / 4, 0, stream));
+    return execute_impl(P, op, src, src_elems, dst, dst_elems, ws, ws_bytes, stream, 0, nullptr, false);
+}
+} // namespace pfh
+
+extern "C" {
 
 pfh_status pfh_workspace_init(pfh_plan plan, void *workspace, size_t workspace_bytes, pfh_stream_t stream_)
 {

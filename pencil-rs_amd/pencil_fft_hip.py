@@ -183,7 +183,7 @@ _lib = None
 EXPORTED_SYMBOLS = (
     "pfh_version", "pfh_last_error", "pfh_status_string", "pfh_plan_create", "pfh_plan_destroy",
     "pfh_plan_workspace_bytes", "pfh_plan_layout", "pfh_execute", "pfh_execute_in_place", "pfh_local_execute",
-    "pfh_local_execute_in_place", "pfh_execute_with_timing", "pfh_workspace_take_status", "pfh_workspace_init", "pfh_batch_create",
+    "pfh_local_execute_in_place", "pfh_execute_with_timing", "pfh_workspace_take_status", "pfh_workspace_init", "pfh_execute_noalloc", "pfh_batch_create",
     "pfh_batch_destroy", "pfh_batch_workspace_bytes", "pfh_batch_execute", "pfh_batch_tuning",
     "pfh_pointwise_scale_axes", "pfh_pointwise_scale_axes_complex", "pfh_pointwise_quadratic", "pfh_pointwise_quadratic_adjoint", "pfh_sum_squares",
     "pfh_reduce_workspace_bytes", "pfh_reduce", "pfh_pointwise", "pfh_io_workspace_bytes", "pfh_io_write",
@@ -218,6 +218,7 @@ def load_library(path: str = LIB_PATH):
     lib.pfh_execute_with_timing.argtypes = [vp, i, vp, sz, vp, sz, vp, sz, vp, ctypes.POINTER(TransformTiming)]
     lib.pfh_workspace_take_status.argtypes = [vp, vp, vp]
     lib.pfh_workspace_init.argtypes = [vp, vp, sz, vp]
+    lib.pfh_execute_noalloc.argtypes = [vp, i, vp, sz, vp, sz, vp, i]
     lib.pfh_batch_create.argtypes = [ctypes.POINTER(_Desc), i, i, ctypes.POINTER(vp)]
     lib.pfh_batch_destroy.argtypes = [vp]
     lib.pfh_batch_workspace_bytes.argtypes = [vp, ctypes.POINTER(sz)]
