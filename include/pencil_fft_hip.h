@@ -357,11 +357,13 @@ pfh_status pfh_batch_execute(pfh_batch batch, int op, const void *const *srcs, v
                              size_t workspace_bytes, pfh_stream_t stream, int use_graph);
 /* The tuning the batch runs with (threads per block, largest radix, lines
  * per tile on column-access stages, whether the items of all compile-time
- * radix plans of a stage share one launch, and whether a launch keeps one
+ * radix plans of a stage share one launch, whether a launch keeps one
  * resident set of workgroups looping over the tiles or takes one workgroup
- * per tile), as chosen by the autotuner. */
+ * per tile, and whether fused plane stages run: the two inner route stages of
+ * a 3D f32 problem in one pass over whole (axis 0, axis 1) planes in LDS), as
+ * chosen by the autotuner. */
 pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines, int *one_launch,
-                            int *persistent);
+                            int *persistent, int *plane);
 
 /* ---------------------------------------------------------------------- */
 /* Pointwise operations between transforms (pseudo-spectral steps).        */

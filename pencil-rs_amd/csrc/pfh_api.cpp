@@ -19,6 +19,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -109,7 +110,8 @@ static pfh_status device_info(DeviceInfo &out)
         size_t lim = prop.sharedMemPerBlock;
         if (lim < 65536) lim = 65536;
         if (lim > 65536) {
-            if (set_line_stage_lds_limit(lim) != hipSuccess || set_fast_lds_limit(lim) != hipSuccess) {
+            if (set_line_stage_lds_limit(lim) != hipSuccess || set_fast_lds_limit(lim) != hipSuccess ||
+                set_plane_lds_limit(lim) != hipSuccess) {
                 (void)hipGetLastError(); // keep the failed attribute call out of later error checks
                 lim = 65536;
             }
@@ -2411,17 +2413,42 @@ pfh_status pfh_workspace_take_status(pfh_plan plan, void *workspace, pfh_stream_
 // One persistent launch: the items of one route stage that share a kernel
 // (a compile-time radix plan, or the generic kernel's narrow / wide radix set).
 struct BatchStage {
-    void *d_items = nullptr;     // StageArgs<T>[nitems]
+    void *d_items = nullptr;     // StageArgs<T>[nitems] (PlaneArgs<T>[nitems] for a plane stage)
     int64_t *d_prefix = nullptr; // [nitems + 1]
     int nitems = 0;
     int64_t total = 0;
     size_t lds = 0; // largest (table + lines) footprint of an item
     int threads = 64;
     int64_t grid = 0;
+    bool plane = false;            // k_plane_batch: fused two-axis plane stages
+    int tw_cap = 0, work_cap = 0, raw_cap = 0; // plane stages: LDS regions (complex values)
     bool multi = false;            // k_line_multi_batch over items of several compile-time plans
     bool fast = false, wide = false; // else one compile-time plan, or the generic kernel
     int fr[3] = {0, 0, 0};
     int fem = 0;
+};
+
+// A fused plane stage of one batched problem (host side of PlaneArgs).
+struct PlaneDesc {
+    int in_buf = BUF_SRC, out_buf = BUF_DST;
+    int64_t nplanes = 0;
+    int nr = 0, nc = 0;
+    int rr[3] = {0, 0, 0}, rem = 0, cr[3] = {0, 0, 0}, cem = 0;
+    int tpl_r = 1, tpl_c = 1, rs = 1;
+    int flags = 0;
+    double scale = 1.0;
+    int64_t tw_base_row = 0, tw_base_col = 0;
+    int twn_row = 0, twn_col = 0;
+    int twoff_row[3] = {0, 0, 0}, twoff_col[3] = {0, 0, 0};
+    size_t lds_bytes = 0;
+};
+
+// One launch item of a batched problem's operation: a line step or a fused
+// plane stage.
+struct BatchItem {
+    bool plane = false;
+    Step step;
+    PlaneDesc pd;
 };
 
 struct pfh_batch_s {
@@ -2429,6 +2456,8 @@ struct pfh_batch_s {
     int precision = PFH_F32;
     int threads = 256;
     std::vector<std::unique_ptr<PlanImpl>> plans;
+    // per problem, per op (FORWARD, INVERSE, BACKWARD): the launch items in order
+    std::vector<std::array<std::vector<BatchItem>, 3>> items;
     Tuning tuning;
     std::vector<int64_t> ws_off; // per problem
     int64_t ws_bytes = 0;
@@ -2467,9 +2496,155 @@ struct pfh_batch_s {
     }
 };
 
+// Threads per line of a compile-time plan: every pass's butterflies covered
+// by MB = EM / R of them per thread.
+static int plan_tpl(int M, const int r[3], int em)
+{
+    int need = 1;
+    for (int p = 0; p < 3; ++p) {
+        if (!r[p]) continue;
+        const int mb = std::max(1, em / r[p]);
+        need = std::max(need, ceil_div(M / r[p], mb));
+    }
+    return need;
+}
+
+// The launch items of one operation of a batched problem. A 3D f32 C2C
+// problem runs two consecutive route stages as one plane stage when both
+// lengths have a plane plan and the plane fits in LDS: two passes over HBM
+// instead of three; every FFT line is computed by the same radix plan as in
+// the route. The pair is the inner one (axes 1 and 0, planes (x, y) per z)
+// or the outer one (axes 2 and 1, planes (y, z) per x), whichever has the
+// smaller plane. The remaining line stage writes (forward inner, inverse
+// outer) or reads (forward outer, inverse inner) an intermediate whose
+// planes are the plane stage's output or input: memory order (z, x, y) for
+// the inner pair, (x, z, y) for the outer one (the plane stage reads rows of
+// its first axis and writes columns of its second axis contiguously).
+static pfh_status batch_items(const PlanImpl &P, int op, const Tuning &tu, std::vector<BatchItem> &out)
+{
+    const OpPlan &o = P.ops[op];
+    out.clear();
+    for (const Step &s : o.steps) {
+        BatchItem it;
+        it.step = s;
+        out.push_back(it);
+    }
+    if (!tu.batch_plane || P.precision != PFH_F32 || P.N != 3 || o.steps.size() != 3) return PFH_OK;
+    static const int plane_env = env_int("PFH_BATCH_PLANE", 1);
+    if (!plane_env) return PFH_OK;
+    const bool fwd = op == PFH_FORWARD;
+    // the plan's outer layouts: [extra][x][y][z] and [extra][z][y][x]
+    const Layout &xyz = fwd ? o.in : o.out, &zyx = fwd ? o.out : o.in;
+    if (!(xyz.perm[0] == 0 && xyz.perm[1] == 1 && xyz.perm[2] == 2)) return PFH_OK;
+    if (!(zyx.perm[0] == 2 && zyx.perm[1] == 1 && zyx.perm[2] == 0)) return PFH_OK;
+    for (const Step &s : o.steps)
+        if (s.mode != M_C2C) return PFH_OK;
+    auto plannable = [](const Step &s) {
+        return s.fast && plane_plan_available(4, s.fr[0], s.fr[1], s.fr[2], s.fem);
+    };
+    const int conj = F_CONJ_IN | F_CONJ_OUT;
+    struct Option {
+        bool ok = false;
+        int a = 0, b = 0, l = 0; // step indexes: plane rows, plane columns, line stage
+        bool line_first = false;
+        int mid0 = 0, mid1 = 0, mid2 = 0; // intermediate memory order
+        int64_t nplanes = 0;
+        PlaneDesc pd;
+    };
+    auto make = [&](int a, int b, int l, bool line_first, int m0, int m1, int m2, int64_t nplanes) {
+        Option op_;
+        const Step &A = o.steps[a], &Bs = o.steps[b];
+        if (!plannable(A) || !plannable(Bs)) return op_;
+        const int fa = A.flags & conj, fb = Bs.flags & conj;
+        if (fa != fb || (fa != 0 && fa != conj)) return op_; // one sign on both axes
+        const int nr = A.m, nc = Bs.m;
+        if (nr % 2 != 0) return op_; // the prefetch moves pairs of values
+        PlaneDesc &pd = op_.pd;
+        pd.nr = nr;
+        pd.nc = nc;
+        for (int r = 0; r < 3; ++r) {
+            pd.rr[r] = A.fr[r];
+            pd.cr[r] = Bs.fr[r];
+            pd.twoff_row[r] = A.twoff[r];
+            pd.twoff_col[r] = Bs.twoff[r];
+        }
+        pd.rem = A.fem;
+        pd.cem = Bs.fem;
+        pd.tpl_r = plan_tpl(nr, pd.rr, pd.rem);
+        pd.tpl_c = plan_tpl(nc, pd.cr, pd.cem);
+        if (pd.tpl_r > kPlaneThreads / 4 || pd.tpl_c > kPlaneThreads / 4) return op_; // any workgroup size
+        int rs = (nr - 1) + ((nr - 1) >> 4) + 1; // padded row (one slot per 16 values)
+        if (rs % 2 == 0) ++rs;
+        pd.rs = rs;
+        pd.flags = fa;
+        pd.scale = A.scale * Bs.scale;
+        pd.tw_base_row = A.tw_base;
+        pd.tw_base_col = Bs.tw_base;
+        pd.twn_row = A.twn;
+        pd.twn_col = Bs.twn;
+        pd.lds_bytes = (size_t)(A.twn + Bs.twn + (int64_t)nc * rs) * 8;
+        if (pd.lds_bytes + 1024 > P.dev.lds_limit) return op_;
+        pd.nplanes = nplanes;
+        op_.ok = true;
+        op_.a = a;
+        op_.b = b;
+        op_.l = l;
+        op_.line_first = line_first;
+        op_.mid0 = m0;
+        op_.mid1 = m1;
+        op_.mid2 = m2;
+        op_.nplanes = nplanes;
+        return op_;
+    };
+    // route steps: forward transforms axes 2, 1, 0; inverse / backward 0, 1, 2
+    const int64_t E = xyz.extra, X = xyz.shape[0], Z = xyz.shape[2];
+    const Option inner = fwd ? make(1, 2, 0, true, 2, 0, 1, E * Z) : make(0, 1, 2, false, 2, 0, 1, E * Z);
+    const Option outer = fwd ? make(0, 1, 2, false, 0, 2, 1, E * X) : make(1, 2, 0, true, 0, 2, 1, E * X);
+    const Option *pick = nullptr;
+    if (inner.ok) pick = &inner;
+    if (outer.ok && (!pick || outer.pd.lds_bytes < inner.pd.lds_bytes)) pick = &outer;
+    if (!pick) return PFH_OK;
+    Layout mid = xyz;
+    mid.perm[0] = pick->mid0;
+    mid.perm[1] = pick->mid1;
+    mid.perm[2] = pick->mid2;
+    if (o.w_bytes[0] < mid.bytes()) return PFH_OK;
+    Step line = o.steps[pick->l];
+    PlaneDesc pd = pick->pd;
+    if (pick->line_first) {
+        line.out = mid;
+        line.in_buf = BUF_SRC;
+        line.out_buf = BUF_W0;
+        pd.in_buf = BUF_W0;
+        pd.out_buf = BUF_DST;
+    } else {
+        line.in = mid;
+        line.in_buf = BUF_W0;
+        line.out_buf = BUF_DST;
+        pd.in_buf = BUF_SRC;
+        pd.out_buf = BUF_W0;
+    }
+    pfh_status st = finish_step(line, P.dev, 4, tu);
+    if (st) return st;
+    BatchItem li, pi;
+    li.step = line;
+    pi.plane = true;
+    pi.pd = pd;
+    out.clear();
+    if (pick->line_first) {
+        out.push_back(li);
+        out.push_back(pi);
+    } else {
+        out.push_back(pi);
+        out.push_back(li);
+    }
+    return PFH_OK;
+}
+
 static pfh_status batch_plan(pfh_batch_s &B, const pfh_desc *descs, int count, const Tuning &tu)
 {
     B.plans.clear();
+    B.items.clear();
     B.release_stages();
     B.tuning = tu;
     B.ws_off.assign(count, 0);
@@ -2493,6 +2668,12 @@ static pfh_status batch_plan(pfh_batch_s &B, const pfh_desc *descs, int count, c
             }
             need = std::max(need, op_ws_bytes(*P, o));
         }
+        std::array<std::vector<BatchItem>, 3> its;
+        for (int op : {PFH_FORWARD, PFH_INVERSE, PFH_BACKWARD}) {
+            st = batch_items(*P, op, tu, its[op]);
+            if (st) return st;
+        }
+        B.items.push_back(std::move(its));
         B.ws_off[i] = off;
         off += align256(need);
         B.plans.push_back(std::move(P));
@@ -2510,7 +2691,7 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
 {
     B.release_stages();
     int nst = 0;
-    for (auto &P : B.plans) nst = std::max(nst, (int)P->ops[op].steps.size());
+    for (auto &its : B.items) nst = std::max(nst, (int)its[op].size());
     const int cus = B.plans.empty() ? 256 : B.plans[0]->dev.cus;
     // Problem groups: consecutive problems whose arrays add up to at most the
     // group budget run all their stages before the next group starts, and
@@ -2550,14 +2731,69 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
         struct Group {
             BatchStage bs;
             std::vector<StageArgs<T>> items;
+            std::vector<PlaneArgs<T>> planes;
             std::vector<int64_t> prefix{0};
         };
         std::vector<Group> groups;
         for (int i = pg.first; i < pg.second; ++i) {
             const PlanImpl &P = *B.plans[i];
             const OpPlan &o = P.ops[op];
-            if (s >= (int)o.steps.size()) continue;
-            const Step &st = o.steps[s];
+            const std::vector<BatchItem> &its = B.items[i][op];
+            if (s >= (int)its.size()) continue;
+            if (its[s].plane) {
+                const PlaneDesc &pd = its[s].pd;
+                // Plane items go to launches by footprint: a plane that fits
+                // in a half / quarter of the LDS runs in 512- / 256-thread
+                // workgroups, two / four per CU (the kernel's registers allow
+                // 16 waves per CU), so their load, transform and store
+                // phases overlap; a launch's workgroups all take the LDS of
+                // its largest item.
+                const int64_t lim = (int64_t)P.dev.lds_limit;
+                const int64_t foot = (int64_t)pd.lds_bytes + 1024; // + one prefetch chunk
+                const int share = foot * 4 <= lim ? 4 : foot * 2 <= lim ? 2 : 1;
+                Group *g = nullptr;
+                for (Group &c : groups)
+                    if (c.bs.plane && c.bs.threads == kPlaneThreads / share) g = &c;
+                if (!g) {
+                    groups.emplace_back();
+                    g = &groups.back();
+                    g->bs.plane = true;
+                    g->bs.threads = kPlaneThreads / share;
+                }
+                WsView wv;
+                pfh_status e = view_workspace(P, o, static_cast<char *>(ws) + off[i], (size_t)(B.ws_bytes), wv);
+                if (e) return e;
+                void *bufs[4] = {const_cast<void *>(srcs[i]), dsts[i], wv.w[0], wv.w[1]};
+                PlaneArgs<T> pa{};
+                pa.src = bufs[pd.in_buf];
+                pa.dst = bufs[pd.out_buf];
+                pa.tw_row = reinterpret_cast<const cpx<T> *>(P.dev_tables) + pd.tw_base_row;
+                pa.tw_col = reinterpret_cast<const cpx<T> *>(P.dev_tables) + pd.tw_base_col;
+                pa.nplanes = pd.nplanes;
+                pa.nr = pd.nr;
+                pa.nc = pd.nc;
+                pa.twn_row = pd.twn_row;
+                pa.twn_col = pd.twn_col;
+                for (int r = 0; r < 3; ++r) {
+                    pa.twoff_row[r] = pd.twoff_row[r];
+                    pa.twoff_col[r] = pd.twoff_col[r];
+                    pa.rr[r] = pd.rr[r];
+                    pa.cr[r] = pd.cr[r];
+                }
+                pa.rem = pd.rem;
+                pa.cem = pd.cem;
+                pa.tpl_r = pd.tpl_r;
+                pa.tpl_c = pd.tpl_c;
+                pa.rs = pd.rs;
+                pa.flags = pd.flags;
+                pa.scale = (T)pd.scale;
+                g->planes.push_back(pa);
+                g->prefix.push_back(g->prefix.back() + pd.nplanes);
+                g->bs.tw_cap = std::max(g->bs.tw_cap, pd.twn_row + pd.twn_col);
+                g->bs.work_cap = std::max(g->bs.work_cap, pd.nc * pd.rs);
+                continue;
+            }
+            const Step &st = its[s].step;
             bool wide = false;
             for (int R : st.radices) wide = wide || is_wide_radix(R);
             // every compile-time plan of the standard register budget shares
@@ -2567,7 +2803,7 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
             Group *g = nullptr;
             for (Group &c : groups) {
                 const BatchStage &k = c.bs;
-                if (k.multi != multi || k.fast != (st.fast && !multi)) continue;
+                if (k.plane || k.multi != multi || k.fast != (st.fast && !multi)) continue;
                 if (multi || (st.fast ? (k.fr[0] == st.fr[0] && k.fr[1] == st.fr[1] && k.fr[2] == st.fr[2] &&
                                          k.fem == st.fem)
                                       : (k.wide == wide))) {
@@ -2595,17 +2831,33 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
         }
         for (Group &g : groups) {
             BatchStage bs = g.bs;
-            bs.nitems = (int)g.items.size();
+            if (bs.plane) {
+                // tables and working plane, then the prefetch region (16-byte
+                // aligned) with the rest of the workgroup's LDS share
+                if ((bs.tw_cap + bs.work_cap) % 2) ++bs.tw_cap;
+                const int share = kPlaneThreads / bs.threads;
+                const int64_t cap = (int64_t)(B.plans[pg.first]->dev.lds_limit / share / sizeof(cpx<T>));
+                bs.raw_cap = (int)std::max<int64_t>(0, cap - bs.tw_cap - bs.work_cap);
+                bs.lds = (size_t)(bs.tw_cap + bs.work_cap + bs.raw_cap) * sizeof(cpx<T>);
+            }
+            const size_t isz = bs.plane ? sizeof(PlaneArgs<T>) : sizeof(StageArgs<T>);
+            const size_t nit = bs.plane ? g.planes.size() : g.items.size();
+            const void *hdata = bs.plane ? static_cast<const void *>(g.planes.data())
+                                         : static_cast<const void *>(g.items.data());
+            bs.nitems = (int)nit;
             bs.total = g.prefix.back();
-            PFH_HIP_CHECK(hipMalloc(&bs.d_items, std::max<size_t>(1, g.items.size()) * sizeof(StageArgs<T>)));
+            PFH_HIP_CHECK(hipMalloc(&bs.d_items, std::max<size_t>(1, nit) * isz));
             B.stages.push_back(bs); // owned from here (freed by release_stages)
             BatchStage &own = B.stages.back();
             PFH_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&own.d_prefix), g.prefix.size() * sizeof(int64_t)));
-            PFH_HIP_CHECK(
-                hipMemcpy(own.d_items, g.items.data(), g.items.size() * sizeof(StageArgs<T>), hipMemcpyHostToDevice));
+            PFH_HIP_CHECK(hipMemcpy(own.d_items, hdata, nit * isz, hipMemcpyHostToDevice));
             PFH_HIP_CHECK(
                 hipMemcpy(own.d_prefix, g.prefix.data(), g.prefix.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-            const int per_cu = own.multi  ? line_multi_batch_blocks_per_cu<T>(own.threads, own.lds)
+            int plane_per_cu = 1;
+            if constexpr (std::is_same<T, float>::value)
+                if (own.plane) plane_per_cu = plane_batch_blocks_per_cu<T>(own.threads, own.lds);
+            const int per_cu = own.plane  ? plane_per_cu
+                               : own.multi ? line_multi_batch_blocks_per_cu<T>(own.threads, own.lds)
                                : own.fast ? line_fast_batch_blocks_per_cu<T>(own.fr[0], own.fr[1], own.fr[2],
                                                                              own.fem, own.threads, own.lds)
                                           : line_batch_blocks_per_cu<T>(own.wide, own.threads, own.lds);
@@ -2627,7 +2879,14 @@ static pfh_status batch_launch(pfh_batch_s &B, hipStream_t stream)
 {
     for (const BatchStage &bs : B.stages) {
         const StageArgs<T> *items = reinterpret_cast<const StageArgs<T> *>(bs.d_items);
-        if (bs.multi)
+        if (bs.plane) {
+            if constexpr (std::is_same<T, float>::value)
+                PFH_HIP_CHECK(launch_plane_batch<T>(reinterpret_cast<const PlaneArgs<T> *>(bs.d_items), bs.d_prefix,
+                                                    bs.nitems, bs.total, bs.threads, bs.tw_cap, bs.work_cap,
+                                                    bs.raw_cap, bs.grid, stream));
+            else
+                return set_err(PFH_ERR_UNSUPPORTED, "plane stages are f32 only");
+        } else if (bs.multi)
             PFH_HIP_CHECK(launch_line_multi_batch<T>(items, bs.d_prefix, bs.nitems, bs.total, bs.threads, bs.lds,
                                                      bs.grid, stream));
         else if (bs.fast)
@@ -2690,17 +2949,22 @@ static pfh_status batch_run(pfh_batch_s &B, int op, const void *const *srcs, voi
 // times a forward execution (device events); the fastest is kept.
 static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int count)
 {
+    // Candidates: threads per block of the line stages, column-tile lines,
+    // persistent workgroups or one per tile, plane stages on or off (every
+    // compile-time plan shares one launch: a launch per plan measured slower
+    // on every candidate of the 96-problem workload; the largest generic
+    // radix only matters to lengths without a compile-time plan)
     std::vector<Tuning> cands;
     for (int block : {128, 256, 512})
-        for (int maxr : {8, 16})
-            for (int col : {0, 32})
-                for (bool multi : {true, false})
-                    for (bool pers : {true, false}) {
+        for (int col : {0, 32})
+            for (bool plane : {true, false})
+                for (bool pers : {true, false}) {
                 Tuning t;
-                t.batch_multi = multi;
+                t.batch_multi = true;
                 t.batch_persistent = pers;
+                t.batch_plane = plane;
                 t.block = block;
-                t.max_radix = maxr;
+                t.max_radix = 8;
                 t.col_lines = col;
                 t.block_cap = kMaxThreadsWide;
                 t.allow_fast = true;
@@ -2809,9 +3073,17 @@ pfh_status pfh_batch_workspace_bytes(pfh_batch batch, size_t *bytes)
 }
 
 pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines, int *one_launch,
-                            int *persistent)
+                            int *persistent, int *plane)
 {
     if (!batch) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    if (plane) {
+        // fused plane stages on, and used by at least one problem
+        int used = 0;
+        for (const auto &its : batch->items)
+            for (const auto &v : its)
+                for (const BatchItem &it : v) used |= it.plane ? 1 : 0;
+        *plane = batch->tuning.batch_plane && used ? 1 : 0;
+    }
     if (block) *block = batch->tuning.block;
     if (max_radix) *max_radix = batch->tuning.max_radix;
     if (col_lines) *col_lines = batch->tuning.col_lines;

@@ -85,6 +85,18 @@ bool fast_plan_available(int tbytes, int r0, int r1, int r2, int em);
 bool multi_batch_plan_available(int tbytes, int r0, int r1, int r2, int em);
 hipError_t set_fast_lds_limit(size_t bytes);
 
+// Fused two-axis plane stage over a work list of items (pfh_plane.hip);
+// plane_plan_available: the (radices, values per thread) plan is in its list.
+// LDS: tw_cap + work_cap + raw_cap complex values (tables, working plane,
+// prefetch region; tw_cap + work_cap even).
+template <typename T>
+hipError_t launch_plane_batch(const PlaneArgs<T> *items, const int64_t *prefix, int nitems, int64_t total,
+                              int threads, int tw_cap, int work_cap, int raw_cap, int64_t grid, hipStream_t s);
+template <typename T>
+int plane_batch_blocks_per_cu(int threads, size_t lds_bytes);
+bool plane_plan_available(int tbytes, int r0, int r1, int r2, int em);
+hipError_t set_plane_lds_limit(size_t bytes);
+
 template <typename T>
 hipError_t launch_endpoint_check(const EndpointArgs &e, const void *data, EndpointPartial *part, int *gate,
                                  int *status, hipStream_t s);

@@ -151,6 +151,30 @@ struct StageArgs {
     T prew0, prew1, postw0, postw1;
 };
 
+// Two route stages fused over whole planes (k_plane_batch): a plane of nc
+// rows of nr contiguous elements is loaded into LDS, every row is
+// transformed (first FFT axis, length nr), then every column (second axis,
+// length nc), and the plane is written with the second axis contiguous
+// (column c at c * nc). Planes of one item lie back to back (plane p at
+// p * nr * nc in both arrays).
+template <typename T>
+struct PlaneArgs {
+    const void *src;
+    void *dst;
+    const cpx<T> *tw_row;   // pass twiddles of the row FFT's plan (offsets twoff_row)
+    const cpx<T> *tw_col;   // and of the column FFT's plan
+    int64_t nplanes;
+    int nr, nc;             // row length (first FFT), rows (second FFT length)
+    int twn_row, twn_col;   // table entries staged into LDS
+    int twoff_row[3], twoff_col[3];
+    int rr[3], rem;         // compile-time plan of the rows (radices, values per thread)
+    int cr[3], cem;         // and of the columns
+    int tpl_r, tpl_c;       // threads per row / column line
+    int rs;                 // LDS stride of a row (complex units, odd)
+    int flags;              // F_CONJ_IN (rows' load), F_CONJ_OUT (columns' store)
+    T scale;                // applied at the store
+};
+
 // Instantiated fast plans: {R0, R1, R2, EM}.
 // Compile-time plans for route stages only (the batched kernels, which
 // carry C2C lines, do not instantiate them): the 385-point FFT of a DST-I
@@ -185,5 +209,30 @@ struct StageArgs {
     X(double, 4, 4, 8, 8)      \
     X(double, 4, 8, 8, 8)      \
     X(double, 8, 8, 8, 8)
+
+// Threads of a plane-stage workgroup (the kernel's launch bound).
+#ifndef PFH_PLANE_THREADS
+#define PFH_PLANE_THREADS 1024
+#endif
+constexpr int kPlaneThreads = PFH_PLANE_THREADS;
+
+// Compile-time plans the plane stage dispatches on (f32, 16 values per
+// thread; lengths up to 384, longer lines do not leave room for a plane).
+#define PFH_PLANE_PLANS(X)     \
+    X(float, 8, 8, 0, 16)      \
+    X(float, 8, 16, 0, 16)     \
+    X(float, 16, 16, 0, 16)    \
+    X(float, 16, 3, 0, 16)     \
+    X(float, 4, 3, 5, 16)      \
+    X(float, 16, 5, 0, 16)     \
+    X(float, 4, 3, 7, 16)      \
+    X(float, 8, 4, 3, 16)      \
+    X(float, 16, 7, 0, 16)     \
+    X(float, 8, 3, 5, 16)      \
+    X(float, 8, 4, 5, 16)      \
+    X(float, 16, 4, 3, 16)     \
+    X(float, 16, 3, 5, 16)     \
+    X(float, 16, 4, 5, 16)     \
+    X(float, 16, 8, 3, 16)
 
 } // namespace pfh

@@ -224,7 +224,7 @@ def load_library(path: str = LIB_PATH):
     lib.pfh_batch_workspace_bytes.argtypes = [vp, ctypes.POINTER(sz)]
     lib.pfh_batch_execute.argtypes = [vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), vp, sz, vp, i]
     ip = ctypes.POINTER(i)
-    lib.pfh_batch_tuning.argtypes = [vp, ip, ip, ip, ip, ip]
+    lib.pfh_batch_tuning.argtypes = [vp, ip, ip, ip, ip, ip, ip]
     d = ctypes.c_double
     lib.pfh_pointwise_scale_axes.argtypes = [vp, i, vp, sz, vp, vp]
     lib.pfh_pointwise_scale_axes_complex.argtypes = [vp, i, vp, sz, vp, vp]
@@ -792,11 +792,11 @@ class BatchC2cPlan:
     def tuning(self):
         """(threads per block target, largest radix, lines per column tile,
         one launch per stage over all compile-time plans, persistent
-        workgroups rather than one workgroup per tile)."""
-        b, r, c, m, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        workgroups rather than one workgroup per tile, fused plane stages)."""
+        b, r, c, m, p, q = (ctypes.c_int() for _ in range(6))
         _check(_lib.pfh_batch_tuning(self._h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(c), ctypes.byref(m),
-                                     ctypes.byref(p)))
-        return b.value, r.value, c.value, bool(m.value), bool(p.value)
+                                     ctypes.byref(p), ctypes.byref(q)))
+        return b.value, r.value, c.value, bool(m.value), bool(p.value), bool(q.value)
 
     def allocate_workspace(self) -> torch.Tensor:
         return torch.zeros(max(self.workspace_bytes, 256), dtype=torch.uint8, device="cuda")

@@ -565,6 +565,80 @@ template int line_fast_batch_blocks_per_cu<float>(int, int, int, int, int, size_
 template int line_fast_batch_blocks_per_cu<double>(int, int, int, int, int, size_t);
 
 hipError_t set_line_stage_lds_limit(size_t) { return hipSuccess; }
+hipError_t set_plane_lds_limit(size_t) { return hipSuccess; }
+
+bool plane_plan_available(int tbytes, int r0, int r1, int r2, int em)
+{
+#define PFH_PLANE_HAS(TT, A, B, C, E) \
+    if ((int)sizeof(TT) == tbytes && r0 == A && r1 == B && r2 == C && em == E) return true;
+    PFH_PLANE_PLANS(PFH_PLANE_HAS)
+#undef PFH_PLANE_HAS
+    return false;
+}
+
+template <typename T>
+int plane_batch_blocks_per_cu(int, size_t)
+{
+    return 1;
+}
+template int plane_batch_blocks_per_cu<float>(int, size_t);
+
+// Plane stage: every plane's element (row r, column c) is read from
+// src + r nr + c and written to dst + c nc + r (the two transforms replaced by
+// the identity); shape checks against the kernel's assumptions.
+template <typename T>
+hipError_t launch_plane_batch(const PlaneArgs<T> *items, const int64_t *prefix, int nitems, int64_t total,
+                              int threads, int tw_cap, int work_cap, int raw_cap, int64_t grid, hipStream_t)
+{
+    ++g_launches;
+    ++g_all_launches;
+    begin_launch();
+    const size_t lds_bytes = (size_t)(tw_cap + work_cap + raw_cap) * sizeof(cpx<T>);
+    if ((tw_cap + work_cap) % 2 != 0 || raw_cap < 0) fail("plane_batch: LDS regions");
+    if (threads != kPlaneThreads && threads != kPlaneThreads / 2 && threads != kPlaneThreads / 4)
+        fail("plane_batch: %d threads", threads);
+    if (lds_bytes > kLdsLimit) fail("plane_batch: %zu bytes of LDS", lds_bytes);
+    if (grid <= 0 || grid > total) fail("plane_batch: grid");
+    if (prefix[0] != 0 || prefix[nitems] != total)
+        fail("plane_batch: prefix ends %lld %lld %lld (nitems %d)", (long long)prefix[0], (long long)prefix[nitems], (long long)total, nitems);
+    for (int q = 0; q < nitems; ++q) {
+        const PlaneArgs<T> &a = items[q];
+        if (prefix[q + 1] - prefix[q] != a.nplanes) fail("plane_batch: item %d planes", q);
+        if (!plane_plan_available((int)sizeof(T), a.rr[0], a.rr[1], a.rr[2], a.rem) ||
+            !plane_plan_available((int)sizeof(T), a.cr[0], a.cr[1], a.cr[2], a.cem))
+            fail("plane_batch: item %d has no plane plan", q);
+        const int mr = a.rr[0] * a.rr[1] * (a.rr[2] ? a.rr[2] : 1), mc = a.cr[0] * a.cr[1] * (a.cr[2] ? a.cr[2] : 1);
+        if (mr != a.nr || mc != a.nc) fail("plane_batch: item %d plan lengths %d/%d vs %d/%d", q, mr, mc, a.nr, a.nc);
+        // every pass's butterflies are covered by the threads of a line
+        auto covered = [](int M, const int r[3], int em, int tpl) {
+            for (int p = 0; p < 3; ++p)
+                if (r[p] && tpl * std::max(1, em / r[p]) < M / r[p]) return false;
+            return tpl <= 256;
+        };
+        if (!covered(mr, a.rr, a.rem, a.tpl_r) || !covered(mc, a.cr, a.cem, a.tpl_c))
+            fail("plane_batch: item %d threads per line", q);
+        if (a.rs < (a.nr - 1) + ((a.nr - 1) >> 4) + 1 || a.rs % 2 == 0) fail("plane_batch: item %d row stride", q);
+        // the load phase moves pairs of values: even rows, at most one pair per thread per row
+        if (a.nr % 2 != 0 || a.nr / 2 > threads) fail("plane_batch: item %d row length %d", q, a.nr);
+        if (a.twn_row + a.twn_col > tw_cap || (int64_t)a.nc * a.rs > work_cap) fail("plane_batch: item %d LDS", q);
+        check_read("plane_batch tables", a.tw_row, (size_t)a.twn_row * sizeof(cpx<T>));
+        check_read("plane_batch tables", a.tw_col, (size_t)a.twn_col * sizeof(cpx<T>));
+        const int64_t pe = (int64_t)a.nr * a.nc;
+        for (int64_t p = 0; p < a.nplanes; ++p) {
+            const cpx<T> *s = static_cast<const cpx<T> *>(a.src) + p * pe;
+            cpx<T> *d = static_cast<cpx<T> *>(a.dst) + p * pe;
+            if (!check_read("plane_batch", s, (size_t)pe * sizeof(cpx<T>))) return hipSuccess;
+            for (int r = 0; r < a.nc; ++r)
+                for (int c = 0; c < a.nr; ++c) {
+                    if (!check_write("plane_batch", d + (int64_t)c * a.nc + r, sizeof(cpx<T>))) return hipSuccess;
+                    d[(int64_t)c * a.nc + r] = s[(int64_t)r * a.nr + c];
+                }
+        }
+    }
+    return hipSuccess;
+}
+template hipError_t launch_plane_batch<float>(const PlaneArgs<float> *, const int64_t *, int, int64_t, int, int, int,
+                                              int, int64_t, hipStream_t);
 hipError_t set_fast_lds_limit(size_t) { return hipSuccess; }
 
 template <typename T>
@@ -1094,7 +1168,12 @@ void route_cases(int prec)
 
 void batch_cases()
 {
-    std::vector<std::vector<size_t>> shapes = {{48, 60, 64}, {12, 7, 9}, {84, 16, 20}, {5, 5, 5}, {64, 32, 16}};
+    // the first, {128, 112, 20}, {96, 60, 7} and {160, 84, 5} fuse the inner
+    // stage pair into plane stages ((x, y) planes), {160, 128, 48} the outer
+    // pair ((y, z) planes: its (x, y) plane does not fit in LDS)
+    std::vector<std::vector<size_t>> shapes = {{48, 60, 64},   {12, 7, 9},   {84, 16, 20}, {5, 5, 5},
+                                               {64, 32, 16},   {128, 112, 20}, {96, 60, 7}, {160, 84, 5},
+                                               {160, 128, 48}};
     std::vector<pfh_desc> ds;
     for (auto &s : shapes) ds.push_back(route_desc(PFH_C2C, PFH_F32, s, {3}));
     // mode 2: problems in groups of at most 1 MiB that share one workspace region

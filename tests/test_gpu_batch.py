@@ -50,9 +50,9 @@ def test_small_batch_matches_oracle(dt, autotune):
     torch_dt = torch.complex64 if dt == np.complex64 else torch.complex128
     plan = pf.BatchC2cPlan(shapes, dtype=torch_dt, autotune=autotune, extra_shapes=[(3,)] * len(shapes))
     if autotune:
-        block, maxr, col, one, persistent = plan.tuning()
+        block, maxr, col, one, persistent, plane = plan.tuning()
         assert block in (128, 256, 512) and maxr in (8, 16) and col in (0, 32) and one in (True, False)
-        assert persistent in (True, False)
+        assert persistent in (True, False) and not plane  # no length here has a plane plan
     srcs = [_tdev(x) for x in xs]
     for op in (O.FORWARD, O.INVERSE, O.BACKWARD):
         for use_graph in (False, True):
@@ -102,6 +102,35 @@ def test_rejects_short_workspace():
     x = torch.zeros(64, dtype=torch.complex64, device="cuda")
     with pytest.raises(pf.PfhError):
         plan.forward([x], workspace=torch.zeros(0, dtype=torch.uint8, device="cuda"))
+
+
+@pytest.mark.parametrize("extra", [1, 3])
+def test_plane_stages_match_oracle_and_per_problem_plans(extra):
+    """Problems with plane plans on two consecutive route axes run those two
+    stages as one fused plane stage: the inner pair (forward: axes 1 then 0;
+    inverse / backward: axes 0 then 1; e.g. (48, 60, 64), (160, 84, 5)) or
+    the outer pair where its plane is smaller ((112, 96, 48),
+    (160, 128, 48), whose (x, y) plane does not fit in LDS): every
+    operation against the oracle and against the problem's own three-stage
+    C2cPlan, with and without a graph; the other problems keep their line
+    stages in the same launches."""
+    rng = np.random.default_rng(60 + extra)
+    shapes = [(48, 60, 64), (128, 112, 20), (96, 60, 7), (160, 84, 5), (64, 160, 3), (112, 96, 48),
+              (160, 128, 48), (12, 7, 9), (84, 16, 20), (60, 48, 1)]
+    xs = _inputs(rng, shapes, extra, np.complex64)
+    ex = [(extra,) if extra > 1 else ()] * len(shapes)
+    plan = pf.BatchC2cPlan(shapes, autotune=False, extra_shapes=ex)
+    assert plan.tuning()[5], "plane stages are expected for these shapes"
+    srcs = [_tdev(x) for x in xs]
+    for op in (O.FORWARD, O.INVERSE, O.BACKWARD):
+        for use_graph in (False, True):
+            outs = plan.execute(op, srcs, use_graph=use_graph)
+            for k, (s, x) in enumerate(zip(shapes, xs)):
+                got = _host(outs[k])
+                ref = O.transform(O.KIND_C2C, x.astype(np.complex128), op, shape=s, extra=extra)
+                assert rel_l2(got, ref) <= tol(np.complex64), (op, use_graph, k, s)
+                single = pf.C2cPlan(s, extra_shape=ex[k]).execute(op, srcs[k])
+                assert rel_l2(got, _host(single)) <= 2 * tol(np.complex64), (op, use_graph, k, s)
 
 
 @pytest.mark.slow
