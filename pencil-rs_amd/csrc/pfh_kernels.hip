@@ -90,19 +90,11 @@ __device__ __forceinline__ void LRset(cpx<T> *ln, int p, T v)
 
 // One input value of a real line after the reversal / alternating sign /
 // adjoint edge weights requested by the stage.
-// Raw real line values for the pre-processing: from the tile's LDS line
-// (generic kernel) or straight from global memory (fast kernels' first pass).
+// Raw real line values for the pre-processing, read from the tile's LDS line.
 template <typename T>
 struct LdsReal {
     const cpx<T> *ln;
     __device__ __forceinline__ T operator()(int p) const { return LR(ln, p); }
-};
-
-template <typename T, bool NT>
-struct GlobalReal {
-    const T *s;
-    int64_t ips;
-    __device__ __forceinline__ T operator()(int p) const { return gld<NT>(s + (int64_t)p * ips); }
 };
 
 template <typename T, typename Rd>
@@ -840,34 +832,35 @@ __device__ __forceinline__ void fast_first(const StageArgs<T> &a, cpx<T> *ln, bo
     __syncthreads();
 }
 
-// First pass of a real-input R2R line: the pre-processing gathers each FFT
-// input value from global memory (pre_real through GlobalReal: reordering,
-// odd / even extension, pre-twiddles from the staged tables) into registers.
-template <typename T, int R, int EM, int M, bool NT>
-__device__ __forceinline__ void fast_first_pre(const StageArgs<T> &a, cpx<T> *ln, const cpx<T> *tws, bool valid,
-                                               int64_t io, int tl, int tpl)
+// First pass of a real-input R2R line whose raw values are already in the
+// LDS line (stage_load): the pre-processing gathers each FFT input value from
+// LDS (reordering, odd / even extension, pre-twiddles), then, after every
+// thread of the tile has read its values, the radix-R0 butterflies write the
+// pass output over the raw line.
+template <typename T, int R, int EM, int M>
+__device__ __forceinline__ void fast_first_pre_lds(const StageArgs<T> &a, cpx<T> *ln, const cpx<T> *tws, int tl,
+                                                   int tpl)
 {
     constexpr int MB = fast_mb<EM, R>();
     constexpr int NB = M / R;
     cpx<T> v[MB][R];
-    if (valid) {
-        const GlobalReal<T, NT> rd{reinterpret_cast<const T *>(a.src) + io, a.ips};
+    const LdsReal<T> rd{ln};
 #pragma unroll
-        for (int b = 0; b < MB; ++b) {
-            const int j = tl + tpl * b;
-            if (j < NB) {
+    for (int b = 0; b < MB; ++b) {
+        const int j = tl + tpl * b;
+        if (j < NB) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[b][r] = pre_real<T>(a, rd, tws, j + r * NB);
-            }
+            for (int r = 0; r < R; ++r) v[b][r] = pre_real<T>(a, rd, tws, j + r * NB);
         }
+    }
+    __syncthreads();
 #pragma unroll
-        for (int b = 0; b < MB; ++b) {
-            const int j = tl + tpl * b;
-            if (j < NB) {
-                Dft<T, R>::run(v[b]);
+    for (int b = 0; b < MB; ++b) {
+        const int j = tl + tpl * b;
+        if (j < NB) {
+            Dft<T, R>::run(v[b]);
 #pragma unroll
-                for (int r = 0; r < R; ++r) Lset(ln, j * R + r, v[b][r]);
-            }
+            for (int r = 0; r < R; ++r) Lset(ln, j * R + r, v[b][r]);
         }
     }
     __syncthreads();
@@ -1159,15 +1152,15 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
     cpx<T> *ln = data + l * a.ls;
     const int mode = a.mode;
     if constexpr (KIND == 2) {
-        // real-input R2R modes: gathered first pass, generic post-processing
+        // real-input R2R modes: the raw lines go to LDS with coalesced loads
+        // (the generic kernel's tile load), the first pass gathers its
+        // pre-processed inputs from there; generic post-processing at the
+        // store. (Gathering straight from global memory, 4-byte loads at
+        // the reorder's strides, made the 256x384x240 f32 R2R stages run at
+        // 0.8-2.0 TB/s.)
         __syncthreads(); // the pre-processing reads the staged tables
-        const int fl = !act ? 0 : (a.in_col ? tid % lt : l);
-        const int ft = !act ? (1 << 28) : (a.in_col ? tid / lt : tl);
-        const int64_t line = tile * lt + fl;
-        int64_t io = 0, oo = 0;
-        const bool valid = act && line < a.nlines;
-        if (valid) line_offsets(a.g, line, io, oo);
-        fast_first_pre<T, R0, EM, M, NT>(a, data + fl * a.ls, tws, valid, io, ft, tpl);
+        stage_load<T, NT>(a, data, tile, tid, l, act ? tl : (1 << 28));
+        fast_first_pre_lds<T, R0, EM, M>(a, ln, tws, tl, tpl);
         if constexpr (R2 != 0) fast_mid<T, R1, EM, M>(ln, tws + a.twoff[1], R0, tl, tpl);
         fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
         if (act) {

@@ -1463,6 +1463,27 @@ static void contract_cases()
 
 int main(int argc, char **argv)
 {
+    if (argc == 7 && std::strcmp(argv[1], "plan") == 0) {
+        // one R2R (DCT-II, DST-I, DCT-IV) or DHT plan: emu plan r2r|dht n0 n1 n2 f32|f64
+        // (launch shapes printed with PFH_EMU_TRACE set)
+        const bool dht = std::strcmp(argv[2], "dht") == 0, f32 = std::strcmp(argv[6], "f32") == 0;
+        std::vector<size_t> shape = {(size_t)std::atol(argv[3]), (size_t)std::atol(argv[4]), (size_t)std::atol(argv[5])};
+        pfh_desc d = route_desc(PFH_R2R, f32 ? PFH_F32 : PFH_F64, shape, {});
+        const int kinds[3] = {PFH_DCT2, PFH_DST1, PFH_DCT4};
+        for (int a = 0; a < 3; ++a) d.axis_kind[a] = dht ? PFH_DHT : kinds[a];
+        pfh_plan p = nullptr;
+        if (pfh_plan_create(&d, &p) != PFH_OK) return 2;
+        for (int op : {PFH_FORWARD, PFH_INVERSE}) {
+            std::printf("op %d\n", op);
+            if (f32)
+                run_case<float>(argv[2], p, 3, op, true, true, -1, 0);
+            else
+                run_case<double>(argv[2], p, 3, op, true, true, -1, 0);
+        }
+        pfh_plan_destroy(p);
+        std::printf("launches %d, failures %d\n", pfh::g_launches, g_fail);
+        return g_fail ? 1 : 0;
+    }
     if (argc == 2 && std::strcmp(argv[1], "contract") == 0) {
         contract_cases();
         std::printf("launches %d, failures %d\n", pfh::g_launches, g_fail);
