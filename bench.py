@@ -211,9 +211,31 @@ def batched_vs_plain(torch, pf, reps: int = 5):
     g.manual_seed(4)
     srcs = [torch.randn(int(np.prod(s)) * 3, generator=g, dtype=torch.complex64, device="cuda") for s in shapes]
     dsts = [torch.empty_like(t) for t in srcs]
+    # the tunings go to a file of this run (unless PFH_TUNE_CACHE names one),
+    # so the first creation measures; a second creation, after the process
+    # map is cleared as in a new process, reads the file
+    tune_dir = None
+    if "PFH_TUNE_CACHE" not in os.environ:
+        import tempfile
+
+        tune_dir = tempfile.mkdtemp(prefix="pfh_tune_")
+        os.environ["PFH_TUNE_CACHE"] = os.path.join(tune_dir, "batch_tuning.txt")
+    pf.clear_tuning_cache()
     t0 = time.perf_counter()
     batch = pf.BatchC2cPlan(shapes, autotune=True, extra_shapes=[(3,)] * 96)
     tune_s = time.perf_counter() - t0
+    first_source = batch.tuning_source
+    pf.clear_tuning_cache()
+    t0 = time.perf_counter()
+    again = pf.BatchC2cPlan(shapes, autotune=True, extra_shapes=[(3,)] * 96)
+    cached_s = time.perf_counter() - t0
+    cached = {"create_s": cached_s, "source": again.tuning_source, "same_tuning": again.tuning() == batch.tuning()}
+    del again
+    if tune_dir is not None:
+        import shutil
+
+        del os.environ["PFH_TUNE_CACHE"]
+        shutil.rmtree(tune_dir, ignore_errors=True)
     bws = batch.allocate_workspace()
     plans = [pf.C2cPlan(s, extra_shape=(3,)) for s in shapes]
     pws = [p.allocate_workspace() for p in plans]
@@ -245,6 +267,8 @@ def batched_vs_plain(torch, pf, reps: int = 5):
         "batched_points_per_s": elems / tb,
         "tuning": list(batch.tuning()),
         "autotune_s": tune_s,
+        "autotune_source": first_source,
+        "cached_creation": cached,
     }
 
 

@@ -342,9 +342,30 @@ typedef struct pfh_batch_s *pfh_batch;
  * plans (PFH_C2C descriptors selecting every axis), all of one precision,
  * uneven shapes allowed; ops PFH_FORWARD, PFH_INVERSE, PFH_BACKWARD.
  * Tiles, radix split and lines per block come from the autotuner when
- * autotune != 0 (measured on the current device), else from the planner's
- * defaults. */
+ * autotune != 0, else from the planner's defaults.
+ *
+ * The autotuner times every candidate tuning on the current device and
+ * keeps the fastest under a key of the descriptors, the precision and the
+ * device (architecture name, CU count, LDS per workgroup): in the process,
+ * and appended to a cache file that later processes read, so a batch of the
+ * same problems on the same kind of device is measured once. The file is
+ * the environment's PFH_TUNE_CACHE (empty, "0" or "off": the process only),
+ * else $XDG_CACHE_HOME/pencil-fft-hip/batch_tuning.txt, else
+ * $HOME/.cache/pencil-fft-hip/batch_tuning.txt; a file that cannot be read
+ * or written means measuring again. */
 pfh_status pfh_batch_create(const pfh_desc *descs, int count, int autotune, pfh_batch *out);
+
+/* Where a batch's tuning came from (pfh_batch_tuning_source). */
+enum {
+    PFH_TUNED_DEFAULT = 0,       /* autotune == 0: the planner's defaults */
+    PFH_TUNED_MEASURED = 1,      /* timed on the device by this creation */
+    PFH_TUNED_PROCESS_CACHE = 2, /* measured earlier in this process */
+    PFH_TUNED_FILE_CACHE = 3     /* read from the cache file */
+};
+pfh_status pfh_batch_tuning_source(pfh_batch batch, int *source);
+/* Forgets the tunings measured or read in this process (the cache file is
+ * kept): the next autotuned creation reads the file or measures again. */
+void pfh_batch_tuning_cache_clear(void);
 void pfh_batch_destroy(pfh_batch batch);
 pfh_status pfh_batch_workspace_bytes(pfh_batch batch, size_t *bytes);
 /* srcs[i] / dsts[i]: device buffers of problem i (input / output layout of
@@ -359,9 +380,10 @@ pfh_status pfh_batch_execute(pfh_batch batch, int op, const void *const *srcs, v
  * per tile on column-access stages, whether the items of all compile-time
  * radix plans of a stage share one launch, whether a launch keeps one
  * resident set of workgroups looping over the tiles or takes one workgroup
- * per tile, and whether fused plane stages run: the two inner route stages of
- * a 3D f32 problem in one pass over whole (axis 0, axis 1) planes in LDS), as
- * chosen by the autotuner. */
+ * per tile, and whether fused plane stages run: two route stages of a 3D f32
+ * problem in one pass over whole planes in LDS, the planes of the inner or
+ * of the outer axis pair, whichever are smaller), as chosen by the
+ * autotuner. */
 pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines, int *one_launch,
                             int *persistent, int *plane);
 

@@ -25,6 +25,7 @@
 
 #include "pfh_collectives.hpp"
 #include "pfh_plan.hpp"
+#include "pfh_tune_cache.hpp"
 
 using namespace pfh;
 
@@ -104,6 +105,7 @@ static pfh_status device_info(DeviceInfo &out)
         DeviceInfo d;
         d.device = dev;
         d.cus = prop.multiProcessorCount;
+        std::snprintf(d.arch, sizeof d.arch, "%s", prop.gcnArchName);
         // The planner's budget leaves kStaticLds bytes per workgroup for the
         // kernels' static LDS (the pass plan copy of k_line_stage).
         constexpr size_t kStaticLds = 256;
@@ -2461,6 +2463,7 @@ struct pfh_batch_s {
     Tuning tuning;
     std::vector<int64_t> ws_off; // per problem
     int64_t ws_bytes = 0;
+    int tune_source = PFH_TUNED_DEFAULT;
     std::vector<BatchStage> stages;
     // pointers the uploaded work list was built for
     int cur_op = -1;
@@ -2945,10 +2948,49 @@ static pfh_status batch_run(pfh_batch_s &B, int op, const void *const *srcs, voi
     return PFH_OK;
 }
 
+// The tuning of one autotuner candidate.
+static Tuning batch_candidate(int block, int col_lines, bool plane, bool persistent, bool multi, int max_radix)
+{
+    Tuning t;
+    t.batch_multi = multi;
+    t.batch_persistent = persistent;
+    t.batch_plane = plane;
+    t.block = block;
+    t.max_radix = max_radix;
+    t.col_lines = col_lines;
+    t.block_cap = kMaxThreadsWide;
+    t.allow_fast = true;
+    t.allow_xcd = false;
+    t.slab_mb = 0;
+    t.fill_block = true;
+    return t;
+}
+
 // Measured autotuning: every candidate tuning re-plans the whole batch and
-// times a forward execution (device events); the fastest is kept.
+// times forward executions (device events); the fastest is kept, and
+// recorded in the tuning cache under the batch's key. A batch whose key is
+// cached is planned with the recorded tuning without measuring.
 static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int count)
 {
+    DeviceInfo dev;
+    {
+        pfh_status st = device_info(dev);
+        if (st) return st;
+    }
+    const std::string key = batch_tune_key(dev.arch, dev.cus, dev.lds_limit, descs, count);
+    {
+        TuneRecord rec;
+        int source = PFH_TUNED_DEFAULT;
+        if (tune_cache_lookup(key, rec, &source)) {
+            const Tuning t = batch_candidate(rec.block, rec.col_lines, rec.plane != 0, rec.persistent != 0,
+                                             rec.multi != 0, rec.max_radix);
+            if (batch_plan(B, descs, count, t) == PFH_OK) {
+                B.tune_source = source;
+                return ok();
+            }
+            tune_cache_forget(key); // a record this build cannot plan: measure again
+        }
+    }
     // Candidates: threads per block of the line stages, column-tile lines,
     // persistent workgroups or one per tile, plane stages on or off (every
     // compile-time plan shares one launch: a launch per plan measured slower
@@ -2958,22 +3000,10 @@ static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int coun
     for (int block : {128, 256, 512})
         for (int col : {0, 32})
             for (bool plane : {true, false})
-                for (bool pers : {true, false}) {
-                Tuning t;
-                t.batch_multi = true;
-                t.batch_persistent = pers;
-                t.batch_plane = plane;
-                t.block = block;
-                t.max_radix = 8;
-                t.col_lines = col;
-                t.block_cap = kMaxThreadsWide;
-                t.allow_fast = true;
-                t.allow_xcd = false;
-                t.slab_mb = 0;
-                t.fill_block = true;
-                cands.push_back(t);
-                }
-    // scratch buffers for the measurement
+                for (bool pers : {true, false}) cands.push_back(batch_candidate(block, col, plane, pers, true, 8));
+    // measurement buffers: inputs hold finite nonzero values (every byte
+    // 0x3c), so the timed launches move and compute what real data makes
+    // them move and compute
     std::vector<void *> bufs;
     std::vector<const void *> srcs(count);
     std::vector<void *> dsts(count);
@@ -2981,7 +3011,6 @@ static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int coun
         for (void *p : bufs) (void)hipFree(p);
         bufs.clear();
     };
-    int64_t maxws = 0;
     {
         pfh_status st = batch_plan(B, descs, count, cands[0]);
         if (st) return st;
@@ -2990,10 +3019,11 @@ static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int coun
         const Layout &L = B.plans[i]->ops[PFH_FORWARD].in;
         void *a = nullptr, *b = nullptr;
         if (hipMalloc(&a, (size_t)L.bytes()) != hipSuccess || hipMalloc(&b, (size_t)L.bytes()) != hipSuccess) {
+            if (a) (void)hipFree(a);
             cleanup();
             return set_err(PFH_ERR_ALLOCATION_FAILED, "autotune buffers");
         }
-        (void)hipMemset(a, 0, (size_t)L.bytes());
+        (void)hipMemset(a, 0x3c, (size_t)L.bytes());
         bufs.push_back(a);
         bufs.push_back(b);
         srcs[i] = a;
@@ -3004,39 +3034,67 @@ static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int coun
         cleanup();
         return set_err(PFH_ERR_DEVICE, "stream");
     }
-    hipEvent_t e0, e1;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    double best = 1e30;
-    Tuning best_t = cands[0];
+    constexpr int kMaxReps = 9;
+    hipEvent_t ev[kMaxReps + 1];
+    for (hipEvent_t &e : ev) (void)hipEventCreate(&e);
+    int64_t maxws = 0;
     void *ws = nullptr;
-    for (const Tuning &t : cands) {
-        pfh_status st = batch_plan(B, descs, count, t);
-        if (st) continue;
+    // The median of `reps` executions of candidate t, each timed on its own
+    // after one untimed warm-up; negative if t does not plan or run.
+    auto time_candidate = [&](const Tuning &t, int reps) -> double {
+        if (batch_plan(B, descs, count, t)) return -1;
         if (B.ws_bytes > maxws) {
             if (ws) (void)hipFree(ws);
             ws = nullptr;
-            if (hipMalloc(&ws, (size_t)B.ws_bytes) != hipSuccess) break;
+            maxws = 0;
+            if (hipMalloc(&ws, (size_t)B.ws_bytes) != hipSuccess) return -1;
             maxws = B.ws_bytes;
         }
-        if (batch_run(B, PFH_FORWARD, srcs.data(), dsts.data(), ws, stream, 0)) continue;
-        (void)hipEventRecord(e0, stream);
-        for (int r = 0; r < 3; ++r) (void)batch_run(B, PFH_FORWARD, srcs.data(), dsts.data(), ws, stream, 0);
-        (void)hipEventRecord(e1, stream);
-        (void)hipEventSynchronize(e1);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best) {
-            best = ms;
-            best_t = t;
+        if (batch_run(B, PFH_FORWARD, srcs.data(), dsts.data(), ws, stream, 0)) return -1;
+        (void)hipEventRecord(ev[0], stream);
+        for (int r = 0; r < reps; ++r) {
+            if (batch_run(B, PFH_FORWARD, srcs.data(), dsts.data(), ws, stream, 0)) return -1;
+            (void)hipEventRecord(ev[r + 1], stream);
         }
+        (void)hipEventSynchronize(ev[reps]);
+        std::vector<float> ms(reps);
+        for (int r = 0; r < reps; ++r) (void)hipEventElapsedTime(&ms[r], ev[r], ev[r + 1]);
+        std::nth_element(ms.begin(), ms.begin() + reps / 2, ms.end());
+        return ms[reps / 2];
+    };
+    // Round 1: every candidate, median of 5. Round 2: the four fastest again,
+    // median of 9, each keeping the better of its two medians, so that one
+    // disturbed round does not decide.
+    std::vector<std::pair<double, int>> score;
+    for (int c = 0; c < (int)cands.size(); ++c) {
+        const double m = time_candidate(cands[c], 5);
+        if (m >= 0) score.push_back({m, c});
     }
+    std::sort(score.begin(), score.end());
+    for (int i = 0; i < (int)score.size() && i < 4; ++i) {
+        const double m = time_candidate(cands[score[i].second], kMaxReps);
+        if (m >= 0 && m < score[i].first) score[i].first = m;
+    }
+    std::sort(score.begin(), score.end());
     if (ws) (void)hipFree(ws);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    for (hipEvent_t &e : ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(stream);
     cleanup();
-    return batch_plan(B, descs, count, best_t);
+    if (score.empty()) return batch_plan(B, descs, count, cands[0]);
+    const Tuning &best = cands[score[0].second];
+    pfh_status st = batch_plan(B, descs, count, best);
+    if (st) return st;
+    TuneRecord rec;
+    rec.block = best.block;
+    rec.max_radix = best.max_radix;
+    rec.col_lines = best.col_lines;
+    rec.multi = best.batch_multi ? 1 : 0;
+    rec.persistent = best.batch_persistent ? 1 : 0;
+    rec.plane = best.batch_plane ? 1 : 0;
+    rec.ms = score[0].first;
+    tune_cache_store(key, rec);
+    B.tune_source = PFH_TUNED_MEASURED;
+    return ok();
 }
 
 extern "C" {
@@ -3071,6 +3129,15 @@ pfh_status pfh_batch_workspace_bytes(pfh_batch batch, size_t *bytes)
     *bytes = (size_t)std::max<int64_t>(batch->ws_bytes, 256);
     return ok();
 }
+
+pfh_status pfh_batch_tuning_source(pfh_batch batch, int *source)
+{
+    if (!batch || !source) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
+    *source = batch->tune_source;
+    return ok();
+}
+
+void pfh_batch_tuning_cache_clear(void) { tune_cache_clear(); }
 
 pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines, int *one_launch,
                             int *persistent, int *plane)

@@ -122,6 +122,7 @@ struct DeviceInfo {
     int device = 0;
     size_t lds_limit = 65536;
     int cus = 256;
+    char arch[64] = {}; // architecture name with its feature suffixes (gfx950:sramecc+:xnack-)
 };
 
 // Planner knobs (defaults, or chosen by the batch autotuner).

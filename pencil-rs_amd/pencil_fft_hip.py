@@ -185,6 +185,7 @@ EXPORTED_SYMBOLS = (
     "pfh_plan_workspace_bytes", "pfh_plan_layout", "pfh_execute", "pfh_execute_in_place", "pfh_local_execute",
     "pfh_local_execute_in_place", "pfh_execute_with_timing", "pfh_workspace_take_status", "pfh_workspace_init", "pfh_execute_noalloc", "pfh_batch_create",
     "pfh_batch_destroy", "pfh_batch_workspace_bytes", "pfh_batch_execute", "pfh_batch_tuning",
+    "pfh_batch_tuning_source", "pfh_batch_tuning_cache_clear",
     "pfh_pointwise_scale_axes", "pfh_pointwise_scale_axes_complex", "pfh_pointwise_quadratic", "pfh_pointwise_quadratic_adjoint", "pfh_sum_squares",
     "pfh_reduce_workspace_bytes", "pfh_reduce", "pfh_pointwise", "pfh_io_workspace_bytes", "pfh_io_write",
     "pfh_io_read", "pfh_io_read_raw", "pfh_io_catalog", "pfh_io_write_named", "pfh_io_append_named",
@@ -225,6 +226,9 @@ def load_library(path: str = LIB_PATH):
     lib.pfh_batch_execute.argtypes = [vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), vp, sz, vp, i]
     ip = ctypes.POINTER(i)
     lib.pfh_batch_tuning.argtypes = [vp, ip, ip, ip, ip, ip, ip]
+    lib.pfh_batch_tuning_source.argtypes = [vp, ip]
+    lib.pfh_batch_tuning_cache_clear.argtypes = []
+    lib.pfh_batch_tuning_cache_clear.restype = None
     d = ctypes.c_double
     lib.pfh_pointwise_scale_axes.argtypes = [vp, i, vp, sz, vp, vp]
     lib.pfh_pointwise_scale_axes_complex.argtypes = [vp, i, vp, sz, vp, vp]
@@ -755,6 +759,16 @@ class AllToAllvTransposePlan(LocalTransposePlan):
     _kind = ALLTOALLV_TRANSPOSE
 
 
+TUNING_SOURCES = {0: "default", 1: "measured", 2: "process", 3: "file"}
+
+
+def clear_tuning_cache():
+    """Forgets the batch tunings measured or read in this process (the cache
+    file, PFH_TUNE_CACHE or ~/.cache/pencil-fft-hip/batch_tuning.txt, is
+    kept): the next autotuned BatchC2cPlan reads the file or measures."""
+    load_library().pfh_batch_tuning_cache_clear()
+
+
 class BatchC2cPlan:
     """A batch of independent C2C problems of uneven shapes (one C2cPlan
     each, every axis selected, one precision), executed as one persistent
@@ -797,6 +811,15 @@ class BatchC2cPlan:
         _check(_lib.pfh_batch_tuning(self._h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(c), ctypes.byref(m),
                                      ctypes.byref(p), ctypes.byref(q)))
         return b.value, r.value, c.value, bool(m.value), bool(p.value), bool(q.value)
+
+    @property
+    def tuning_source(self) -> str:
+        """Where the tuning came from: "default" (autotune off), "measured"
+        (timed by this creation), "process" (measured earlier in this
+        process) or "file" (read from the tuning cache file)."""
+        v = ctypes.c_int()
+        _check(_lib.pfh_batch_tuning_source(self._h, ctypes.byref(v)))
+        return TUNING_SOURCES[v.value]
 
     def allocate_workspace(self) -> torch.Tensor:
         return torch.zeros(max(self.workspace_bytes, 256), dtype=torch.uint8, device="cuda")

@@ -25,3 +25,12 @@ def gpu():
 
     pencil_fft_hip.load_library()
     return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _batch_tuning_cache(tmp_path_factory):
+    """Autotuned batches keep their tunings in a file of this session, not in
+    the user's cache directory, unless PFH_TUNE_CACHE is set already."""
+    if "PFH_TUNE_CACHE" not in os.environ:
+        os.environ["PFH_TUNE_CACHE"] = str(tmp_path_factory.mktemp("tune") / "batch_tuning.txt")
+    yield

@@ -149,6 +149,7 @@ hipError_t hipGetDevicePropertiesR0600(hipDeviceProp_tR0600 *p, int)
     std::memset(p, 0, sizeof(*p));
     p->sharedMemPerBlock = kLdsLimit;
     p->multiProcessorCount = 256;
+    std::snprintf(p->gcnArchName, sizeof p->gcnArchName, "gfx950:sramecc+:xnack-");
     return hipSuccess;
 }
 const char *hipGetErrorString(hipError_t) { return "emulated error"; }
@@ -161,9 +162,13 @@ hipError_t hipEventCreate(hipEvent_t *e)
 }
 hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
 hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+// Elapsed times follow a fixed pseudo-random sequence, so that a measured
+// choice (the batch autotuner's) is not simply the first candidate.
+unsigned g_elapsed_calls = 0;
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t)
 {
-    *ms = 1.0f;
+    const unsigned c = g_elapsed_calls++;
+    *ms = 1.0f + (float)((c * 2654435761u) >> 24) / 256.0f;
     return hipSuccess;
 }
 hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
@@ -1461,8 +1466,109 @@ static void contract_cases()
     std::printf("contract: %s\n", g_fail == before_fail ? "ok" : "FAILED");
 }
 
+// The batch tuning cache: a measured tuning is reused within the process and,
+// through the cache file, by a later creation after the process map is
+// cleared; a record that does not parse is skipped; a cache switched off
+// measures every time.
+void tune_cache_cases(const char *path)
+{
+    const int before_fail = g_fail;
+    std::vector<std::vector<size_t>> shapes = {{48, 60, 64}, {12, 7, 9}, {84, 16, 20}, {160, 128, 48}};
+    std::vector<pfh_desc> ds;
+    for (auto &s : shapes) ds.push_back(route_desc(PFH_C2C, PFH_F32, s, {3}));
+    setenv("PFH_TUNE_CACHE", path, 1);
+    std::remove(path);
+    struct Got {
+        int src = -1, t[6] = {};
+    };
+    auto create = [&](const std::vector<pfh_desc> &d, int tune) {
+        Got g;
+        pfh_batch b = nullptr;
+        if (pfh_batch_create(d.data(), (int)d.size(), tune, &b) != PFH_OK) {
+            fail("tune cache: batch create");
+            return g;
+        }
+        pfh_batch_tuning_source(b, &g.src);
+        pfh_batch_tuning(b, &g.t[0], &g.t[1], &g.t[2], &g.t[3], &g.t[4], &g.t[5]);
+        pfh_batch_destroy(b);
+        return g;
+    };
+    auto same = [](const Got &a, const Got &b) {
+        for (int i = 0; i < 6; ++i)
+            if (a.t[i] != b.t[i]) return false;
+        return true;
+    };
+    const int calls0 = (int)g_elapsed_calls;
+    Got m = create(ds, 1);
+    if (m.src != PFH_TUNED_MEASURED) fail("tune cache: first creation source %d", m.src);
+    const int timed = (int)g_elapsed_calls - calls0;
+    if (timed < 24 * 5) fail("tune cache: %d timings for 24 candidates", timed);
+    Got d0 = create(ds, 0);
+    if (d0.src != PFH_TUNED_DEFAULT) fail("tune cache: default source %d", d0.src);
+    const int calls1 = (int)g_elapsed_calls;
+    Got p = create(ds, 1);
+    if (p.src != PFH_TUNED_PROCESS_CACHE || !same(p, m)) fail("tune cache: process cache (source %d)", p.src);
+    if ((int)g_elapsed_calls != calls1) fail("tune cache: a cached creation measured");
+    // another process: the map is empty, the file holds the record
+    pfh_batch_tuning_cache_clear();
+    Got f = create(ds, 1);
+    if (f.src != PFH_TUNED_FILE_CACHE || !same(f, m)) fail("tune cache: file cache (source %d)", f.src);
+    // a different batch (one shape changed) is measured
+    std::vector<pfh_desc> ds2 = ds;
+    ds2[1].shape[2] = 10;
+    Got o = create(ds2, 1);
+    if (o.src != PFH_TUNED_MEASURED) fail("tune cache: another batch's source %d", o.src);
+    // records that do not parse are skipped; the last good record of a key wins
+    {
+        FILE *fp = std::fopen(path, "r");
+        std::string first;
+        char buf[1 << 14];
+        if (fp && std::fgets(buf, sizeof buf, fp)) first = buf;
+        if (fp) std::fclose(fp);
+        const size_t sp = first.find(' ');
+        if (sp == std::string::npos) {
+            fail("tune cache: no record in the file");
+        } else {
+            const std::string key = first.substr(0, sp);
+            const int alt_block = m.t[0] == 128 ? 256 : 128;
+            fp = std::fopen(path, "a");
+            std::fprintf(fp, "%s 256 8 0 1 1\n", key.c_str());            // a field missing
+            std::fprintf(fp, "%s 300 8 0 1 1 1 2.0\n", key.c_str());      // not a power of two
+            std::fprintf(fp, "%s 256 8 0 1 1 1 2.0 junk\n", key.c_str()); // trailing text
+            std::fprintf(fp, "garbage line\n");
+            std::fprintf(fp, "%s %d 8 32 1 0 0 2.5\n", key.c_str(), alt_block);
+            std::fclose(fp);
+            pfh_batch_tuning_cache_clear();
+            Got g = create(ds, 1);
+            if (g.src != PFH_TUNED_FILE_CACHE || g.t[0] != alt_block || g.t[2] != 32 || g.t[4] != 0)
+                fail("tune cache: last record (source %d, block %d, col %d, persistent %d)", g.src, g.t[0], g.t[2],
+                     g.t[4]);
+        }
+    }
+    // cache off: every autotuned creation measures, nothing is written
+    setenv("PFH_TUNE_CACHE", "off", 1);
+    pfh_batch_tuning_cache_clear();
+    std::remove(path);
+    Got x = create(ds, 1);
+    Got y = create(ds, 1);
+    if (x.src != PFH_TUNED_MEASURED || y.src != PFH_TUNED_PROCESS_CACHE)
+        fail("tune cache off: sources %d %d", x.src, y.src);
+    if (FILE *fp = std::fopen(path, "r")) {
+        std::fclose(fp);
+        fail("tune cache off: a file was written");
+    }
+    std::printf("tune cache: %s\n", g_fail == before_fail ? "ok" : "FAILED");
+}
+
 int main(int argc, char **argv)
 {
+    if (argc == 3 && std::strcmp(argv[1], "tunecache") == 0) {
+        tune_cache_cases(argv[2]);
+        std::printf("launches %d, failures %d\n", pfh::g_launches, g_fail);
+        return g_fail ? 1 : 0;
+    }
+    // the scenarios below keep their tunings in the process
+    setenv("PFH_TUNE_CACHE", "off", 1);
     if (argc == 7 && std::strcmp(argv[1], "plan") == 0) {
         // one R2R (DCT-II, DST-I, DCT-IV) or DHT plan: emu plan r2r|dht n0 n1 n2 f32|f64
         // (launch shapes printed with PFH_EMU_TRACE set)

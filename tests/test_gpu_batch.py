@@ -63,6 +63,37 @@ def test_small_batch_matches_oracle(dt, autotune):
                 assert err <= tol(dt), (op, use_graph, k, s, err)
 
 
+def test_tuning_cache_reuses_the_measured_tuning(tmp_path, monkeypatch):
+    """An autotuned batch is measured once: the same problems on the same
+    device reuse the tuning in the process and, after the process map is
+    cleared (as in a new process), from the cache file; the file-tuned plan
+    gives the oracle's results."""
+    path = tmp_path / "batch_tuning.txt"
+    monkeypatch.setenv("PFH_TUNE_CACHE", str(path))
+    pf.clear_tuning_cache()
+    shapes = [(48, 60, 64), (12, 7, 9), (84, 16, 20)]
+    ex = [(3,)] * len(shapes)
+    try:
+        a = pf.BatchC2cPlan(shapes, autotune=True, extra_shapes=ex)
+        assert a.tuning_source == "measured"
+        b = pf.BatchC2cPlan(shapes, autotune=True, extra_shapes=ex)
+        assert b.tuning_source == "process" and b.tuning() == a.tuning()
+        pf.clear_tuning_cache()
+        c = pf.BatchC2cPlan(shapes, autotune=True, extra_shapes=ex)
+        assert c.tuning_source == "file" and c.tuning() == a.tuning()
+        lines = path.read_text().splitlines()
+        assert len(lines) == 1 and lines[0].startswith("pfhbt2;arch=gfx950"), lines
+        assert pf.BatchC2cPlan(shapes, autotune=False, extra_shapes=ex).tuning_source == "default"
+        rng = np.random.default_rng(5)
+        xs = _inputs(rng, shapes, 3, np.complex64)
+        outs = c.forward([_tdev(x) for x in xs], use_graph=True)
+        for k, (s, x) in enumerate(zip(shapes, xs)):
+            ref = O.transform(O.KIND_C2C, x.astype(np.complex128), O.FORWARD, shape=s, extra=3)
+            assert rel_l2(_host(outs[k]), ref) <= tol(np.complex64), (k, s)
+    finally:
+        pf.clear_tuning_cache()
+
+
 def test_graph_replays_match_eager():
     """Graph replays, with eager work and synchronizations between them,
     give the same bits as the direct launches."""

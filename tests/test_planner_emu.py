@@ -23,6 +23,7 @@ def emu(tmp_path_factory):
     out = tmp_path_factory.mktemp("emu") / "emu"
     cmd = ["g++", "-std=c++17", "-O2", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
            os.path.join(ROOT, "pencil-rs_amd", "csrc", "pfh_api.cpp"),
+           os.path.join(ROOT, "pencil-rs_amd", "csrc", "pfh_tune_cache.cpp"),
            os.path.join(ROOT, "tests", "emu", "hip_emu.cpp"), "-o", str(out)]
     subprocess.run(cmd, check=True, capture_output=True)
     return str(out)
@@ -47,3 +48,15 @@ def test_boundary_contract_refuses_before_device_work(emu):
     r = subprocess.run([emu, "contract"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
     assert "contract: ok" in r.stdout
+
+
+def test_batch_tuning_cache(emu, tmp_path):
+    """The autotuner's choice is kept per (descriptors, precision, device):
+    reused in the process, read back from the cache file by a later
+    creation, malformed records skipped, the last good record of a key
+    winning, another batch measured, and no file written when the cache is
+    switched off."""
+    path = tmp_path / "tune" / "batch_tuning.txt"
+    r = subprocess.run([emu, "tunecache", str(path)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
+    assert "tune cache: ok" in r.stdout
