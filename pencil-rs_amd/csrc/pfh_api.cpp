@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <complex>
 #include <atomic>
 #include <cfloat>
 #include <cmath>
@@ -503,8 +504,9 @@ static pfh_status fill_transform(Step &st, const StageSpec &sp, std::vector<doub
     if (sp.kind == SK_R2C && (sp.flags & F_HERM_DOUBLE)) st.flags |= F_HERM_DOUBLE;
     if (sp.kind == SK_C2R && (sp.flags & F_HERM_HALF)) st.flags |= F_HERM_HALF;
     const int max_r = tu.max_radix ? tu.max_radix : (tbytes == 4 ? 16 : 8);
-    if (!radix_plan(st.m, st.radices, max_r)) return set_err(PFH_ERR_UNSUPPORTED, "FFT length needs too many passes");
-    if (tu.allow_fast) choose_fast(st, tbytes);
+    st.long_only = !radix_plan(st.m, st.radices, max_r); // more passes than a stage holds: pieces only
+    if (st.long_only) st.radices.clear();
+    if (tu.allow_fast && !st.long_only) choose_fast(st, tbytes);
 
     // tables
     st.tw_base = (int64_t)(tab.size() / 2);
@@ -607,9 +609,8 @@ static void set_tiles(Step &st, const Tuning &tu)
     }
 }
 
-// Line grid, access patterns and launch shape of a step whose in/out
-// layouts are set.
-static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const Tuning &tu)
+// Line grid and access patterns of a step whose in/out layouts are set.
+static pfh_status step_geometry(Step &st)
 {
     const int a = st.axis;
     const auto sti = st.in.strides(), sto = st.out.strides();
@@ -650,7 +651,12 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
     if (nlines > INT32_MAX) return set_err(PFH_ERR_UNSUPPORTED, "more than 2^31 - 1 lines in one stage");
     st.g = g;
     st.nlines = nlines;
+    return PFH_OK;
+}
 
+// Launch shape (tile, threads per line, LDS) of a step whose line grid is set.
+static pfh_status size_step(Step &st, const DeviceInfo &dev, int tbytes, const Tuning &tu)
+{
     const int cb = 2 * tbytes;
     const int in_eb = st.in_cplx ? cb : tbytes, out_eb = st.out_cplx ? cb : tbytes;
     // column access: enough lines per tile that each point of the tile is a
@@ -683,7 +689,7 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
     const int target = tu.block > 0 ? tu.block : block_env;
     int lt = (st.in_col || st.out_col) ? lt_col : std::max(1, target / tpl);
     if (tu.fill_block && (st.in_col || st.out_col)) lt = std::max(lt, target / tpl);
-    while (lt > 1 && (int64_t)(lt / 2) >= nlines) lt /= 2;
+    while (lt > 1 && (int64_t)(lt / 2) >= st.nlines) lt /= 2;
     while (lt * tpl < 64) tpl *= 2;
     const int cap = tu.block_cap > 0 ? std::min(tu.block_cap, max_threads) : max_threads;
     // A generic stage with a column side whose tile of lt lines x tpl
@@ -740,6 +746,571 @@ static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const
     else
         st.flags &= ~F_NT;
     return PFH_OK;
+}
+
+// Line grid, access patterns and launch shape of a step whose in/out
+// layouts are set.
+static pfh_status finish_step(Step &st, const DeviceInfo &dev, int tbytes, const Tuning &tu)
+{
+    pfh_status r = step_geometry(st);
+    if (r) return r;
+    return size_step(st, dev, tbytes, tu);
+}
+
+// ---------------------------------------------------------------------------
+// long lines: a stage whose line does not fit one workgroup, or whose FFT
+// length has a large prime factor, runs as pieces (generic line stages on
+// workspace regions, plus the prep / post kernels of real lines)
+// ---------------------------------------------------------------------------
+
+static int64_t largest_prime_factor(int64_t m)
+{
+    int64_t best = 1;
+    for (int64_t p = 2; p * p <= m; ++p)
+        while (m % p == 0) {
+            best = p;
+            m /= p;
+        }
+    return m > 1 ? std::max(best, m) : best;
+}
+
+// Prime factors above this run through Bluestein's chirp convolution. The
+// generic pass costs about 4p flops per point for a factor p, against 16
+// bytes of HBM traffic per point and pass: at p near 100 a stage is as
+// compute bound as it is HBM bound on MI355X (157 TFLOP/s f32 against
+// 8 TB/s), while Bluestein's pieces move about five times a stage's bytes.
+// PFH_BLUESTEIN_PRIME overrides it; read at every plan creation.
+static int64_t bluestein_prime()
+{
+    const char *e = std::getenv("PFH_BLUESTEIN_PRIME");
+    const int64_t v = e ? std::atoll(e) : 0;
+    return v > 0 ? v : 128;
+}
+
+// PFH_LONG_LINE_MAX: complex FFT lengths above it run as pieces even where
+// one workgroup would hold the line (0 or unset: only where it does not);
+// read at every plan creation.
+static int64_t long_line_max()
+{
+    const char *e = std::getenv("PFH_LONG_LINE_MAX");
+    return e ? std::atoll(e) : 0;
+}
+
+static bool wants_pieces(const Step &s)
+{
+    if (s.long_only) return true;
+    if (s.m <= 1) return false;
+    const int64_t lim = long_line_max();
+    if (lim > 0 && s.m > lim) return true;
+    return largest_prime_factor(s.m) > bluestein_prime();
+}
+
+// Pass tables of a radix split (roots of generic radices, then the pass
+// twiddles), offsets relative to the block; appended to tab when non-null.
+static int pass_tables(const std::vector<int> &radices, std::vector<double> *tab, std::array<int, kMaxPasses> &twoff)
+{
+    int off = 0, Ns = 1;
+    for (size_t p = 0; p < radices.size(); ++p) {
+        const int R = radices[p];
+        twoff[p] = off;
+        if (is_generic_radix(R)) {
+            if (tab)
+                for (int e = 0; e < R; ++e) push_cexp(*tab, e, R);
+            off += R;
+        }
+        if (Ns > 1) {
+            if (tab)
+                for (int k = 0; k < Ns; ++k)
+                    for (int r = 1; r < R; ++r) push_cexp(*tab, (long long)r * k, (long long)Ns * R);
+            off += Ns * (R - 1);
+        }
+        Ns *= R;
+    }
+    return off;
+}
+
+// A named table of the plan (built once; long-line pieces of several
+// operations share it). fill appends the entries to host_tables.
+template <typename F>
+static pfh_status plan_table(PlanImpl &P, const std::string &name, int64_t &offset, F fill)
+{
+    auto it = P.table_index.find(name);
+    if (it != P.table_index.end()) {
+        offset = it->second;
+        return PFH_OK;
+    }
+    if (P.tables_frozen) return set_err(PFH_ERR_PREPARATION_FAILED, "long-line table missing after upload");
+    offset = (int64_t)(P.host_tables.size() / 2);
+    fill(P.host_tables);
+    P.table_index[name] = offset;
+    return PFH_OK;
+}
+
+// Forward DFT of a 5-smooth length in long double (decimation in time over
+// the factors 2, 3, 5; w = exp(-2 pi i e / n) for the full length n).
+using cxl = std::complex<long double>;
+static void host_dft_rec(const cxl *in, size_t stride, cxl *out, size_t len, const std::vector<cxl> &w, size_t wstep)
+{
+    if (len == 1) {
+        out[0] = in[0];
+        return;
+    }
+    const size_t p = (len % 2 == 0) ? 2 : (len % 3 == 0) ? 3 : (len % 5 == 0) ? 5 : len;
+    const size_t q = len / p;
+    if (p == len) {
+        for (size_t k = 0; k < len; ++k) {
+            cxl acc = 0;
+            for (size_t j = 0; j < len; ++j) acc += in[j * stride] * w[((j * k) % len) * wstep];
+            out[k] = acc;
+        }
+        return;
+    }
+    for (size_t r = 0; r < p; ++r) host_dft_rec(in + r * stride, stride * p, out + r * q, q, w, wstep * p);
+    cxl t[5];
+    for (size_t k = 0; k < q; ++k) {
+        for (size_t r = 0; r < p; ++r) t[r] = out[r * q + k];
+        for (size_t s = 0; s < p; ++s) {
+            cxl acc = 0;
+            for (size_t r = 0; r < p; ++r) acc += t[r] * w[((r * (k + q * s)) % len) * wstep];
+            out[k + q * s] = acc;
+        }
+    }
+}
+
+static std::vector<cxl> host_dft(const std::vector<cxl> &x)
+{
+    const size_t n = x.size();
+    std::vector<cxl> w(n), out(n);
+    for (size_t e = 0; e < n; ++e) {
+        std::vector<double> t;
+        push_cexp(t, (long long)e, (long long)n);
+        w[e] = cxl(t[0], t[1]);
+    }
+    host_dft_rec(x.data(), 1, out.data(), n, w, 1);
+    return out;
+}
+
+// exp(s i pi j^2 / m), the phase reduced exactly (j^2 mod 2m) before the
+// trigonometry.
+static cxl chirp(int64_t j, int64_t m, int s)
+{
+    const unsigned __int128 jj = (unsigned __int128)(uint64_t)j * (uint64_t)j;
+    const int64_t r = (int64_t)(jj % (unsigned __int128)(2 * m));
+    const long double a = (long double)s * kPiL * (long double)r / (long double)m;
+    return cxl(cosl(a), sinl(a));
+}
+
+static bool five_smooth(int64_t v)
+{
+    for (int p : {2, 3, 5})
+        while (v % p == 0) v /= p;
+    return v == 1;
+}
+
+// Does a complex FFT of length m run as one generic line stage?
+static bool piece_fits(int64_t m, const DeviceInfo &dev, int tb, const Tuning &tu)
+{
+    if (m < 1 || m > INT32_MAX) return false;
+    const int64_t lim = long_line_max();
+    if (lim > 0 && m > lim) return false;
+    if (m > 1 && largest_prime_factor(m) > bluestein_prime()) return false;
+    Step p;
+    p.mode = M_C2C;
+    p.n = p.m = p.nin = p.nout = (int)m;
+    const int max_r = tu.max_radix ? tu.max_radix : (tb == 4 ? 16 : 8);
+    if (!radix_plan((int)m, p.radices, max_r)) return false;
+    p.twn = pass_tables(p.radices, nullptr, p.twoff);
+    p.g.nd = 1;
+    p.g.ext[0] = 64;
+    p.g.is[0] = p.g.os[0] = m;
+    p.nlines = 64;
+    p.ips = p.ops = 1;
+    const pfh_error_info saved = g_err;
+    const bool fits = size_step(p, dev, tb, tu) == PFH_OK;
+    g_err = saved;
+    return fits;
+}
+
+// N1 (the first piece's length) of a split N1 x N2 = m whose pieces both
+// fit one workgroup, N1 as close to sqrt(m) as the divisors allow; 0: none.
+static int64_t split_factor(int64_t m, const DeviceInfo &dev, int tb, const Tuning &tu)
+{
+    int64_t best = 0;
+    double best_d = 1e300;
+    for (int64_t a = 2; a * a <= m; ++a) {
+        if (m % a) continue;
+        for (int64_t n1 : {a, m / a}) {
+            const int64_t n2 = m / n1;
+            if (!piece_fits(n1, dev, tb, tu) || !piece_fits(n2, dev, tb, tu)) continue;
+            const double d = std::fabs(std::log((double)n1) - 0.5 * std::log((double)m));
+            if (d < best_d) {
+                best_d = d;
+                best = n1;
+            }
+        }
+    }
+    return best;
+}
+
+// Length of Bluestein's cyclic convolution for an FFT of length m: the
+// smallest 5-smooth M >= 2m - 1 that runs as one piece or a split.
+static int64_t bluestein_length(int64_t m, const DeviceInfo &dev, int tb, const Tuning &tu)
+{
+    for (int64_t M = 2 * m - 1; M <= 4 * m && M <= ((int64_t)1 << 25); ++M) {
+        if (!five_smooth(M)) continue;
+        if (piece_fits(M, dev, tb, tu) || split_factor(M, dev, tb, tu) > 0) return M;
+    }
+    return 0;
+}
+
+// Strides of one side of the long lines: the stage's line digits (is or os
+// of its geometry) and the point stride; line-major regions have point
+// stride 1 and line l at l * len.
+struct LineSide {
+    int buf = BUF_SRC;
+    int64_t point = 1;
+    int64_t dig[kMaxDigits] = {0, 0, 0, 0, 0};
+};
+
+static LineSide region_side(const Step &outer, int buf, int64_t len)
+{
+    LineSide v;
+    v.buf = buf;
+    v.point = 1;
+    int64_t run = len;
+    for (int d = 0; d < outer.g.nd; ++d) {
+        v.dig[d] = run;
+        run *= outer.g.ext[d];
+    }
+    return v;
+}
+
+struct PieceDigit {
+    int64_t ext, is, os;
+    bool special;
+};
+
+struct PieceX {
+    int flags = 0; // F_CONJ_IN / F_CONJ_OUT / F_XIN / F_XOUT / F_XTW
+    double scale = 1.0;
+    int64_t tab_in = -1, tab_out = -1, tw = -1;
+    int mul_in = 1, mul_out = 1, valid_in = 0, valid_out = 0, tw_m = 0, tw_bsh = 0;
+};
+
+// Two-level table of exp(-2 pi i q / M): 2^bsh low entries, then the
+// powers of exp(-2 pi i 2^bsh / M).
+static pfh_status twiddle_table(PlanImpl &P, int64_t M, int64_t &off, int &bsh)
+{
+    bsh = 0;
+    while (((int64_t)1 << (2 * bsh)) < M) ++bsh;
+    const int b = bsh;
+    return plan_table(P, "tw2:" + std::to_string(M), off, [&](std::vector<double> &t) {
+        const int64_t B = (int64_t)1 << b;
+        for (int64_t q = 0; q < B; ++q) push_cexp(t, q, M);
+        for (int64_t h = 0; h * B < M; ++h) push_cexp(t, h * B, M);
+    });
+}
+
+// One generic C2C line stage of length N over the given digits.
+static pfh_status make_piece(PlanImpl &P, Step &p, int64_t N, std::vector<PieceDigit> digs, int64_t ips,
+                             int64_t ops, int in_buf, int out_buf, const PieceX &x, int tb)
+{
+    p = Step();
+    p.mode = M_C2C;
+    p.n = p.m = p.nin = p.nout = (int)N;
+    p.in_cplx = p.out_cplx = true;
+    p.flags = x.flags;
+    p.scale = x.scale;
+    const int max_r = P.tuning.max_radix ? P.tuning.max_radix : (tb == 4 ? 16 : 8);
+    if (!radix_plan((int)N, p.radices, max_r)) return set_err(PFH_ERR_UNSUPPORTED, "long-line piece radix plan");
+    int64_t base = 0;
+    std::array<int, kMaxPasses> twoff{};
+    const int twn = pass_tables(p.radices, nullptr, twoff);
+    pfh_status st = plan_table(P, "pass:" + std::to_string(N) + ":" + std::to_string(max_r), base,
+                               [&](std::vector<double> &t) { pass_tables(p.radices, &t, twoff); });
+    if (st) return st;
+    p.tw_base = base;
+    p.twn = twn;
+    p.twoff = twoff;
+    std::vector<PieceDigit> keep;
+    for (const PieceDigit &d : digs)
+        if (d.ext > 1) keep.push_back(d);
+    if ((int)keep.size() > kMaxDigits) return set_err(PFH_ERR_UNSUPPORTED, "too many line digits");
+    const bool in_col = ips != 1, out_col = ops != 1;
+    std::stable_sort(keep.begin(), keep.end(), [&](const PieceDigit &a, const PieceDigit &b) {
+        return (!in_col && out_col) ? a.os < b.os : a.is < b.is;
+    });
+    int64_t nlines = 1;
+    p.g.nd = (int)keep.size();
+    for (int d = 0; d < p.g.nd; ++d) {
+        p.g.ext[d] = keep[d].ext;
+        p.g.is[d] = keep[d].is;
+        p.g.os[d] = keep[d].os;
+        if (keep[d].special) p.xdig = d;
+        nlines *= keep[d].ext;
+    }
+    if (nlines > INT32_MAX) return set_err(PFH_ERR_UNSUPPORTED, "more than 2^31 - 1 lines in one stage");
+    p.nlines = nlines;
+    p.ips = ips;
+    p.ops = ops;
+    p.in_col = in_col ? 1 : 0;
+    p.out_col = out_col ? 1 : 0;
+    p.in_buf = in_buf;
+    p.out_buf = out_buf;
+    p.xtab_in = x.tab_in;
+    p.xtab_out = x.tab_out;
+    p.xtw = x.tw;
+    p.xmul_in = x.mul_in;
+    p.xmul_out = x.mul_out;
+    p.xvalid_in = x.valid_in;
+    p.xvalid_out = x.valid_out;
+    p.xtw_m = x.tw_m;
+    p.xtw_bsh = x.tw_bsh;
+    Tuning tu = P.tuning;
+    tu.allow_fast = false;
+    return size_step(p, P.dev, tb, tu);
+}
+
+// A complex FFT of length N from side `in` to side `out` over the outer
+// stage's lines: one piece, or the four-step split N = N1 x N2 through
+// region rs (line-major [line][k1][n2]): the first piece transforms n1 (index
+// N2 n1 + n2, lines n2) and multiplies by exp(-2 pi i n2 k1 / N) at its
+// store, the second transforms n2 (lines k1) and writes index k1 + N1 k2.
+// xin / xout (load / store factors, padding and crop) go to the first /
+// last piece, with full indexes N2 n1 + n2 and k1 + N1 k2.
+static pfh_status fft_pieces(PlanImpl &P, const Step &outer, int64_t N, const LineSide &in, const LineSide &out,
+                             const PieceX &xin, const PieceX &xout, int rs_buf, int64_t xb[3], int tb,
+                             std::vector<Step> &pieces)
+{
+    const int nd = outer.g.nd;
+    if (piece_fits(N, P.dev, tb, P.tuning)) {
+        std::vector<PieceDigit> digs;
+        for (int d = 0; d < nd; ++d) digs.push_back({outer.g.ext[d], in.dig[d], out.dig[d], false});
+        PieceX x = xout;
+        x.flags = (xin.flags & (F_CONJ_IN | F_XIN)) | (xout.flags & (F_CONJ_OUT | F_XOUT));
+        x.tab_in = xin.tab_in;
+        x.mul_in = 1;
+        x.valid_in = xin.valid_in;
+        x.mul_out = 1;
+        Step p;
+        pfh_status st = make_piece(P, p, N, digs, in.point, out.point, in.buf, out.buf, x, tb);
+        if (st) return st;
+        pieces.push_back(p);
+        return PFH_OK;
+    }
+    const int64_t N1 = split_factor(N, P.dev, tb, P.tuning);
+    if (N1 <= 0) return set_err(PFH_ERR_UNSUPPORTED, "no split of the long line into workgroup pieces");
+    const int64_t N2 = N / N1;
+    const LineSide rs = region_side(outer, rs_buf, N);
+    xb[rs_buf - BUF_X0] = std::max(xb[rs_buf - BUF_X0], outer.nlines * N * 2 * tb);
+    // first piece: lines (n2, outer lines), points n1
+    {
+        std::vector<PieceDigit> digs;
+        digs.push_back({N2, in.point, 1, true});
+        for (int d = 0; d < nd; ++d) digs.push_back({outer.g.ext[d], in.dig[d], rs.dig[d], false});
+        PieceX x;
+        x.flags = (xin.flags & (F_CONJ_IN | F_XIN)) | F_XTW;
+        x.tab_in = xin.tab_in;
+        x.mul_in = (int)N2;
+        x.valid_in = xin.valid_in;
+        pfh_status st = twiddle_table(P, N, x.tw, x.tw_bsh);
+        if (st) return st;
+        x.tw_m = (int)N;
+        Step p;
+        st = make_piece(P, p, N1, digs, N2 * in.point, N2, in.buf, rs_buf, x, tb);
+        if (st) return st;
+        pieces.push_back(p);
+    }
+    // second piece: lines (k1, outer lines), points n2 -> index k1 + N1 k2
+    {
+        std::vector<PieceDigit> digs;
+        digs.push_back({N1, N2, out.point, true});
+        for (int d = 0; d < nd; ++d) digs.push_back({outer.g.ext[d], rs.dig[d], out.dig[d], false});
+        PieceX x = xout;
+        x.flags = xout.flags & (F_CONJ_OUT | F_XOUT);
+        x.mul_out = (int)N1;
+        Step p;
+        pfh_status st = make_piece(P, p, N2, digs, 1, N1 * out.point, rs_buf, out.buf, x, tb);
+        if (st) return st;
+        pieces.push_back(p);
+    }
+    return PFH_OK;
+}
+
+// The complex FFT of length m (sign from conj: F_CONJ_IN | F_CONJ_OUT =
+// positive exponent), scaled, from `in` to `out`: directly in pieces, or
+// through Bluestein's convolution when m has a prime factor above the
+// threshold. Bluestein (sign s = -1 forward, +1 positive):
+//   Z_k = c_k sum_j (x_j c_j) b_(k-j),  c_j = exp(s i pi j^2 / m),
+//   b_t = exp(-s i pi t^2 / m),
+// a cyclic convolution of length M >= 2m - 1: piece B1 loads x_j c_j (zero
+// from m on), runs FFT_M and stores A_k * FFT_M(b)_k / M into region rb;
+// piece B2 loads the conjugate, runs FFT_M, stores the conjugate (= the
+// convolution) times c_k * scale for k < m.
+static pfh_status core_pieces(PlanImpl &P, const Step &outer, int64_t m, const LineSide &in, const LineSide &out,
+                              int conj, double scale, int rs_buf, int rb_buf, int64_t xb[3], int tb,
+                              std::vector<Step> &pieces)
+{
+    if (m <= 1 || largest_prime_factor(m) <= bluestein_prime()) {
+        PieceX xi, xo;
+        xi.flags = conj & F_CONJ_IN;
+        xi.valid_in = (int)m;
+        xo.flags = conj & F_CONJ_OUT;
+        xo.scale = scale;
+        xo.valid_out = (int)m;
+        return fft_pieces(P, outer, m, in, out, xi, xo, rs_buf, xb, tb, pieces);
+    }
+    const int64_t M = bluestein_length(m, P.dev, tb, P.tuning);
+    if (M <= 0) return set_err(PFH_ERR_UNSUPPORTED, "no Bluestein length for this line");
+    const int s = (conj & F_CONJ_IN) ? 1 : -1;
+    int64_t c_off = 0, bh_off = 0;
+    pfh_status st = plan_table(P, "chirp:" + std::to_string(m) + ":" + std::to_string(s), c_off,
+                               [&](std::vector<double> &t) {
+                                   for (int64_t j = 0; j < m; ++j) {
+                                       const cxl c = chirp(j, m, s);
+                                       t.push_back((double)c.real());
+                                       t.push_back((double)c.imag());
+                                   }
+                               });
+    if (st) return st;
+    st = plan_table(P, "blue:" + std::to_string(m) + ":" + std::to_string(M) + ":" + std::to_string(s), bh_off,
+                    [&](std::vector<double> &t) {
+                        std::vector<cxl> b((size_t)M, cxl(0, 0));
+                        for (int64_t j = 0; j < m; ++j) {
+                            const cxl v = chirp(j, m, -s);
+                            b[(size_t)j] = v;
+                            if (j > 0) b[(size_t)(M - j)] = v;
+                        }
+                        const std::vector<cxl> B = host_dft(b);
+                        for (int64_t k = 0; k < M; ++k) {
+                            t.push_back((double)(B[(size_t)k].real() / (long double)M));
+                            t.push_back((double)(B[(size_t)k].imag() / (long double)M));
+                        }
+                    });
+    if (st) return st;
+    const LineSide rb = region_side(outer, rb_buf, M);
+    xb[rb_buf - BUF_X0] = std::max(xb[rb_buf - BUF_X0], outer.nlines * M * 2 * tb);
+    {
+        PieceX xi, xo;
+        xi.flags = F_XIN;
+        xi.tab_in = c_off;
+        xi.valid_in = (int)m;
+        xo.flags = F_XOUT;
+        xo.tab_out = bh_off;
+        xo.valid_out = (int)M;
+        st = fft_pieces(P, outer, M, in, rb, xi, xo, rs_buf, xb, tb, pieces);
+        if (st) return st;
+    }
+    PieceX xi, xo;
+    xi.flags = F_CONJ_IN;
+    xi.valid_in = (int)M;
+    xo.flags = F_CONJ_OUT | F_XOUT;
+    xo.scale = scale;
+    xo.tab_out = c_off;
+    xo.valid_out = (int)m;
+    return fft_pieces(P, outer, M, rb, out, xi, xo, rs_buf, xb, tb, pieces);
+}
+
+// Expands a stage (line grid set, launch shape not) into its pieces:
+// complex lines directly; real lines through prep (the promoted FFT input,
+// line-major in region X2), the complex FFT (X2 -> X2) and post. Regions:
+// X0 the split intermediate, X1 Bluestein's, X2 the promoted lines.
+static pfh_status expand_long(PlanImpl &P, Step &s, int tb, int64_t xb[3])
+{
+    s.pieces.clear();
+    s.fast = false;
+    if (s.g.nd + 1 > kMaxDigits) return set_err(PFH_ERR_UNSUPPORTED, "too many line digits for a long line");
+    LineSide in, out;
+    in.buf = s.in_buf;
+    in.point = s.ips;
+    out.buf = s.out_buf;
+    out.point = s.ops;
+    for (int d = 0; d < s.g.nd; ++d) {
+        in.dig[d] = s.g.is[d];
+        out.dig[d] = s.g.os[d];
+    }
+    if (s.mode == M_C2C)
+        return core_pieces(P, s, s.n, in, out, s.flags & (F_CONJ_IN | F_CONJ_OUT), s.scale, BUF_X0, BUF_X1, xb, tb,
+                           s.pieces);
+    const int64_t n = s.n;
+    int kind;
+    int64_t m;
+    switch (s.mode) {
+    case M_R2C: case M_R2C_ODD: kind = LK_R2C; m = n; break;
+    case M_C2R: case M_C2R_ODD: kind = LK_C2R; m = n; break;
+    case M_DCT2: case M_DCT2_ODD: kind = LK_DCT2; m = n; break;
+    case M_DCT3: case M_DCT3_ODD: kind = LK_DCT3; m = n; break;
+    case M_DCT4: case M_DCT4_ODD: kind = LK_DCT4; m = 2 * n; break;
+    case M_DCT1: kind = LK_DCT1; m = 2 * (n - 1); break;
+    case M_DST1: kind = LK_DST1; m = 2 * (n + 1); break;
+    case M_DHT: case M_DHT_ODD: kind = LK_DHT; m = n; break;
+    default: return set_err(PFH_ERR_UNSUPPORTED, "long line of this stage kind");
+    }
+    if (m > INT32_MAX) return set_err(PFH_ERR_UNSUPPORTED, "line too long");
+    int64_t pre0 = 0, post0 = 0;
+    const std::string nk = std::to_string(n);
+    pfh_status st = PFH_OK;
+    if (kind == LK_DCT3 || kind == LK_DCT4)
+        st = plan_table(P, "lpre4n:" + nk, pre0, [&](std::vector<double> &t) {
+            for (int64_t j = 0; j < n; ++j) push_cexp(t, j, 4 * n);
+        });
+    if (st) return st;
+    if (kind == LK_DCT2)
+        st = plan_table(P, "lpre4n:" + nk, post0, [&](std::vector<double> &t) {
+            for (int64_t j = 0; j < n; ++j) push_cexp(t, j, 4 * n);
+        });
+    if (kind == LK_DCT4)
+        st = plan_table(P, "lpost8n:" + nk, post0, [&](std::vector<double> &t) {
+            for (int64_t k = 0; k < n; ++k) push_cexp(t, 2 * k + 1, 8 * n);
+        });
+    if (st) return st;
+    xb[2] = std::max(xb[2], s.nlines * m * 2 * tb);
+    const LineSide ra = region_side(s, BUF_X2, m);
+    Step pre;
+    pre.launch = 1;
+    pre.long_kind = kind;
+    pre.g = s.g;
+    pre.nlines = s.nlines;
+    pre.ips = s.ips;
+    pre.ops = 1;
+    pre.n = (int)n;
+    pre.m = (int)m;
+    pre.nout = s.nout;
+    pre.pre0 = (int)pre0;
+    pre.post0 = (int)post0;
+    pre.tw_base = 0;
+    pre.flags = s.flags;
+    pre.scale = s.scale;
+    pre.prew0 = s.prew0;
+    pre.prew1 = s.prew1;
+    pre.postw0 = s.postw0;
+    pre.postw1 = s.postw1;
+    pre.in_buf = s.in_buf;
+    pre.out_buf = BUF_X2;
+    s.pieces.push_back(pre);
+    st = core_pieces(P, s, m, ra, ra, 0, 1.0, BUF_X0, BUF_X1, xb, tb, s.pieces);
+    if (st) return st;
+    Step post = pre;
+    post.launch = 2;
+    post.in_buf = BUF_X2;
+    post.out_buf = s.out_buf;
+    post.ops = s.ops;
+    s.pieces.push_back(post);
+    return PFH_OK;
+}
+
+// Shape of a stage: its launch shape, or its pieces when its line needs them.
+static pfh_status finish_or_expand(PlanImpl &P, Step &s, int tb, int64_t xb[3])
+{
+    if (!wants_pieces(s)) {
+        const pfh_error_info saved = g_err;
+        const pfh_status st = finish_step(s, P.dev, tb, P.tuning);
+        if (st != PFH_ERR_UNSUPPORTED) return st;
+        g_err = saved;
+    }
+    pfh_status st = step_geometry(s);
+    if (st) return st;
+    return expand_long(P, s, tb, xb);
 }
 
 // ---------------------------------------------------------------------------
@@ -1038,6 +1609,7 @@ static pfh_status upload_tables(PlanImpl &P)
 {
     const size_t nent = P.host_tables.size() / 2;
     P.table_entries = (int64_t)nent;
+    P.tables_frozen = true;
     if (nent == 0) return PFH_OK;
     std::vector<T> tmp(P.host_tables.size());
     for (size_t i = 0; i < P.host_tables.size(); ++i) tmp[i] = (T)P.host_tables[i];
@@ -1132,6 +1704,7 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
     if (!assign_route(specs, in_place, dst_cap, pair_score, X, Y, ib, ob, wb))
         return set_err(PFH_ERR_PREPARATION_FAILED, "no buffer assignment");
     out.steps.clear();
+    for (int r = 0; r < 3; ++r) out.x_bytes[r] = 0;
     for (int e = 0; e < K; ++e) {
         Step s;
         st = fill_transform(s, specs[e], P.host_tables, tb, P.tuning);
@@ -1148,7 +1721,7 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
         }
         s.in_buf = ib[e];
         s.out_buf = ob[e];
-        st = finish_step(s, P.dev, tb, P.tuning);
+        st = finish_or_expand(P, s, tb, out.x_bytes);
         if (st) return st;
         if (checked[e]) {
             // acceptance check on the C2R stage's input (distributed/r2c.rs:4709-4807)
@@ -1264,7 +1837,7 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
     for (int e = 0; e + 1 < K;) {
         const int A = slab_axis_of(e, X[e], Y[e]);
         const Step &a = out.steps[e], &b = out.steps[e + 1];
-        if (A < 0 || a.in_buf == b.out_buf) {
+        if (A < 0 || a.in_buf == b.out_buf || !a.pieces.empty() || !b.pieces.empty()) {
             ++e;
             continue;
         }
@@ -1358,8 +1931,23 @@ static pfh_status build_local_op(PlanImpl &P, int op, OpPlan &out)
     s.out = make_layout(1, id, shout, 1, (sp.out_c ? 2 : 1) * tb);
     s.in_buf = BUF_SRC;
     s.out_buf = BUF_DST;
-    st = finish_step(s, P.dev, tb, P.tuning);
+    for (int r = 0; r < 3; ++r) out.x_bytes[r] = 0;
+    out.long_lines = 0;
+    st = finish_or_expand(P, s, tb, out.x_bytes);
     if (st) return st;
+    if (!s.pieces.empty()) {
+        // long lines run in chunks of lines whose piece regions take about
+        // 64 MiB of the workspace (at least one line)
+        const int64_t one = out.x_bytes[0] + out.x_bytes[1] + out.x_bytes[2];
+        const int64_t chunk = std::max<int64_t>(1, ((int64_t)64 << 20) / std::max<int64_t>(one, 1));
+        s.in.extra = s.out.extra = chunk;
+        for (int r = 0; r < 3; ++r) out.x_bytes[r] = 0;
+        st = step_geometry(s);
+        if (!st) st = expand_long(P, s, tb, out.x_bytes);
+        if (st) return st;
+        s.in.extra = s.out.extra = 1;
+        out.long_lines = chunk;
+    }
     if (P.kind == PFH_LOCAL_R2C && !fwd && op != PFH_ADJOINT_FORWARD) s.local_endpoint_check = true;
     out.steps.assign(1, s);
     out.in = s.in;
@@ -1553,6 +2141,7 @@ struct WsView {
     EndpointPartial *ep = nullptr;
     void *w[2] = {nullptr, nullptr};
     void *t = nullptr; // slab intermediate
+    void *x[3] = {nullptr, nullptr, nullptr}; // regions of the pieces of long lines
     // in-place calls: every stage reads the gate (set from the poison word by
     // the call's first kernel), and a rejected spectrum poisons the workspace
     int *poison = nullptr;
@@ -1653,7 +2242,8 @@ static bool overlaps(const void *a, size_t na, const void *b, size_t nb)
 // Workspace of one operation: status area, W0, W1, slab intermediate T.
 static int64_t op_ws_bytes(const PlanImpl &P, const OpPlan &o)
 {
-    return status_area_bytes(P) + align256(o.w_bytes[0]) + align256(o.w_bytes[1]) + align256(o.t_bytes);
+    return status_area_bytes(P) + align256(o.w_bytes[0]) + align256(o.w_bytes[1]) + align256(o.t_bytes) +
+           align256(o.x_bytes[0]) + align256(o.x_bytes[1]) + align256(o.x_bytes[2]);
 }
 
 static pfh_status view_workspace(const PlanImpl &P, const OpPlan &o, void *ws, size_t ws_bytes, WsView &v)
@@ -1668,11 +2258,16 @@ static pfh_status view_workspace(const PlanImpl &P, const OpPlan &o, void *ws, s
     v.w[0] = w;
     v.w[1] = w + align256(o.w_bytes[0]);
     v.t = w + align256(o.w_bytes[0]) + align256(o.w_bytes[1]);
+    char *x = static_cast<char *>(v.t) + align256(o.t_bytes);
+    for (int r = 0; r < 3; ++r) {
+        v.x[r] = o.x_bytes[r] > 0 ? x : nullptr;
+        x += align256(o.x_bytes[r]);
+    }
     return PFH_OK;
 }
 
 template <typename T>
-static StageArgs<T> fill_args(const PlanImpl &P, const Step &s, void *const bufs[4], const int *gate)
+static StageArgs<T> fill_args(const PlanImpl &P, const Step &s, void *const bufs[BUF_COUNT], const int *gate)
 {
     StageArgs<T> a{};
     a.src = bufs[s.in_buf];
@@ -1723,6 +2318,50 @@ static StageArgs<T> fill_args(const PlanImpl &P, const Step &s, void *const bufs
     a.prew1 = (T)s.prew1;
     a.postw0 = (T)s.postw0;
     a.postw1 = (T)s.postw1;
+    const cpx<T> *tabs = reinterpret_cast<const cpx<T> *>(P.dev_tables);
+    a.xtab_in = s.xtab_in >= 0 ? tabs + s.xtab_in : nullptr;
+    a.xtab_out = s.xtab_out >= 0 ? tabs + s.xtab_out : nullptr;
+    a.xtw = s.xtw >= 0 ? tabs + s.xtw : nullptr;
+    a.xdig = s.xdig;
+    a.xmul_in = s.xmul_in;
+    a.xmul_out = s.xmul_out;
+    a.xvalid_in = s.xvalid_in;
+    a.xvalid_out = s.xvalid_out;
+    a.xtw_m = s.xtw_m;
+    a.xtw_bsh = s.xtw_bsh;
+    return a;
+}
+
+// Arguments of a long real line's prep (launch 1) or post (launch 2) piece.
+template <typename T>
+static LongArgs<T> fill_long_args(const PlanImpl &P, const Step &s, const void *in, void *out, const int *gate)
+{
+    LongArgs<T> a{};
+    a.src = in;
+    a.dst = out;
+    a.tab = reinterpret_cast<const cpx<T> *>(P.dev_tables);
+    a.gate = gate;
+    a.nlines = s.nlines;
+    a.g = s.g;
+    for (int d = 0; d < a.g.nd; ++d) {
+        const FastDiv f = make_fastdiv((uint32_t)a.g.ext[d]);
+        a.g.dmag[d] = f.mag;
+        a.g.dsh[d] = f.sh;
+    }
+    a.ips = s.ips;
+    a.ops = s.ops;
+    a.kind = s.long_kind;
+    a.flags = s.flags;
+    a.n = s.n;
+    a.m = s.m;
+    a.nout = s.nout;
+    a.pre0 = s.pre0;
+    a.post0 = s.post0;
+    a.scale = (T)s.scale;
+    a.prew0 = (T)s.prew0;
+    a.prew1 = (T)s.prew1;
+    a.postw0 = (T)s.postw0;
+    a.postw1 = (T)s.postw1;
     return a;
 }
 
@@ -1741,6 +2380,7 @@ template <typename T>
 static pfh_status launch_step_ptrs(const PlanImpl &P, const Step &s, const void *in, void *out, const WsView &wv,
                                    hipStream_t stream)
 {
+    if (!s.pieces.empty()) return set_err(PFH_ERR_PREPARATION_FAILED, "a long-line stage runs through its pieces");
     if (s.endpoint_check) {
         PFH_HIP_CHECK(launch_check<T>(s, in, wv, stream));
     }
@@ -1748,7 +2388,7 @@ static pfh_status launch_step_ptrs(const PlanImpl &P, const Step &s, const void 
         PFH_HIP_CHECK(launch_local_endpoint_check<T>(in, s.nlines, s.nin, (s.n % 2 == 0) ? 1 : 0, wv.first_code,
                                                      wv.gate, wv.status, stream));
     }
-    void *io[4] = {const_cast<void *>(in), out, nullptr, nullptr};
+    void *io[BUF_COUNT] = {const_cast<void *>(in), out};
     Step sp = s;
     sp.in_buf = 0;
     sp.out_buf = 1;
@@ -1765,10 +2405,48 @@ static pfh_status launch_step_ptrs(const PlanImpl &P, const Step &s, const void 
 }
 
 template <typename T>
-static pfh_status launch_step(const PlanImpl &P, const Step &s, void *const bufs[4], const WsView &wv,
+static pfh_status launch_step(const PlanImpl &P, const Step &s, void *const bufs[BUF_COUNT], const WsView &wv,
                               hipStream_t stream)
 {
     return launch_step_ptrs<T>(P, s, bufs[s.in_buf], bufs[s.out_buf], wv, stream);
+}
+
+// A stage that runs as pieces: its boundary check on the stage's input,
+// then every piece in order, all reading the gate when the stage does.
+template <typename T>
+static pfh_status launch_pieces(const PlanImpl &P, const Step &s, void *const bufs[BUF_COUNT], const WsView &wv,
+                                hipStream_t stream)
+{
+    const void *in = bufs[s.in_buf];
+    if (s.endpoint_check) {
+        PFH_HIP_CHECK(launch_check<T>(s, in, wv, stream));
+    }
+    if (s.local_endpoint_check) {
+        PFH_HIP_CHECK(launch_local_endpoint_check<T>(in, s.nlines, s.nin, (s.n % 2 == 0) ? 1 : 0, wv.first_code,
+                                                     wv.gate, wv.status, stream));
+    }
+    const bool gated = s.endpoint_check || s.local_endpoint_check || s.gated || wv.gate_all;
+    const int *gate = gated ? wv.gate : nullptr;
+    for (const Step &pc : s.pieces) {
+        const void *pin = bufs[pc.in_buf];
+        void *pout = bufs[pc.out_buf];
+        if (!pin || !pout) return set_err(PFH_ERR_PREPARATION_FAILED, "long-line region missing");
+        if (pc.launch == 1) {
+            PFH_HIP_CHECK(launch_long_prep<T>(fill_long_args<T>(P, pc, pin, pout, gate), stream));
+        } else if (pc.launch == 2) {
+            PFH_HIP_CHECK(launch_long_post<T>(fill_long_args<T>(P, pc, pin, pout, gate), stream));
+        } else {
+            void *io[BUF_COUNT] = {const_cast<void *>(pin), pout};
+            Step sp = pc;
+            sp.in_buf = 0;
+            sp.out_buf = 1;
+            const StageArgs<T> a = fill_args<T>(P, sp, io, gate);
+            bool wide = false;
+            for (int R : pc.radices) wide = wide || is_wide_radix(R);
+            PFH_HIP_CHECK(launch_line_stage<T>(a, wide, pc.threads, pc.lds_bytes, pc.grid, stream));
+        }
+    }
+    return PFH_OK;
 }
 
 static pfh_status launch_any(const PlanImpl &P, const Step &s, const void *in, void *out, const WsView &wv,
@@ -1776,6 +2454,13 @@ static pfh_status launch_any(const PlanImpl &P, const Step &s, const void *in, v
 {
     return (P.precision == PFH_F32) ? launch_step_ptrs<float>(P, s, in, out, wv, stream)
                                     : launch_step_ptrs<double>(P, s, in, out, wv, stream);
+}
+
+static pfh_status launch_any_pieces(const PlanImpl &P, const Step &s, void *const bufs[BUF_COUNT], const WsView &wv,
+                                    hipStream_t stream)
+{
+    return (P.precision == PFH_F32) ? launch_pieces<float>(P, s, bufs, wv, stream)
+                                    : launch_pieces<double>(P, s, bufs, wv, stream);
 }
 
 // Device timing of an execution: events recorded between launches; each
@@ -1804,13 +2489,29 @@ static double step_bytes(const Step &s)
     return (double)s.nlines * ((double)s.nin * s.in.elem + (double)s.nout * s.out.elem);
 }
 
-static pfh_status run_op(const PlanImpl &P, const OpPlan &o, void *const bufs[4], const WsView &wv,
+static double step_bytes(const Step &s);
+
+static pfh_status run_op(const PlanImpl &P, const OpPlan &o, void *const bufs_in[4], const WsView &wv,
                          hipStream_t stream, TimingTrace *tr)
 {
+    void *bufs[BUF_COUNT] = {bufs_in[0], bufs_in[1], bufs_in[2], bufs_in[3], wv.x[0], wv.x[1], wv.x[2]};
     size_t pi = 0;
     for (size_t i = 0; i < o.steps.size(); ++i) {
         const Step &s = o.steps[i];
         int idx = 0;
+        if (!s.pieces.empty()) {
+            pfh_status st = launch_any_pieces(P, s, bufs, wv, stream);
+            if (st) return st;
+            if (tr) {
+                if (tr->mark(stream, idx)) return PFH_ERR_DEVICE;
+                TimingTrace::Entry es;
+                es.axis = s.axis;
+                es.bytes = step_bytes(s);
+                es.iv.push_back(idx);
+                tr->entries.push_back(es);
+            }
+            continue;
+        }
         if (pi < o.pairs.size() && o.pairs[pi].e == (int)i) {
             // slab-pipelined pair: steps i and i + 1 through the slab region
             const SlabPair &pr = o.pairs[pi++];
@@ -2325,15 +3026,24 @@ static pfh_status local_execute_impl(pfh_plan plan, int op, const void *src, siz
         batch = (int64_t)src_elems / lin;
     }
     const bool c2r_check = tmpl.steps[0].local_endpoint_check;
+    const bool long_lines = tmpl.long_lines > 0;
+    // workspace: the status area (C2R) and the piece regions of long lines
+    const int64_t ws_need = long_lines ? op_ws_bytes(P, tmpl) : (c2r_check ? status_area_bytes(P) : 0);
     WsView wv;
-    if (c2r_check) {
-        if (!workspace || (int64_t)workspace_bytes < status_area_bytes(P))
-            return set_err(PFH_ERR_SCRATCH_TOO_SMALL, "status area", (size_t)status_area_bytes(P),
-                           workspace ? workspace_bytes : 0);
+    if (ws_need > 0) {
+        if (!workspace || (int64_t)workspace_bytes < ws_need)
+            return set_err(PFH_ERR_SCRATCH_TOO_SMALL, long_lines ? "long-line workspace" : "status area",
+                           (size_t)ws_need, workspace ? workspace_bytes : 0);
         char *b = static_cast<char *>(workspace);
         wv.status = reinterpret_cast<int *>(b);
         wv.gate = reinterpret_cast<int *>(b + 4);
         wv.first_code = reinterpret_cast<unsigned long long *>(b + 8);
+        if (long_lines) {
+            WsView full;
+            pfh_status vs = view_workspace(P, tmpl, workspace, workspace_bytes, full);
+            if (vs) return vs;
+            for (int r = 0; r < 3; ++r) wv.x[r] = full.x[r];
+        }
     }
     if (batch == 0) return ok();
     if (!src || !dst) return set_err(PFH_ERR_INVALID_ARGUMENT, "null buffer");
@@ -2341,15 +3051,49 @@ static pfh_status local_execute_impl(pfh_plan plan, int op, const void *src, siz
         const size_t sb = (size_t)(batch * lin * tmpl.in.elem), db = (size_t)(batch * lout * tmpl.out.elem);
         if (!(in_place && src == dst) && overlaps(src, sb, dst, db))
             return set_err(PFH_ERR_ALIASED_BUFFERS, "source and destination overlap");
-        if (c2r_check && (overlaps(src, sb, workspace, (size_t)status_area_bytes(P)) ||
-                          overlaps(dst, db, workspace, (size_t)status_area_bytes(P))))
+        if (ws_need > 0 && (overlaps(src, sb, workspace, (size_t)ws_need) || overlaps(dst, db, workspace, (size_t)ws_need)))
             return set_err(PFH_ERR_ALIASED_BUFFERS, "buffers and workspace overlap");
+    }
+    const int tb = (P.precision == PFH_F32) ? 4 : 8;
+    if (long_lines) {
+        // the C2R boundary check covers the whole batch before any chunk
+        // writes; then chunks of long_lines lines run their pieces
+        const Step &s0 = tmpl.steps[0];
+        if (c2r_check) {
+            if (P.precision == PFH_F32) {
+                PFH_HIP_CHECK(launch_local_endpoint_check<float>(src, batch, s0.nin, (s0.n % 2 == 0) ? 1 : 0,
+                                                                 wv.first_code, wv.gate, wv.status, stream));
+            } else {
+                PFH_HIP_CHECK(launch_local_endpoint_check<double>(src, batch, s0.nin, (s0.n % 2 == 0) ? 1 : 0,
+                                                                  wv.first_code, wv.gate, wv.status, stream));
+            }
+        }
+        for (int64_t c0 = 0; c0 < batch; c0 += tmpl.long_lines) {
+            const int64_t cnt = std::min<int64_t>(tmpl.long_lines, batch - c0);
+            OpPlan o = tmpl;
+            Step &s = o.steps[0];
+            s.in.extra = s.out.extra = cnt;
+            s.local_endpoint_check = false;
+            s.gated = c2r_check;
+            int64_t xb[3] = {0, 0, 0};
+            pfh_status st = step_geometry(s);
+            if (!st) st = expand_long(P, s, tb, xb);
+            if (st) return st;
+            for (int r = 0; r < 3; ++r)
+                if (xb[r] > tmpl.x_bytes[r]) return set_err(PFH_ERR_PREPARATION_FAILED, "long-line chunk regions");
+            const char *cs = static_cast<const char *>(src) + c0 * lin * tmpl.in.elem;
+            char *cd = static_cast<char *>(dst) + c0 * lout * tmpl.out.elem;
+            void *bufs[4] = {const_cast<char *>(cs), cd, nullptr, nullptr};
+            st = run_op(P, o, bufs, wv, stream, nullptr);
+            if (st) return st;
+        }
+        if (c2r_check && !(flags & PFH_EXEC_DEFER_STATUS)) return collect_status(P, wv, stream);
+        return ok();
     }
     OpPlan o = tmpl;
     Step &s = o.steps[0];
     s.in.extra = s.out.extra = batch;
     s.flags &= ~F_XCD_REMAP;
-    const int tb = (P.precision == PFH_F32) ? 4 : 8;
     pfh_status st = finish_step(s, P.dev, tb, P.tuning);
     if (st) return st;
     void *bufs[4] = {const_cast<void *>(src), dst, nullptr, nullptr};
@@ -2380,13 +3124,15 @@ pfh_status pfh_local_execute_in_place(pfh_plan plan, int op, void *data, size_t 
         return local_execute_impl(plan, op, data, elems, data, elems, workspace, workspace_bytes, stream_, flags,
                                   true);
     }
-    // packed real / half-complex storage: stage the input through the workspace
-    const int64_t ws_need = status_area_bytes(P) + align256(in_bytes);
+    // packed real / half-complex storage: stage the input through the
+    // workspace, after the status area and the piece regions of long lines
+    const int64_t base = tmpl.long_lines > 0 ? op_ws_bytes(P, tmpl) : status_area_bytes(P);
+    const int64_t ws_need = base + align256(in_bytes);
     if (!workspace || (int64_t)workspace_bytes < ws_need)
         return set_err(PFH_ERR_SCRATCH_TOO_SMALL, "workspace too small", (size_t)ws_need,
                        workspace ? workspace_bytes : 0);
     if (batch == 0) return ok();
-    char *stage = static_cast<char *>(workspace) + status_area_bytes(P);
+    char *stage = static_cast<char *>(workspace) + base;
     if (overlaps(data, (size_t)need, workspace, (size_t)ws_need))
         return set_err(PFH_ERR_ALIASED_BUFFERS, "data and workspace overlap");
     PFH_HIP_CHECK(hipMemcpyAsync(stage, data, (size_t)in_bytes, hipMemcpyDeviceToDevice, stream));
@@ -2667,6 +3413,8 @@ static pfh_status batch_plan(pfh_batch_s &B, const pfh_desc *descs, int count, c
             for (const Step &s : o.steps) {
                 if (s.mode != M_C2C || s.endpoint_check || s.local_endpoint_check)
                     return set_err(PFH_ERR_UNSUPPORTED, "batched problems must select every axis");
+                if (!s.pieces.empty())
+                    return set_err(PFH_ERR_UNSUPPORTED, "batched problems with lines longer than one workgroup");
                 threads = std::max(threads, s.threads);
             }
             need = std::max(need, op_ws_bytes(*P, o));

@@ -47,6 +47,22 @@ __device__ __forceinline__ void line_offsets(const LineGeom &g, int64_t line, in
     }
 }
 
+// Coordinate of line digit xdig of a line (pieces of a long line).
+__device__ __forceinline__ int digit_coord(const LineGeom &g, int64_t line, int xdig)
+{
+    uint32_t x = (uint32_t)line;
+    int c = 0;
+#pragma unroll
+    for (int d = 0; d < kMaxDigits; ++d) {
+        if (d < g.nd) {
+            const uint32_t q = fastdiv(x, g.dmag[d], g.dsh[d]);
+            if (d == xdig) c = (int)(x - q * (uint32_t)g.ext[d]);
+            x = q;
+        }
+    }
+    return c;
+}
+
 __device__ __forceinline__ bool complex_input(int mode)
 {
     return mode == M_COPY_C || mode == M_C2C || mode == M_C2R || mode == M_C2R_ODD;
@@ -462,7 +478,35 @@ __device__ __forceinline__ void load_line(const StageArgs<T> &a, cpx<T> *data, i
         int64_t io, oo;
         line_offsets(a.g, line, io, oo);
         cpx<T> *lln = data + ll * a.ls;
-        if (cin) {
+        if (cin && (a.flags & F_XIN)) {
+            // piece of a long line: point p is index f = p * xmul_in + d of
+            // the long line; indexes from xvalid_in on are zero padding
+            const cpx<T> *s = reinterpret_cast<const cpx<T> *>(a.src) + io;
+            const bool cj = (a.flags & F_CONJ_IN) != 0;
+            const int dco = a.xdig >= 0 ? digit_coord(a.g, line, a.xdig) : 0;
+            for (int base = p0; base < a.nin; base += step * Regs<T>::gunroll) {
+                cpx<T> v[Regs<T>::gunroll];
+#pragma unroll
+                for (int u = 0; u < Regs<T>::gunroll; ++u) {
+                    const int p = base + u * step;
+                    v[u] = mk((T)0, (T)0);
+                    if (p < a.nin && p * a.xmul_in + dco < a.xvalid_in) v[u] = gld<false>(s + (int64_t)p * a.ips);
+                }
+#pragma unroll
+                for (int u = 0; u < Regs<T>::gunroll; ++u) {
+                    const int p = base + u * step;
+                    if (p < a.nin) {
+                        cpx<T> w = v[u];
+                        const int f = p * a.xmul_in + dco;
+                        if (f < a.xvalid_in) {
+                            if (cj) w.y = -w.y;
+                            if (a.xtab_in) w = cmul(w, a.xtab_in[f]);
+                        }
+                        Lset(lln, p, w);
+                    }
+                }
+            }
+        } else if (cin) {
             const cpx<T> *s = reinterpret_cast<const cpx<T> *>(a.src) + io;
             const bool cj = (a.flags & F_CONJ_IN) != 0;
             for (int base = p0; base < a.nin; base += step * Regs<T>::gunroll) {
@@ -555,7 +599,39 @@ __device__ __forceinline__ void store_line(const StageArgs<T> &a, const cpx<T> *
                 }
             }
             } else {
-            if (MG == 1 && complex_output(a.mode)) {
+            if (MG == 1 && complex_output(a.mode) && (a.flags & (F_XOUT | F_XTW))) {
+                // piece of a long line: twiddle by d * k, conjugate, scale,
+                // then the factor of index f = k * xmul_out + d; indexes from
+                // xvalid_out on are not stored
+                cpx<T> *d = reinterpret_cast<cpx<T> *>(a.dst) + oo;
+                const int dco = a.xdig >= 0 ? digit_coord(a.g, line, a.xdig) : 0;
+                const int lo_mask = (1 << a.xtw_bsh) - 1;
+                for (int base = k0; base < a.nout; base += tpl * U) {
+                    cpx<T> v[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout) {
+                            cpx<T> w = L(Z, k);
+                            if (a.flags & F_XTW) {
+                                const int q = dco * k; // < xtw_m: d < N2 and k < N1
+                                w = cmul(w, cmul(a.xtw[q & lo_mask], a.xtw[(1 << a.xtw_bsh) + (q >> a.xtw_bsh)]));
+                            }
+                            if (a.flags & F_CONJ_OUT) w.y = -w.y;
+                            w = scal(w, a.scale);
+                            const int f = k * a.xmul_out + dco;
+                            if ((a.flags & F_XOUT) && a.xtab_out && f < a.xvalid_out) w = cmul(w, a.xtab_out[f]);
+                            v[u] = w;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = base + u * tpl;
+                        if (k < a.nout && (!(a.flags & F_XOUT) || k * a.xmul_out + dco < a.xvalid_out))
+                            gst<false>(d + (int64_t)k * a.ops, v[u]);
+                    }
+                }
+            } else if (MG == 1 && complex_output(a.mode)) {
                 cpx<T> *d = reinterpret_cast<cpx<T> *>(a.dst) + oo;
                 for (int base = k0; base < a.nout; base += tpl * U) {
                     cpx<T> v[U];

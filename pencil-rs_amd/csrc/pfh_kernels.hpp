@@ -97,6 +97,12 @@ int plane_batch_blocks_per_cu(int threads, size_t lds_bytes);
 bool plane_plan_available(int tbytes, int r0, int r1, int r2, int em);
 hipError_t set_plane_lds_limit(size_t bytes);
 
+// Pre- / post-processing of long real lines (pfh_long.hip).
+template <typename T>
+hipError_t launch_long_prep(const LongArgs<T> &a, hipStream_t s);
+template <typename T>
+hipError_t launch_long_post(const LongArgs<T> &a, hipStream_t s);
+
 template <typename T>
 hipError_t launch_endpoint_check(const EndpointArgs &e, const void *data, EndpointPartial *part, int *gate,
                                  int *status, hipStream_t s);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <map>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -53,7 +54,8 @@ struct Layout {
     }
 };
 
-enum BufId : int { BUF_SRC = 0, BUF_DST = 1, BUF_W0 = 2, BUF_W1 = 3, BUF_COUNT = 4 };
+// BUF_X0..BUF_X2: workspace regions of the pieces of long lines.
+enum BufId : int { BUF_SRC = 0, BUF_DST = 1, BUF_W0 = 2, BUF_W1 = 3, BUF_X0 = 4, BUF_X1 = 5, BUF_X2 = 6, BUF_COUNT = 7 };
 
 // One kernel launch of a plan operation.
 struct Step {
@@ -90,6 +92,16 @@ struct Step {
     int fr[3] = {0, 0, 0};
     int fem = 16;
     int digit_axis[kMaxDigits] = {-1, -1, -1, -1, -1}; // logical axis of each line digit (-1: extras)
+    // Long lines: a stage whose line does not fit one workgroup (or whose
+    // FFT length has a large prime factor) runs as `pieces`, in order; a
+    // piece is a generic line stage (launch 0) or the prep / post kernel of
+    // a long real line (launch 1 / 2, long_kind a LongKind).
+    bool long_only = false; // no single-workgroup radix plan: pieces required
+    std::vector<Step> pieces;
+    int launch = 0;
+    int long_kind = 0;
+    int64_t xtab_in = -1, xtab_out = -1, xtw = -1; // table offsets (complex entries), -1: none
+    int xdig = -1, xmul_in = 1, xmul_out = 1, xvalid_in = 0, xvalid_out = 0, xtw_m = 0, xtw_bsh = 0;
 };
 
 // Two consecutive steps run slab by slab over an axis that neither
@@ -116,6 +128,8 @@ struct OpPlan {
     int64_t w_bytes[2] = {0, 0};
     std::vector<SlabPair> pairs; // slab-pipelined step pairs (disjoint, ascending e)
     int64_t t_bytes = 0;         // slab intermediate region of the workspace
+    int64_t x_bytes[3] = {0, 0, 0}; // regions of the pieces of long lines (BUF_X0..BUF_X2)
+    int64_t long_lines = 0;      // local plans: lines per chunk of a long-line operation (0: no pieces)
 };
 
 struct DeviceInfo {
@@ -159,6 +173,8 @@ struct PlanImpl {
     OpPlan ops_inplace[6];
     // twiddle/pre/post tables of every step of every op, one device buffer
     std::vector<double> host_tables; // complex pairs, f64
+    std::map<std::string, int64_t> table_index; // long-line tables already in host_tables (name -> offset)
+    bool tables_frozen = false;                 // host_tables uploaded: no table may be added
     void *dev_tables = nullptr;
     int64_t table_entries = 0;
     // transposes

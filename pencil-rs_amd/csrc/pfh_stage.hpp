@@ -84,6 +84,10 @@ enum StageFlags : int {
     F_DIRECT_FIRST = 1024, // first radix pass reads global memory into registers
     F_DIRECT_LAST = 2048,  // last radix pass writes global memory from registers (C2C)
     F_NT = 4096,           // stage arrays exceed the last-level cache: non-temporal loads and stores
+    // Pieces of a long line (generic kernel, complex lines; see StageArgs::x*):
+    F_XIN = 8192,   // points whose full index reaches xvalid_in load as zero; xtab_in factor on load
+    F_XOUT = 16384, // points whose full index reaches xvalid_out are not stored; xtab_out factor on store
+    F_XTW = 32768,  // twiddle exp(-2 pi i d k / xtw_m) on the store of point k
 };
 
 struct LineGeom {
@@ -149,6 +153,41 @@ struct StageArgs {
     int xcd_per;            // tiles per XCD when F_XCD_REMAP
     T scale;
     T prew0, prew1, postw0, postw1;
+    // Pieces of a long line (F_XIN / F_XOUT / F_XTW): a line too long for one
+    // workgroup runs as several launches. d is the coordinate of line digit
+    // xdig (0 when xdig < 0), and point p of a line stands for the index
+    // p * xmul + d of the long line.
+    const cpx<T> *xtab_in;  // factor of full index f on load (null: none)
+    const cpx<T> *xtab_out; // factor of full index f on store (null: none)
+    const cpx<T> *xtw;      // exp(-2 pi i q / xtw_m) = xtw[q mod 2^xtw_bsh] * xtw[2^xtw_bsh + (q >> xtw_bsh)]
+    int xdig;
+    int xmul_in, xmul_out;
+    int xvalid_in, xvalid_out;
+    int xtw_m, xtw_bsh;
+};
+
+// Pre- and post-processing of a long real line (k_long_prep / k_long_post,
+// pfh_long.hip): the transform runs as a complex FFT of length m of the
+// promoted line (real part, zero imaginary part, or the even / odd
+// extension), whose input the prep kernel writes and whose output the post
+// kernel reads, line-major (line l at l * m).
+enum LongKind : int { LK_R2C = 0, LK_C2R, LK_DCT2, LK_DCT3, LK_DCT4, LK_DCT1, LK_DST1, LK_DHT };
+
+template <typename T>
+struct LongArgs {
+    const void *src;
+    void *dst;
+    const cpx<T> *tab;   // plan tables (pre0 / post0 offsets)
+    const int *gate;     // if non-null and *gate != 0 nothing is written
+    int64_t nlines;
+    LineGeom g;          // the stage's lines: is = input strides, os = output strides
+    int64_t ips, ops;    // point strides of the stage's input / output
+    int kind, flags;     // LongKind, stage flags (F_PRE_* / F_POST_* / F_WEIGHTS / F_HERM_*)
+    int n;               // logical line length
+    int m;               // promoted complex FFT length
+    int nout;            // output points per line (post)
+    int pre0, post0;     // table offsets
+    T scale, prew0, prew1, postw0, postw1;
 };
 
 // Two route stages fused over whole planes (k_plane_batch): a plane of nc

@@ -10,7 +10,10 @@
 //    points, C2R zero-fills), the output array must hold the input value at
 //    the same logical coordinates, so routes, fused transitions and slab
 //    pipelining are checked end to end.
-// The FFT arithmetic itself is not emulated (the GPU tests check it).
+// The FFT arithmetic is not emulated there (the GPU tests check it), except
+// in the `long` scenario: the pieces of long lines (split and Bluestein
+// line stages, the prep / post kernels of real lines) are computed in long
+// double, and whole transforms are compared with naive sums.
 #include <hip/hip_runtime_api.h>
 
 #include <cinttypes>
@@ -338,11 +341,101 @@ static void check_shape(const char *what, const StageArgs<T> &a, int threads, si
         fail("%s: pre-processing not covered", what);
 }
 
+bool g_math = false; // the `long` scenario: line stages compute their transform
+bool g_math_dft = true; // false: bounds and shapes only (the DFTs of real-size lines take too long here)
+
+using ld = long double;
+struct cld {
+    ld re, im;
+};
+static cld cmul_l(cld a, cld b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+
+// Arithmetic emulation of a C2C line stage, pieces of long lines included:
+// load (zero padding, conjugation, load factors), DFT of length m, store
+// (twiddle d k, conjugation, scale, store factors, crop).
+template <typename T>
+static void emulate_math(const char *what, const StageArgs<T> &a)
+{
+    if (a.mode != M_C2C) {
+        fail("%s: arithmetic emulation of mode %d", what, a.mode);
+        return;
+    }
+    const int m = a.m;
+    if (a.nin != m || a.nout != m) fail("%s: nin %d / nout %d for m %d", what, a.nin, a.nout, m);
+    std::vector<cld> z(m), Z(m);
+    const ld pi = 3.141592653589793238462643383279502884L;
+    for (int64_t line = 0; line < a.nlines; ++line) {
+        int64_t io, oo;
+        offsets(a.g, line, io, oo);
+        int dco = 0;
+        {
+            int64_t x = line;
+            for (int d = 0; d < a.g.nd; ++d) {
+                if (d == a.xdig) dco = (int)(x % a.g.ext[d]);
+                x /= a.g.ext[d];
+            }
+        }
+        for (int p = 0; p < m; ++p) {
+            cld v{0, 0};
+            const int64_t f = (int64_t)p * a.xmul_in + dco;
+            if (!(a.flags & F_XIN) || f < a.xvalid_in) {
+                const cpx<T> *q = reinterpret_cast<const cpx<T> *>(a.src) + io + (int64_t)p * a.ips;
+                if (!check_read(what, q, sizeof(*q))) return;
+                v = {(ld)q->x, (ld)q->y};
+                if (a.flags & F_CONJ_IN) v.im = -v.im;
+                if ((a.flags & F_XIN) && a.xtab_in) {
+                    if (!check_read("xtab_in", a.xtab_in + f, sizeof(cpx<T>))) return;
+                    v = cmul_l(v, {(ld)a.xtab_in[f].x, (ld)a.xtab_in[f].y});
+                }
+            }
+            z[p] = v;
+        }
+        for (int k = 0; k < m && !g_math_dft; ++k) Z[k] = z[k];
+        for (int k = 0; k < m && g_math_dft; ++k) {
+            cld acc{0, 0};
+            for (int p = 0; p < m; ++p) {
+                const ld ang = -2 * pi * (ld)(((int64_t)p * k) % m) / (ld)m;
+                acc = {acc.re + z[p].re * cosl(ang) - z[p].im * sinl(ang),
+                       acc.im + z[p].re * sinl(ang) + z[p].im * cosl(ang)};
+            }
+            Z[k] = acc;
+        }
+        for (int k = 0; k < m; ++k) {
+            cld w = Z[k];
+            const int64_t f = (int64_t)k * a.xmul_out + dco;
+            if ((a.flags & F_XOUT) && f >= a.xvalid_out) continue;
+            if (a.flags & F_XTW) {
+                const int64_t q = (int64_t)dco * k;
+                if (q >= a.xtw_m) fail("%s: twiddle index %" PRId64 " >= %d", what, q, a.xtw_m);
+                const int64_t lo = q & ((1 << a.xtw_bsh) - 1), hi = (1 << a.xtw_bsh) + (q >> a.xtw_bsh);
+                if (!check_read("xtw", a.xtw + lo, sizeof(cpx<T>)) || !check_read("xtw", a.xtw + hi, sizeof(cpx<T>)))
+                    return;
+                w = cmul_l(w, cmul_l({(ld)a.xtw[lo].x, (ld)a.xtw[lo].y}, {(ld)a.xtw[hi].x, (ld)a.xtw[hi].y}));
+            }
+            if (a.flags & F_CONJ_OUT) w.im = -w.im;
+            w = {w.re * (ld)a.scale, w.im * (ld)a.scale};
+            if ((a.flags & F_XOUT) && a.xtab_out) {
+                if (!check_read("xtab_out", a.xtab_out + f, sizeof(cpx<T>))) return;
+                w = cmul_l(w, {(ld)a.xtab_out[f].x, (ld)a.xtab_out[f].y});
+            }
+            cpx<T> *q = reinterpret_cast<cpx<T> *>(a.dst) + oo + (int64_t)k * a.ops;
+            if (!check_write(what, q, sizeof(*q))) return;
+            q->x = (T)w.re;
+            q->y = (T)w.im;
+        }
+    }
+}
+
 // identity-on-points emulation of one line stage
 template <typename T>
 static void emulate(const char *what, const StageArgs<T> &a)
 {
     if (a.gate && *a.gate) return;
+    if (g_math) {
+        emulate_math(what, a);
+        return;
+    }
+    if (a.flags & (F_XIN | F_XOUT | F_XTW)) fail("%s: long-line piece outside the arithmetic scenario", what);
     const bool ci = mode_cin(a.mode), co = mode_cout(a.mode);
     const size_t ei = ci ? 2 * sizeof(T) : sizeof(T), eo = co ? 2 * sizeof(T) : sizeof(T);
     std::vector<T> re(a.nin), im(a.nin);
@@ -412,6 +505,164 @@ hipError_t launch_line_stage(const StageArgs<T> &a, bool wide, int threads, size
 }
 template hipError_t launch_line_stage<float>(const StageArgs<float> &, bool, int, size_t, int64_t, hipStream_t);
 template hipError_t launch_line_stage<double>(const StageArgs<double> &, bool, int, size_t, int64_t, hipStream_t);
+
+// The prep / post kernels of long real lines, restated on the host.
+template <typename T>
+static bool long_checks(const char *what, const LongArgs<T> &a)
+{
+    ++g_launches;
+    ++g_all_launches;
+    begin_launch();
+    if (!g_math) {
+        fail("%s outside the arithmetic scenario", what);
+        return false;
+    }
+    if (a.nlines < 1 || a.m < 1 || a.n < 1) {
+        fail("%s: empty shape", what);
+        return false;
+    }
+    return !(a.gate && *a.gate);
+}
+
+template <typename T>
+static int64_t long_base(const LongArgs<T> &a, int64_t line, bool out)
+{
+    int64_t io, oo;
+    offsets(a.g, line, io, oo);
+    return out ? oo : io;
+}
+
+template <typename T>
+hipError_t launch_long_prep(const LongArgs<T> &a, hipStream_t)
+{
+    if (!long_checks("long_prep", a)) return hipSuccess;
+    const int n = a.n;
+    for (int64_t line = 0; line < a.nlines; ++line) {
+        const int64_t io = long_base(a, line, false);
+        auto X = [&](int i) -> ld {
+            const int idx = (a.flags & F_PRE_REV) ? n - 1 - i : i;
+            const T *q = reinterpret_cast<const T *>(a.src) + io + (int64_t)idx * a.ips;
+            if (idx < 0 || idx >= n) fail("long_prep: point %d of %d", idx, n);
+            check_read("long_prep", q, sizeof(T));
+            ld v = (ld)*q;
+            if ((a.flags & F_PRE_ALT) && (idx & 1)) v = -v;
+            if (a.flags & F_WEIGHTS) {
+                if (idx == 0) v *= (ld)a.prew0;
+                if (idx == n - 1) v *= (ld)a.prew1;
+            }
+            return v;
+        };
+        auto H = [&](int k, int last) -> cld {
+            const cpx<T> *q = reinterpret_cast<const cpx<T> *>(a.src) + io + (int64_t)k * a.ips;
+            check_read("long_prep", q, sizeof(*q));
+            cld v{(ld)q->x, (ld)q->y};
+            if (k == 0 || k == last) v.im = 0;
+            if ((a.flags & F_HERM_HALF) && k > 0 && k < last) v = {v.re / 2, v.im / 2};
+            return v;
+        };
+        auto tab = [&](int off) -> cld {
+            check_read("long_prep table", a.tab + off, sizeof(cpx<T>));
+            return {(ld)a.tab[off].x, (ld)a.tab[off].y};
+        };
+        for (int j = 0; j < a.m; ++j) {
+            cld z{0, 0};
+            switch (a.kind) {
+            case LK_C2R: {
+                const int h = n / 2, last = (n % 2 == 0) ? h : h + 1;
+                cld v = j <= h ? H(j, last) : H(n - j, last);
+                if (j > h) v.im = -v.im;
+                z = {v.re, -v.im};
+                break;
+            }
+            case LK_DCT2: z = {j < (n + 1) / 2 ? X(2 * j) : X(2 * (n - 1 - j) + 1), 0}; break;
+            case LK_DCT3: {
+                ld y = X(j);
+                if (j == 0) y /= 2;
+                const cld t = tab(a.pre0 + j);
+                z = {t.re * y, t.im * y};
+                break;
+            }
+            case LK_DCT4:
+                if (j < n) {
+                    const cld t = tab(a.pre0 + j);
+                    const ld y = X(j);
+                    z = {t.re * y, t.im * y};
+                }
+                break;
+            case LK_DCT1: z = {j <= n - 1 ? X(j) : X(2 * (n - 1) - j), 0}; break;
+            case LK_DST1:
+                if (j != 0 && j != n + 1) z = {j <= n ? X(j - 1) : -X(2 * n + 1 - j), 0};
+                break;
+            default: z = {X(j), 0}; break;
+            }
+            cpx<T> *q = reinterpret_cast<cpx<T> *>(a.dst) + line * (int64_t)a.m + j;
+            if (!check_write("long_prep", q, sizeof(*q))) return hipSuccess;
+            q->x = (T)z.re;
+            q->y = (T)z.im;
+        }
+    }
+    return hipSuccess;
+}
+
+template <typename T>
+hipError_t launch_long_post(const LongArgs<T> &a, hipStream_t)
+{
+    if (!long_checks("long_post", a)) return hipSuccess;
+    const int n = a.n;
+    for (int64_t line = 0; line < a.nlines; ++line) {
+        const int64_t oo = long_base(a, line, true);
+        auto Z = [&](int i) -> cld {
+            if (i < 0 || i >= a.m) fail("long_post: index %d of %d", i, a.m);
+            const cpx<T> *q = reinterpret_cast<const cpx<T> *>(a.src) + line * (int64_t)a.m + i;
+            check_read("long_post", q, sizeof(*q));
+            return {(ld)q->x, (ld)q->y};
+        };
+        for (int k = 0; k < a.nout; ++k) {
+            if (a.kind == LK_R2C) {
+                const int last = (n % 2 == 0) ? n / 2 : -1;
+                cld v = Z(k);
+                if (k == 0 || k == last) v.im = 0;
+                if (a.flags & F_HERM_DOUBLE) {
+                    if (k > 0 && k != last) v = {2 * v.re, 2 * v.im};
+                    else v.im = 0;
+                }
+                cpx<T> *q = reinterpret_cast<cpx<T> *>(a.dst) + oo + (int64_t)k * a.ops;
+                if (!check_write("long_post", q, sizeof(*q))) return hipSuccess;
+                q->x = (T)(v.re * (ld)a.scale);
+                q->y = (T)(v.im * (ld)a.scale);
+                continue;
+            }
+            const int kk = (a.flags & F_POST_REV) ? n - 1 - k : k;
+            ld y = 0;
+            switch (a.kind) {
+            case LK_DCT2:
+            case LK_DCT4: {
+                check_read("long_post table", a.tab + a.post0 + kk, sizeof(cpx<T>));
+                const cld t{(ld)a.tab[a.post0 + kk].x, (ld)a.tab[a.post0 + kk].y};
+                y = 2 * cmul_l(t, Z(kk)).re;
+                break;
+            }
+            case LK_DCT3: y = 2 * Z((kk & 1) ? n - 1 - (kk >> 1) : (kk >> 1)).re; break;
+            case LK_DST1: y = -Z(kk + 1).im; break;
+            case LK_DHT: y = Z(kk).re - Z(kk).im; break;
+            default: y = Z(kk).re; break;
+            }
+            if ((a.flags & F_POST_ALT) && (k & 1)) y = -y;
+            if (a.flags & F_WEIGHTS) {
+                if (k == 0) y *= (ld)a.postw0;
+                if (k == n - 1) y *= (ld)a.postw1;
+            }
+            T *q = reinterpret_cast<T *>(a.dst) + oo + (int64_t)k * a.ops;
+            if (!check_write("long_post", q, sizeof(*q))) return hipSuccess;
+            *q = (T)(y * (ld)a.scale);
+        }
+    }
+    return hipSuccess;
+}
+template hipError_t launch_long_prep<float>(const LongArgs<float> &, hipStream_t);
+template hipError_t launch_long_prep<double>(const LongArgs<double> &, hipStream_t);
+template hipError_t launch_long_post<float>(const LongArgs<float> &, hipStream_t);
+template hipError_t launch_long_post<double>(const LongArgs<double> &, hipStream_t);
 
 template <typename T>
 hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int em, int threads, size_t lds_bytes,
@@ -1466,6 +1717,296 @@ static void contract_cases()
     std::printf("contract: %s\n", g_fail == before_fail ? "ok" : "FAILED");
 }
 
+// ---------------------------------------------------------------------------
+// `long` scenario: lines split into pieces (four-step and Bluestein), with
+// the arithmetic emulated, against naive sums in long double
+// ---------------------------------------------------------------------------
+
+uint64_t g_rng = 0x243f6a8885a308d3ull;
+double urand()
+{
+    g_rng ^= g_rng << 13;
+    g_rng ^= g_rng >> 7;
+    g_rng ^= g_rng << 17;
+    return (double)(g_rng >> 11) / (double)(1ull << 53) * 2.0 - 1.0;
+}
+
+const long double kPi = 3.141592653589793238462643383279502884L;
+
+// Naive line transforms (FFTW conventions, unnormalized unless noted).
+std::vector<pfh::cld> naive_dft(const std::vector<pfh::cld> &x, int sign, long double scale)
+{
+    const int64_t n = (int64_t)x.size();
+    std::vector<pfh::cld> y(n);
+    for (int64_t k = 0; k < n; ++k) {
+        long double re = 0, im = 0;
+        for (int64_t j = 0; j < n; ++j) {
+            const long double a = sign * 2 * kPi * (long double)((j * k) % n) / (long double)n;
+            re += x[j].re * cosl(a) - x[j].im * sinl(a);
+            im += x[j].re * sinl(a) + x[j].im * cosl(a);
+        }
+        y[k] = {re * scale, im * scale};
+    }
+    return y;
+}
+
+std::vector<long double> naive_r2r(int kind, const std::vector<long double> &x)
+{
+    const int64_t n = (int64_t)x.size();
+    std::vector<long double> y(n, 0);
+    for (int64_t k = 0; k < n; ++k) {
+        long double acc = 0;
+        for (int64_t j = 0; j < n; ++j) {
+            const long double v = x[j];
+            switch (kind) {
+            case PFH_DCT1:
+                if (j == 0) acc += v;
+                else if (j == n - 1) acc += (k % 2 ? -v : v);
+                else acc += 2 * v * cosl(kPi * j * k / (n - 1));
+                break;
+            case PFH_DCT2: acc += 2 * v * cosl(kPi * (j + 0.5L) * k / n); break;
+            case PFH_DCT3: acc += (j == 0) ? v : 2 * v * cosl(kPi * j * (k + 0.5L) / n); break;
+            case PFH_DCT4: acc += 2 * v * cosl(kPi * (j + 0.5L) * (k + 0.5L) / n); break;
+            case PFH_DST1: acc += 2 * v * sinl(kPi * (j + 1) * (k + 1) / (n + 1)); break;
+            case PFH_DST2: acc += 2 * v * sinl(kPi * (j + 0.5L) * (k + 1) / n); break;
+            case PFH_DST3:
+                acc += (j == n - 1) ? (k % 2 ? -v : v) : 2 * v * sinl(kPi * (j + 1) * (k + 0.5L) / n);
+                break;
+            case PFH_DST4: acc += 2 * v * sinl(kPi * (j + 0.5L) * (k + 0.5L) / n); break;
+            default: { // DHT
+                const long double a = 2 * kPi * (long double)((j * k) % n) / n;
+                acc += v * (cosl(a) + sinl(a));
+            }
+            }
+        }
+        y[k] = acc;
+    }
+    return y;
+}
+
+template <typename T>
+void local_long_case(int kind, int r2r, int64_t n, int op, int64_t batch, double tol)
+{
+    char name[96];
+    std::snprintf(name, sizeof name, "long local kind %d r2r %d n %" PRId64 " op %d %s", kind, r2r, n, op,
+                  sizeof(T) == 4 ? "f32" : "f64");
+    pfh_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.kind = kind;
+    d.precision = sizeof(T) == 4 ? PFH_F32 : PFH_F64;
+    d.line_len = (size_t)n;
+    d.r2r_kind = r2r;
+    pfh_plan p = nullptr;
+    if (pfh_plan_create(&d, &p) != PFH_OK) {
+        pfh_error_info e;
+        pfh_last_error(&e);
+        fail("%s: plan creation: %s", name, e.message);
+        return;
+    }
+    const bool r2c = kind == PFH_LOCAL_R2C, c2r = r2c && op == PFH_INVERSE;
+    const bool cin = kind == PFH_LOCAL_C2C || c2r, cout = kind == PFH_LOCAL_C2C || (r2c && !c2r);
+    const int64_t lin = c2r ? n / 2 + 1 : n, lout = (r2c && !c2r) ? n / 2 + 1 : n;
+    size_t wsb = 0;
+    pfh_plan_workspace_bytes(p, op, &wsb);
+    T *src = static_cast<T *>(dmalloc((size_t)(batch * lin * (cin ? 2 : 1)) * sizeof(T)));
+    T *dst = static_cast<T *>(dmalloc((size_t)(batch * lout * (cout ? 2 : 1)) * sizeof(T)));
+    void *ws = dmalloc(wsb ? wsb : 1, true);
+    for (int64_t i = 0; i < batch * lin * (cin ? 2 : 1); ++i) src[i] = (T)urand();
+    const int64_t launches = pfh::g_launches;
+    if (pfh_local_execute(p, op, src, (size_t)(batch * lin), dst, (size_t)(batch * lout), ws, wsb, nullptr, 0) !=
+        PFH_OK) {
+        pfh_error_info e;
+        pfh_last_error(&e);
+        fail("%s: execute: %s", name, e.message);
+    } else if (pfh::g_launches - launches < 2) {
+        fail("%s: ran as %" PRId64 " launch(es), not as pieces", name, pfh::g_launches - launches);
+    } else if (!pfh::g_math_dft) {
+        // shapes and bounds only
+    } else {
+        long double num = 0, den = 0;
+        for (int64_t b = 0; b < batch; ++b) {
+            std::vector<long double> got, ref;
+            if (kind == PFH_LOCAL_C2C) {
+                std::vector<pfh::cld> x(n);
+                for (int64_t j = 0; j < n; ++j) x[j] = {(long double)src[2 * (b * n + j)], (long double)src[2 * (b * n + j) + 1]};
+                const int sign = op == PFH_FORWARD ? -1 : 1;
+                const auto y = naive_dft(x, sign, op == PFH_INVERSE ? 1.0L / n : 1.0L);
+                for (int64_t k = 0; k < n; ++k) {
+                    ref.push_back(y[k].re);
+                    ref.push_back(y[k].im);
+                    got.push_back(dst[2 * (b * n + k)]);
+                    got.push_back(dst[2 * (b * n + k) + 1]);
+                }
+            } else if (r2c && !c2r) {
+                std::vector<pfh::cld> x(n);
+                for (int64_t j = 0; j < n; ++j) x[j] = {(long double)src[b * n + j], 0};
+                const auto y = naive_dft(x, -1, 1.0L);
+                for (int64_t k = 0; k < lout; ++k) {
+                    ref.push_back(y[k].re);
+                    ref.push_back(y[k].im);
+                    got.push_back(dst[2 * (b * lout + k)]);
+                    got.push_back(dst[2 * (b * lout + k) + 1]);
+                }
+            } else if (c2r) {
+                std::vector<pfh::cld> x(n);
+                for (int64_t k = 0; k <= n / 2; ++k) {
+                    pfh::cld v{(long double)src[2 * (b * lin + k)], (long double)src[2 * (b * lin + k) + 1]};
+                    if (k == 0 || (n % 2 == 0 && k == n / 2)) v.im = 0;
+                    x[k] = v;
+                    if (k > 0) x[n - k] = {v.re, -v.im};
+                }
+                const auto y = naive_dft(x, 1, 1.0L / n);
+                for (int64_t j = 0; j < n; ++j) {
+                    ref.push_back(y[j].re);
+                    got.push_back(dst[b * n + j]);
+                }
+            } else {
+                std::vector<long double> x(n);
+                for (int64_t j = 0; j < n; ++j) x[j] = src[b * n + j];
+                ref = naive_r2r(kind == PFH_LOCAL_DHT ? PFH_DHT : r2r, x);
+                for (int64_t k = 0; k < n; ++k) got.push_back(dst[b * n + k]);
+            }
+            for (size_t i = 0; i < ref.size(); ++i) {
+                num += (got[i] - ref[i]) * (got[i] - ref[i]);
+                den += ref[i] * ref[i];
+            }
+        }
+        const double err = (double)sqrtl(num / (den > 0 ? den : 1));
+        if (!(err <= tol)) fail("%s: relative L2 error %.3g > %.3g", name, err, tol);
+    }
+    for (void *q : {(void *)src, (void *)dst, ws}) (void)hipFree(q);
+    pfh_plan_destroy(p);
+}
+
+// A 3D C2C route with a long axis against the naive separable transform.
+template <typename T>
+void route_long_case(std::vector<size_t> shape, size_t extra, int op, double tol)
+{
+    char name[96];
+    std::snprintf(name, sizeof name, "long route c2c %zux%zux%zu extra %zu op %d %s", shape[0], shape[1], shape[2],
+                  extra, op, sizeof(T) == 4 ? "f32" : "f64");
+    pfh_desc d = route_desc(PFH_C2C, sizeof(T) == 4 ? PFH_F32 : PFH_F64, shape,
+                            extra > 1 ? std::vector<size_t>{extra} : std::vector<size_t>{});
+    pfh_plan p = nullptr;
+    if (pfh_plan_create(&d, &p) != PFH_OK) {
+        pfh_error_info e;
+        pfh_last_error(&e);
+        fail("%s: plan creation: %s", name, e.message);
+        return;
+    }
+    int perm[2][PFH_MAX_DIMS];
+    size_t shp[2][PFH_MAX_DIMS], elems[2], eb[2];
+    for (int side = 0; side < 2; ++side) pfh_plan_layout(p, side, perm[side], shp[side], &elems[side], &eb[side]);
+    const int64_t len = (int64_t)(shape[0] * shape[1] * shape[2]);
+    auto mem_index = [&](int side, const int64_t c[3]) {
+        int64_t idx = 0;
+        for (int pos = 0; pos < 3; ++pos) idx = idx * (int64_t)shape[perm[side][pos]] + c[perm[side][pos]];
+        return idx;
+    };
+    // forward: input layout -> output layout; inverse / backward the other way
+    const int sin = op == PFH_FORWARD ? 0 : 1, sout = 1 - sin;
+    size_t wsb = 0;
+    pfh_plan_workspace_bytes(p, op, &wsb);
+    T *src = static_cast<T *>(dmalloc(elems[sin] * 2 * sizeof(T)));
+    T *dst = static_cast<T *>(dmalloc(elems[sout] * 2 * sizeof(T)));
+    void *ws = dmalloc(wsb, true);
+    for (size_t i = 0; i < elems[sin] * 2; ++i) src[i] = (T)urand();
+    if (pfh_execute(p, op, src, elems[sin], dst, elems[sout], ws, wsb, nullptr, 0) != PFH_OK) {
+        pfh_error_info e;
+        pfh_last_error(&e);
+        fail("%s: execute: %s", name, e.message);
+    } else if (!pfh::g_math_dft) {
+        // shapes and bounds only
+    } else {
+        const int sign = op == PFH_FORWARD ? -1 : 1;
+        long double num = 0, den = 0;
+        for (size_t e = 0; e < extra; ++e) {
+            // logical array [a0][a1][a2]
+            std::vector<pfh::cld> A((size_t)len);
+            int64_t c[3];
+            for (c[0] = 0; c[0] < (int64_t)shape[0]; ++c[0])
+                for (c[1] = 0; c[1] < (int64_t)shape[1]; ++c[1])
+                    for (c[2] = 0; c[2] < (int64_t)shape[2]; ++c[2]) {
+                        const int64_t mi = (int64_t)e * len + mem_index(sin, c);
+                        A[(c[0] * shape[1] + c[1]) * shape[2] + c[2]] = {(long double)src[2 * mi], (long double)src[2 * mi + 1]};
+                    }
+            const int64_t st[3] = {(int64_t)(shape[1] * shape[2]), (int64_t)shape[2], 1};
+            for (int ax = 0; ax < 3; ++ax) {
+                const int64_t na = (int64_t)shape[ax];
+                for (int64_t base = 0; base < len; ++base) {
+                    if ((base / st[ax]) % na != 0) continue; // first point of each line along ax
+                    std::vector<pfh::cld> x(na);
+                    for (int64_t j = 0; j < na; ++j) x[j] = A[base + j * st[ax]];
+                    const auto y = naive_dft(x, sign, op == PFH_INVERSE ? 1.0L / na : 1.0L);
+                    for (int64_t j = 0; j < na; ++j) A[base + j * st[ax]] = y[j];
+                }
+            }
+            for (c[0] = 0; c[0] < (int64_t)shape[0]; ++c[0])
+                for (c[1] = 0; c[1] < (int64_t)shape[1]; ++c[1])
+                    for (c[2] = 0; c[2] < (int64_t)shape[2]; ++c[2]) {
+                        const int64_t mo = (int64_t)e * len + mem_index(sout, c);
+                        const pfh::cld r = A[(c[0] * shape[1] + c[1]) * shape[2] + c[2]];
+                        const long double gr = dst[2 * mo], gi = dst[2 * mo + 1];
+                        num += (gr - r.re) * (gr - r.re) + (gi - r.im) * (gi - r.im);
+                        den += r.re * r.re + r.im * r.im;
+                    }
+        }
+        const double err = (double)sqrtl(num / den);
+        if (!(err <= tol)) fail("%s: relative L2 error %.3g > %.3g", name, err, tol);
+    }
+    for (void *q : {(void *)src, (void *)dst, ws}) (void)hipFree(q);
+    pfh_plan_destroy(p);
+}
+
+void long_cases()
+{
+    const int before = g_fail;
+    pfh::g_math = true;
+    // small thresholds, so that small lines exercise every piece shape:
+    // lines above 32 points split, prime factors above 7 go through Bluestein
+    setenv("PFH_LONG_LINE_MAX", "32", 1);
+    setenv("PFH_BLUESTEIN_PRIME", "7", 1);
+    for (int op : {PFH_FORWARD, PFH_BACKWARD, PFH_INVERSE})
+        for (int64_t n : {48, 64, 11, 37, 97, 160})
+            local_long_case<double>(PFH_LOCAL_C2C, 0, n, op, 3, 1e-12);
+    for (int64_t n : {96, 22, 75, 74})
+        for (int op : {PFH_FORWARD, PFH_INVERSE}) local_long_case<double>(PFH_LOCAL_R2C, 0, n, op, 2, 1e-12);
+    for (int k = PFH_DCT1; k <= PFH_DST4; ++k)
+        for (int64_t n : {70, 45}) local_long_case<double>(PFH_LOCAL_R2R, k, n, PFH_FORWARD, 2, 1e-12);
+    for (int64_t n : {70, 45}) local_long_case<double>(PFH_LOCAL_DHT, 0, n, PFH_FORWARD, 2, 1e-12);
+    local_long_case<float>(PFH_LOCAL_C2C, 0, 48, PFH_FORWARD, 2, 2e-6);
+    local_long_case<float>(PFH_LOCAL_C2C, 0, 37, PFH_INVERSE, 2, 2e-6);
+    local_long_case<float>(PFH_LOCAL_R2C, 0, 22, PFH_FORWARD, 2, 2e-6);
+    for (int op : {PFH_FORWARD, PFH_INVERSE}) {
+        route_long_case<double>({37, 6, 4}, 1, op, 1e-12);
+        route_long_case<double>({5, 48, 3}, 2, op, 1e-12);
+        route_long_case<double>({4, 3, 41}, 1, op, 1e-12);
+    }
+    route_long_case<float>({6, 37, 5}, 1, PFH_FORWARD, 2e-6);
+    unsetenv("PFH_LONG_LINE_MAX");
+    unsetenv("PFH_BLUESTEIN_PRIME");
+    // real-size lines with the default thresholds: launch shapes, thread
+    // mappings, LDS and every address, without the arithmetic
+    pfh::g_math_dft = false;
+    for (int64_t n : {32768, 65536, 1009, 10007, 1 << 20}) {
+        for (int op : {PFH_FORWARD, PFH_INVERSE}) {
+            local_long_case<float>(PFH_LOCAL_C2C, 0, n, op, 3, 0);
+            local_long_case<float>(PFH_LOCAL_R2C, 0, 2 * n, op, 2, 0);
+        }
+        local_long_case<double>(PFH_LOCAL_C2C, 0, n, PFH_FORWARD, 2, 0);
+        local_long_case<float>(PFH_LOCAL_R2R, PFH_DCT2, 2 * n, PFH_FORWARD, 2, 0);
+    }
+    for (int64_t n : {10006, 65535}) local_long_case<double>(PFH_LOCAL_R2R, PFH_DST1, n, PFH_FORWARD, 2, 0);
+    for (int op : {PFH_FORWARD, PFH_INVERSE}) {
+        route_long_case<float>({10007, 3, 4}, 2, op, 0);
+        route_long_case<float>({4, 65536, 3}, 1, op, 0);
+        route_long_case<double>({3, 4, 1009}, 1, op, 0);
+    }
+    pfh::g_math_dft = true;
+    pfh::g_math = false;
+    std::printf("long lines: %s\n", g_fail == before ? "ok" : "FAILED");
+}
+
 // The batch tuning cache: a measured tuning is reused within the process and,
 // through the cache file, by a later creation after the process map is
 // cleared; a record that does not parse is skipped; a cache switched off
@@ -1562,6 +2103,11 @@ void tune_cache_cases(const char *path)
 
 int main(int argc, char **argv)
 {
+    if (argc == 2 && std::strcmp(argv[1], "long") == 0) {
+        long_cases();
+        std::printf("launches %d, failures %d\n", pfh::g_launches, g_fail);
+        return g_fail ? 1 : 0;
+    }
     if (argc == 3 && std::strcmp(argv[1], "tunecache") == 0) {
         tune_cache_cases(argv[2]);
         std::printf("launches %d, failures %d\n", pfh::g_launches, g_fail);

@@ -1,0 +1,174 @@
+"""GPU parity of long lines against the oracle.
+
+Every length that LocalC2cPlan::new accepts plans (crates/pencil-fft/src/
+lib.rs:422-438). A line too long for one workgroup runs as a four-step
+split into two line stages, with the twiddles applied at the first stage's
+store. A length with a prime factor above 128 runs through Bluestein's
+chirp convolution of 5-smooth length M >= 2n - 1. Real kinds promote the
+line to a complex FFT between a pre-processing kernel and a
+post-processing kernel. The lengths are 2^15, 2^16 and 2^20 (split), 1009
+and 10007 (Bluestein), and the real kinds' lengths that lead there. They
+run in local plans and in route plans, in every operation with a
+reference counterpart, against the oracle on the same seeded inputs
+(relative L2: 2e-6 f32, 1e-12 f64). At full size, the round trip also
+returns the input.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import pencil_fft_hip as pf
+from util import complex_of, rel_l2, tol, uniform_complex, uniform_real
+
+pytestmark = pytest.mark.gpu
+
+DTYPES = [np.float32, np.float64]
+
+
+def _tdev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("n", [32768, 65536, 1009, 10007, 1 << 20])
+def test_local_c2c_long_lines(gpu, n, dt):
+    rng = np.random.default_rng(n + 7)
+    cd = complex_of(dt)
+    batch = 3 if n < (1 << 20) else 1
+    x = uniform_complex(rng, (batch, n), cd)
+    plan = pf.LocalC2cPlan(n, dtype=torch.from_numpy(x).dtype)
+    xd = _tdev(x)
+    before = xd.clone()
+    for op, fn in ((O.FORWARD, plan.forward), (O.BACKWARD, plan.backward), (O.INVERSE, plan.inverse)):
+        got = _host(fn(xd))
+        ref = O.c2c_lines(x.astype(np.complex128), op)
+        err = rel_l2(got, ref)
+        print(f"c2c n={n} {np.dtype(dt).name} op={op} rel_l2={err:.3g}")
+        assert err <= tol(dt), (n, op, err)
+    assert torch.equal(xd, before), "source must be preserved"
+    back = _host(plan.inverse(plan.forward(xd)))
+    assert rel_l2(back, x) <= tol(dt)
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("n", [65536, 2018, 10007, 131072])
+def test_local_r2c_c2r_long_lines(gpu, n, dt):
+    """Even lengths whose half-length FFT is long (2^15, 2^16) or prime
+    (1009), and an odd prime length: R2C forward, C2R inverse and backward."""
+    rng = np.random.default_rng(n + 11)
+    x = uniform_real(rng, (2, n), dt)
+    plan = pf.LocalR2cPlan(n, dtype=torch.from_numpy(x).dtype)
+    spec = plan.forward(_tdev(x))
+    ref = O.r2c_lines(x.astype(np.float64))
+    err = rel_l2(_host(spec), ref)
+    print(f"r2c n={n} {np.dtype(dt).name} rel_l2={err:.3g}")
+    assert err <= tol(dt), (n, err)
+    # DC and Nyquist come out exactly real (the C2R preflight of the
+    # reference, r2c.rs:798-826, rejects any other value there)
+    hs = _host(spec).reshape(x.shape[0], -1)
+    assert np.all(hs[:, 0].imag == 0) and (n % 2 or np.all(hs[:, n // 2].imag == 0))
+    s = ref.astype(complex_of(dt))
+    s[:, 0] = s[:, 0].real
+    if n % 2 == 0:
+        s[:, n // 2] = s[:, n // 2].real
+    for op, fn in ((O.INVERSE, plan.inverse), (O.BACKWARD, plan.backward)):
+        got = _host(fn(_tdev(s)))
+        want = O.c2r_lines(s.astype(np.complex128), n, op)
+        err = rel_l2(got, want)
+        print(f"c2r n={n} {np.dtype(dt).name} op={op} rel_l2={err:.3g}")
+        assert err <= tol(dt), (n, op, err)
+    back = _host(plan.inverse(spec))
+    assert rel_l2(back, x) <= tol(dt)
+
+
+R2R_KINDS = [O.DCT1, O.DCT2, O.DCT3, O.DCT4, O.DST1, O.DST2, O.DST3, O.DST4]
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("n", [10007, 65536])
+def test_local_r2r_and_dht_long_lines(gpu, n, dt):
+    """All eight kinds and the DHT on a prime length (odd-length pieces,
+    Bluestein) and on 2^16 (split): forward against the oracle, the
+    normalized inverse back to the input, and the adjoint of the forward
+    (its edge weights) against the oracle."""
+    rng = np.random.default_rng(n + 13)
+    x = uniform_real(rng, (2, n), dt)
+    xd = _tdev(x)
+    for kind in R2R_KINDS + [O.DHT]:
+        if kind == O.DHT:
+            plan = pf.LocalDhtPlan(n, dtype=torch.from_numpy(x).dtype)
+        else:
+            plan = pf.LocalR2rPlan(n, kind, dtype=torch.from_numpy(x).dtype)
+        got = _host(plan.forward(xd))
+        ref = O.r2r_lines(x.astype(np.float64), kind, O.FORWARD)
+        err = rel_l2(got, ref)
+        print(f"r2r kind={kind} n={n} {np.dtype(dt).name} rel_l2={err:.3g}")
+        assert err <= tol(dt), (kind, n, err)
+        back = _host(plan.inverse(plan.forward(xd)))
+        assert rel_l2(back, x) <= tol(dt), (kind, n)
+        adj = _host(plan.adjoint(O.ADJ_FORWARD, xd))
+        ref = O.r2r_lines(x.astype(np.float64), kind, O.ADJ_FORWARD)
+        assert rel_l2(adj, ref) <= tol(dt), (kind, n, "adjoint")
+
+
+ROUTE_C2C = [((10007, 3, 4), 2), ((4, 3, 65536), 1), ((3, 1009, 5), 1), ((6, 32768, 2), 1)]
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("shape,extra", ROUTE_C2C)
+def test_route_c2c_long_axis(gpu, shape, extra, dt):
+    """C2cPlan with one long axis (first, middle or last memory axis):
+    forward, inverse, backward and the adjoint of the forward."""
+    rng = np.random.default_rng(sum(shape) + extra)
+    cd = complex_of(dt)
+    x = uniform_complex(rng, (extra,) + shape, cd).reshape(-1)
+    plan = pf.C2cPlan(shape, dtype=torch.from_numpy(x).dtype, extra_shape=(extra,) if extra > 1 else ())
+    y = O.transform(O.KIND_C2C, x.astype(np.complex128), O.FORWARD, shape=shape, extra=extra)
+    got = _host(plan.forward(_tdev(x)))
+    err = rel_l2(got, y)
+    print(f"route c2c {shape} {np.dtype(dt).name} rel_l2={err:.3g}")
+    assert err <= tol(dt), (shape, err)
+    ys = y.astype(cd)
+    for op in (O.INVERSE, O.BACKWARD, O.ADJ_FORWARD):
+        got = _host(plan.execute(op, _tdev(ys)))
+        ref = O.transform(O.KIND_C2C, ys.astype(np.complex128), op, shape=shape, extra=extra)
+        err = rel_l2(got, ref)
+        assert err <= tol(dt), (shape, op, err)
+
+
+ROUTE_REAL = [
+    (O.KIND_R2C, (4, 6, 2018), None),
+    (O.KIND_R2C, (10007, 4, 3), None),
+    (O.KIND_R2R, (10007, 3, 4), [O.DCT2, O.DST1, O.DCT4]),
+    (O.KIND_R2R, (6, 65536, 2), [O.DCT4, O.DHT, O.DST2]),
+]
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("kind,shape,r2r", ROUTE_REAL)
+def test_route_real_long_axis(gpu, kind, shape, r2r, dt):
+    """R2cPlan and R2rPlan with a long axis: forward, inverse and backward
+    against the oracle."""
+    rng = np.random.default_rng(sum(shape) + kind)
+    x = uniform_real(rng, shape, dt).reshape(-1)
+    if kind == O.KIND_R2C:
+        plan = pf.R2cPlan(shape, dtype=torch.from_numpy(x).dtype)
+    else:
+        plan = pf.R2rPlan(shape, r2r, dtype=torch.from_numpy(x).dtype)
+    kw = dict(shape=shape) if r2r is None else dict(shape=shape, r2r=r2r)
+    y = O.transform(kind, x.astype(np.float64), O.FORWARD, **kw)
+    got = _host(plan.forward(_tdev(x)))
+    err = rel_l2(got, y)
+    print(f"route kind={kind} {shape} {np.dtype(dt).name} rel_l2={err:.3g}")
+    assert err <= tol(dt), (shape, err)
+    ys = y.astype(got.dtype)
+    for op in (O.INVERSE, O.BACKWARD):
+        got = _host(plan.execute(op, _tdev(ys)))
+        ref = O.transform(kind, ys.astype(np.complex128 if np.iscomplexobj(ys) else np.float64), op, **kw)
+        assert rel_l2(got, ref) <= tol(dt), (shape, op)

@@ -60,3 +60,15 @@ def test_batch_tuning_cache(emu, tmp_path):
     r = subprocess.run([emu, "tunecache", str(path)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
     assert "tune cache: ok" in r.stdout
+
+
+def test_long_lines_against_naive_sums(emu):
+    """Lines split into pieces, with the arithmetic emulated in long double
+    (small thresholds: lines above 32 points split, prime factors above 7
+    through Bluestein): local C2C in every direction, R2C / C2R, the eight
+    R2R kinds and the DHT, 3D C2C routes with the long axis in each position,
+    against naive sums; then real-size lines (2^15, 2^16, 2^20, 1009, 10007)
+    for launch shapes, thread mappings and every address."""
+    r = subprocess.run([emu, "long"], capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
+    assert "long lines: ok" in r.stdout
