@@ -372,6 +372,37 @@ def r2r_dht_round_trips(torch, pf, shape=(256, 384, 240), steps: int = 20):
     return out
 
 
+def long_lines(torch, pf, reps: int = 20):
+    """Local C2C f32 forward on lines that run as pieces: 2^16 and 2^20
+    (four-step split, two line-stage launches), 10007 (Bluestein, four
+    launches of length 20250 split into 135 x 150), against 8192-point lines
+    (one launch), about 8.4M points each. GB/s counts one read and one write
+    of the array (what a one-launch stage moves)."""
+    out = []
+    for n, batch in ((8192, 1024), (65536, 128), (1 << 20, 8), (10007, 838)):
+        x = torch.randn(batch * n, dtype=torch.complex64, device="cuda")
+        y = torch.empty_like(x)
+        plan = pf.LocalC2cPlan(n, dtype=torch.complex64)
+        ws = plan.allocate_workspace()
+        plan.forward(x, y, ws)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            plan.forward(x, y, ws)
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1))
+        times.sort()
+        ms = times[len(times) // 2]
+        nbytes = 2 * x.numel() * x.element_size()
+        out.append({"n": n, "lines": batch, "ms": ms, "GBps_one_pass": nbytes / (ms * 1e-3) / 1e9,
+                    "workspace_MiB": ws.numel() / 2 ** 20})
+    return {"workload": "LocalC2cPlan f32 forward, ~8.4M points per call, median of 20 (HIP events)",
+            "cases": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,6 +415,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true", help="only run the 1 GiB copy kernel")
     ap.add_argument("--batched-only", action="store_true", help="only run the batched comparison")
     ap.add_argument("--r2r-only", action="store_true", help="only run the 256x384x240 R2R / DHT pairs")
+    ap.add_argument("--long-only", action="store_true", help="only run the long-line (split / Bluestein) timings")
     ap.add_argument("--loop-only", action="store_true",
                     help="only the warm-up and the timed loop (for a kernel-trace profile of the timed region)")
     args = ap.parse_args()
@@ -407,6 +439,9 @@ def main():
         return
     if args.r2r_only:
         print(json.dumps(r2r_dht_round_trips(torch, pf, steps=min(args.steps, 20))))
+        return
+    if args.long_only:
+        print(json.dumps(long_lines(torch, pf)))
         return
     threads = min(16, os.cpu_count() or 1)
     n = args.size
@@ -527,6 +562,7 @@ def main():
         out["batched"] = batched_vs_plain(torch, pf)
         out["f64"] = f64_round_trips(torch, pf, n, threads=threads)
         out["r2r_dht"] = r2r_dht_round_trips(torch, pf)
+        out["long_lines"] = long_lines(torch, pf)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, threads)
     else:

@@ -69,7 +69,9 @@ typedef enum pfh_status {
     PFH_ERR_INVALID_PERMUTATION = 21,     /* AxisError::{OutOfBounds, Duplicate} (axis.rs:40-66) */
     PFH_ERR_ZERO_EXTENT = 22,             /* PencilError::ZeroGlobalExtent */
     PFH_ERR_INVALID_ARGUMENT = 23,        /* null pointer / unknown enum value (no Rust analogue: types forbid it) */
-    PFH_ERR_UNSUPPORTED = 24,             /* a line length the device kernels do not cover */
+    PFH_ERR_UNSUPPORTED = 24,             /* a shape the device kernels do not cover (more than 2^31 - 1
+                                             lines in one stage, more than 5 line digits, Bluestein
+                                             lengths above 2^25, batched long lines) */
     PFH_ERR_DEVICE = 25,                  /* a HIP runtime call failed */
     PFH_ERR_INVALID_GRAPH = 26,           /* MixedError::InvalidGraph (mixed.rs:118-120) */
     /* pointwise validation (pointwise.rs:8-33, :216-230); batch = the failing
@@ -293,7 +295,13 @@ pfh_status pfh_execute_in_place(pfh_plan plan, int op, void *data, size_t capaci
  * inverse} (r2r.rs:530-621, dht.rs:133-200). The batch is
  * src_elems / (input line length); BatchCountMismatch / NonIntegralBatch /
  * BufferLengthMismatch are checked as in the reference. Local plans need no
- * workspace except C2R (its endpoint preflight status area). */
+ * workspace except C2R (its endpoint preflight status area) and plans whose
+ * lines run as pieces (lines too long for one workgroup, or whose FFT length
+ * has a prime factor above 128: a four-step split or Bluestein's chirp
+ * convolution through workspace regions; such plans run their batch in
+ * chunks of lines, so pfh_plan_workspace_bytes does not depend on the
+ * batch). Any length the reference's LocalC2cPlan::new accepts
+ * (lib.rs:422-438) plans. */
 pfh_status pfh_local_execute(pfh_plan plan, int op, const void *src, size_t src_elems, void *dst,
                              size_t dst_elems, void *workspace, size_t workspace_bytes, pfh_stream_t stream,
                              int flags);
