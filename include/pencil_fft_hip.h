@@ -390,10 +390,12 @@ pfh_status pfh_batch_execute(pfh_batch batch, int op, const void *const *srcs, v
  * resident set of workgroups looping over the tiles or takes one workgroup
  * per tile, and whether fused plane stages run: two route stages of a 3D f32
  * problem in one pass over whole planes in LDS, the planes of the inner or
- * of the outer axis pair, whichever are smaller), as chosen by the
+ * of the outer axis pair, whichever are smaller), and on how many streams
+ * the launches run: 2 runs the problems whose first pass is a plane stage on
+ * a second stream, concurrently with the others), as chosen by the
  * autotuner. */
 pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines, int *one_launch,
-                            int *persistent, int *plane);
+                            int *persistent, int *plane, int *streams);
 
 /* ---------------------------------------------------------------------- */
 /* Pointwise operations between transforms (pseudo-spectral steps).        */

@@ -377,7 +377,8 @@ static void choose_fast(Step &st, int tbytes)
         case 240: r0 = 16; r1 = 3; r2 = 5; em = 16; break;
         case 320: r0 = 16; r1 = 4; r2 = 5; em = 16; break;
         case 384: r0 = 16; r1 = 8; r2 = 3; em = 16; break;
-        case 385: // route-only plan (DST-I of length 384)
+        case 385: // real-input plan: DST-I lines of length 384 and the odd 385-point
+                  // DCT-II / DCT-III / DST-II / DST-III / DHT lines (never C2C)
             if (r2r) { r0 = 11; r1 = 7; r2 = 5; em = 22; }
             break;
         default: break;
@@ -3173,6 +3174,7 @@ struct BatchStage {
     bool plane = false;            // k_plane_batch: fused two-axis plane stages
     int tw_cap = 0, work_cap = 0, raw_cap = 0; // plane stages: LDS regions (complex values)
     bool multi = false;            // k_line_multi_batch over items of several compile-time plans
+    int chain = 0;                 // 1: problems whose first pass is a plane stage (second stream)
     bool fast = false, wide = false; // else one compile-time plan, or the generic kernel
     int fr[3] = {0, 0, 0};
     int fem = 0;
@@ -3224,6 +3226,9 @@ struct pfh_batch_s {
     // after / before the caller's work by events
     hipStream_t own_stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    // two-chain execution: chain 1 runs on side_stream between ev_fork / ev_join
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     ~pfh_batch_s() { release(); }
     void release_stages()
     {
@@ -3242,8 +3247,11 @@ struct pfh_batch_s {
         if (ev_in) (void)hipEventDestroy(ev_in);
         if (ev_out) (void)hipEventDestroy(ev_out);
         if (own_stream) (void)hipStreamDestroy(own_stream);
-        ev_in = ev_out = nullptr;
-        own_stream = nullptr;
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side_stream) (void)hipStreamDestroy(side_stream);
+        ev_in = ev_out = ev_fork = ev_join = nullptr;
+        own_stream = side_stream = nullptr;
     }
 };
 
@@ -3478,7 +3486,17 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
             }
         }
     }
+    // chain of a problem: 1 when its first pass is a plane stage and the
+    // batch runs two chains (the chains share no problem, so their launches
+    // need no ordering between them)
+    // (problem groups reuse one workspace region group after group, so they
+    // keep one chain: a second stream would let two groups overlap in it)
+    const bool two_chains = B.tuning.batch_streams > 1 && pgroups.size() == 1;
+    auto chain_of = [&](int i) {
+        return (two_chains && !B.items[i][op].empty() && B.items[i][op][0].plane) ? 1 : 0;
+    };
     for (const auto &pg : pgroups)
+    for (int chain = 0; chain < 2; ++chain)
     for (int s = 0; s < nst; ++s) {
         // group this stage's items by kernel
         struct Group {
@@ -3492,7 +3510,7 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
             const PlanImpl &P = *B.plans[i];
             const OpPlan &o = P.ops[op];
             const std::vector<BatchItem> &its = B.items[i][op];
-            if (s >= (int)its.size()) continue;
+            if (s >= (int)its.size() || chain_of(i) != chain) continue;
             if (its[s].plane) {
                 const PlaneDesc &pd = its[s].pd;
                 // Plane items go to launches by footprint: a plane that fits
@@ -3584,6 +3602,7 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
         }
         for (Group &g : groups) {
             BatchStage bs = g.bs;
+            bs.chain = chain;
             if (bs.plane) {
                 // tables and working plane, then the prefetch region (16-byte
                 // aligned) with the rest of the workgroup's LDS share
@@ -3627,27 +3646,55 @@ static pfh_status batch_upload(pfh_batch_s &B, int op, const void *const *srcs, 
     return PFH_OK;
 }
 
+// One persistent launch of a batch (a kernel class of one route stage).
+template <typename T>
+static pfh_status batch_launch_one(const BatchStage &bs, hipStream_t stream)
+{
+    const StageArgs<T> *items = reinterpret_cast<const StageArgs<T> *>(bs.d_items);
+    if (bs.plane) {
+        if constexpr (std::is_same<T, float>::value)
+            PFH_HIP_CHECK(launch_plane_batch<T>(reinterpret_cast<const PlaneArgs<T> *>(bs.d_items), bs.d_prefix,
+                                                bs.nitems, bs.total, bs.threads, bs.tw_cap, bs.work_cap, bs.raw_cap,
+                                                bs.grid, stream));
+        else
+            return set_err(PFH_ERR_UNSUPPORTED, "plane stages are f32 only");
+    } else if (bs.multi)
+        PFH_HIP_CHECK(launch_line_multi_batch<T>(items, bs.d_prefix, bs.nitems, bs.total, bs.threads, bs.lds,
+                                                 bs.grid, stream));
+    else if (bs.fast)
+        PFH_HIP_CHECK(launch_line_fast_batch<T>(items, bs.d_prefix, bs.nitems, bs.total, bs.fr[0], bs.fr[1],
+                                                bs.fr[2], bs.fem, bs.threads, bs.lds, bs.grid, stream));
+    else
+        PFH_HIP_CHECK(launch_line_batch<T>(items, bs.d_prefix, bs.nitems, bs.total, bs.wide, bs.threads, bs.lds,
+                                           bs.grid, stream));
+    return PFH_OK;
+}
+
 template <typename T>
 static pfh_status batch_launch(pfh_batch_s &B, hipStream_t stream)
 {
+    // two chains: chain 1 on the side stream, forked from and joined back to
+    // the caller's stream by events (a graph captured on the caller's stream
+    // gets the two branches)
+    bool two = false;
+    for (const BatchStage &bs : B.stages) two = two || bs.chain == 1;
+    if (two) {
+        if (!B.side_stream) {
+            PFH_HIP_CHECK(hipStreamCreateWithFlags(&B.side_stream, hipStreamNonBlocking));
+            PFH_HIP_CHECK(hipEventCreateWithFlags(&B.ev_fork, hipEventDisableTiming));
+            PFH_HIP_CHECK(hipEventCreateWithFlags(&B.ev_join, hipEventDisableTiming));
+        }
+        PFH_HIP_CHECK(hipEventRecord(B.ev_fork, stream));
+        PFH_HIP_CHECK(hipStreamWaitEvent(B.side_stream, B.ev_fork, 0));
+    }
     for (const BatchStage &bs : B.stages) {
-        const StageArgs<T> *items = reinterpret_cast<const StageArgs<T> *>(bs.d_items);
-        if (bs.plane) {
-            if constexpr (std::is_same<T, float>::value)
-                PFH_HIP_CHECK(launch_plane_batch<T>(reinterpret_cast<const PlaneArgs<T> *>(bs.d_items), bs.d_prefix,
-                                                    bs.nitems, bs.total, bs.threads, bs.tw_cap, bs.work_cap,
-                                                    bs.raw_cap, bs.grid, stream));
-            else
-                return set_err(PFH_ERR_UNSUPPORTED, "plane stages are f32 only");
-        } else if (bs.multi)
-            PFH_HIP_CHECK(launch_line_multi_batch<T>(items, bs.d_prefix, bs.nitems, bs.total, bs.threads, bs.lds,
-                                                     bs.grid, stream));
-        else if (bs.fast)
-            PFH_HIP_CHECK(launch_line_fast_batch<T>(items, bs.d_prefix, bs.nitems, bs.total, bs.fr[0], bs.fr[1],
-                                                    bs.fr[2], bs.fem, bs.threads, bs.lds, bs.grid, stream));
-        else
-            PFH_HIP_CHECK(launch_line_batch<T>(items, bs.d_prefix, bs.nitems, bs.total, bs.wide, bs.threads, bs.lds,
-                                               bs.grid, stream));
+        const hipStream_t s = bs.chain == 1 ? B.side_stream : stream;
+        pfh_status st = batch_launch_one<T>(bs, s);
+        if (st) return st;
+    }
+    if (two) {
+        PFH_HIP_CHECK(hipEventRecord(B.ev_join, B.side_stream));
+        PFH_HIP_CHECK(hipStreamWaitEvent(stream, B.ev_join, 0));
     }
     return PFH_OK;
 }
@@ -3699,9 +3746,11 @@ static pfh_status batch_run(pfh_batch_s &B, int op, const void *const *srcs, voi
 }
 
 // The tuning of one autotuner candidate.
-static Tuning batch_candidate(int block, int col_lines, bool plane, bool persistent, bool multi, int max_radix)
+static Tuning batch_candidate(int block, int col_lines, bool plane, bool persistent, bool multi, int max_radix,
+                              int streams = 1)
 {
     Tuning t;
+    t.batch_streams = streams;
     t.batch_multi = multi;
     t.batch_persistent = persistent;
     t.batch_plane = plane;
@@ -3733,7 +3782,7 @@ static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int coun
         int source = PFH_TUNED_DEFAULT;
         if (tune_cache_lookup(key, rec, &source)) {
             const Tuning t = batch_candidate(rec.block, rec.col_lines, rec.plane != 0, rec.persistent != 0,
-                                             rec.multi != 0, rec.max_radix);
+                                             rec.multi != 0, rec.max_radix, rec.streams);
             if (batch_plan(B, descs, count, t) == PFH_OK) {
                 B.tune_source = source;
                 return ok();
@@ -3750,7 +3799,11 @@ static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int coun
     for (int block : {128, 256, 512})
         for (int col : {0, 32})
             for (bool plane : {true, false})
-                for (bool pers : {true, false}) cands.push_back(batch_candidate(block, col, plane, pers, true, 8));
+                for (bool pers : {true, false})
+                    for (int streams : {1, 2}) {
+                        if (streams == 2 && !plane) continue; // only plane stages make a second chain
+                        cands.push_back(batch_candidate(block, col, plane, pers, true, 8, streams));
+                    }
     // measurement buffers: inputs hold finite nonzero values (every byte
     // 0x3c), so the timed launches move and compute what real data makes
     // them move and compute
@@ -3841,6 +3894,7 @@ static pfh_status batch_autotune(pfh_batch_s &B, const pfh_desc *descs, int coun
     rec.multi = best.batch_multi ? 1 : 0;
     rec.persistent = best.batch_persistent ? 1 : 0;
     rec.plane = best.batch_plane ? 1 : 0;
+    rec.streams = best.batch_streams;
     rec.ms = score[0].first;
     tune_cache_store(key, rec);
     B.tune_source = PFH_TUNED_MEASURED;
@@ -3890,7 +3944,7 @@ pfh_status pfh_batch_tuning_source(pfh_batch batch, int *source)
 void pfh_batch_tuning_cache_clear(void) { tune_cache_clear(); }
 
 pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *col_lines, int *one_launch,
-                            int *persistent, int *plane)
+                            int *persistent, int *plane, int *streams)
 {
     if (!batch) return set_err(PFH_ERR_INVALID_ARGUMENT, "argument");
     if (plane) {
@@ -3906,6 +3960,7 @@ pfh_status pfh_batch_tuning(pfh_batch batch, int *block, int *max_radix, int *co
     if (col_lines) *col_lines = batch->tuning.col_lines;
     if (one_launch) *one_launch = batch->tuning.batch_multi ? 1 : 0;
     if (persistent) *persistent = batch->tuning.batch_persistent ? 1 : 0;
+    if (streams) *streams = batch->tuning.batch_streams;
     return ok();
 }
 

@@ -154,6 +154,8 @@ struct Tuning {
                                   // over the tiles, else one workgroup per tile
     int batch_group_mb = 0;  // batched problems run group by group (all stages of a group before
                              // the next; groups share one workspace region); 0: one group
+    int batch_streams = 1;   // batched launches: 2 runs the problems whose first pass is a plane stage
+                             // on a second stream, concurrently with the others (no dependency between them)
     bool batch_plane = true; // batched 3D f32 problems: the two inner route stages fused into one
                              // plane stage where both lengths have a plane plan and a plane fits in LDS
 };

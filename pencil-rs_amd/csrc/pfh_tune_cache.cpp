@@ -6,7 +6,7 @@
 // LDS per workgroup), so it is kept under a key made of those: in a map for
 // the process, and as one text line per measurement in a file
 //
-//   <key> <block> <max_radix> <col_lines> <multi> <persistent> <plane> <ms>
+//   <key> <block> <max_radix> <col_lines> <multi> <persistent> <plane> <streams> <ms>
 //
 // that later processes read (the last line of a key wins; lines that do not
 // parse are skipped). The file is PFH_TUNE_CACHE when that is set (an empty
@@ -38,7 +38,7 @@ std::map<std::string, TuneRecord> g_mem;
 
 // Bumped when the meaning of a record's fields or the candidate set changes,
 // so that records written by another build are not reused.
-constexpr const char *kKeyVersion = "pfhbt2";
+constexpr const char *kKeyVersion = "pfhbt3";
 
 std::string cache_path()
 {
@@ -60,7 +60,8 @@ bool valid(const TuneRecord &r)
     const bool radix_ok = r.max_radix >= 2 && r.max_radix <= 64;
     const bool col_ok = r.col_lines >= 0 && r.col_lines <= 256;
     auto flag = [](int v) { return v == 0 || v == 1; };
-    return block_ok && radix_ok && col_ok && flag(r.multi) && flag(r.persistent) && flag(r.plane);
+    return block_ok && radix_ok && col_ok && flag(r.multi) && flag(r.persistent) && flag(r.plane) &&
+           (r.streams == 1 || r.streams == 2);
 }
 
 // Parses one line of the file; false for anything that is not a record.
@@ -72,9 +73,9 @@ bool parse_line(const std::string &line, std::string &key, TuneRecord &r)
     if (key.compare(0, std::strlen(kKeyVersion), kKeyVersion) != 0) return false;
     double ms = 0;
     int used = 0;
-    const int n = std::sscanf(line.c_str() + sp, " %d %d %d %d %d %d %lf%n", &r.block, &r.max_radix, &r.col_lines,
-                              &r.multi, &r.persistent, &r.plane, &ms, &used);
-    if (n != 7) return false;
+    const int n = std::sscanf(line.c_str() + sp, " %d %d %d %d %d %d %d %lf%n", &r.block, &r.max_radix,
+                              &r.col_lines, &r.multi, &r.persistent, &r.plane, &r.streams, &ms, &used);
+    if (n != 8) return false;
     for (const char *p = line.c_str() + sp + used; *p; ++p)
         if (*p != ' ' && *p != '\r') return false;
     r.ms = ms;
@@ -118,8 +119,8 @@ void file_append(const std::string &path, const std::string &key, const TuneReco
     const int fd = ::open(path.c_str(), O_WRONLY | O_CREAT | O_APPEND | O_CLOEXEC, 0644);
     if (fd < 0) return;
     char tail[160];
-    std::snprintf(tail, sizeof tail, " %d %d %d %d %d %d %.6f\n", r.block, r.max_radix, r.col_lines, r.multi,
-                  r.persistent, r.plane, r.ms);
+    std::snprintf(tail, sizeof tail, " %d %d %d %d %d %d %d %.6f\n", r.block, r.max_radix, r.col_lines, r.multi,
+                  r.persistent, r.plane, r.streams, r.ms);
     const std::string line = key + tail;
     // one write per record under an exclusive lock: records of processes
     // that tune at the same time do not interleave

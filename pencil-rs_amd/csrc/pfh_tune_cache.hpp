@@ -16,6 +16,7 @@ struct TuneRecord {
     int multi = 1;
     int persistent = 1;
     int plane = 1;
+    int streams = 1;
     double ms = 0; // the measured forward time (informational)
 };
 

@@ -225,7 +225,7 @@ def load_library(path: str = LIB_PATH):
     lib.pfh_batch_workspace_bytes.argtypes = [vp, ctypes.POINTER(sz)]
     lib.pfh_batch_execute.argtypes = [vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), vp, sz, vp, i]
     ip = ctypes.POINTER(i)
-    lib.pfh_batch_tuning.argtypes = [vp, ip, ip, ip, ip, ip, ip]
+    lib.pfh_batch_tuning.argtypes = [vp, ip, ip, ip, ip, ip, ip, ip]
     lib.pfh_batch_tuning_source.argtypes = [vp, ip]
     lib.pfh_batch_tuning_cache_clear.argtypes = []
     lib.pfh_batch_tuning_cache_clear.restype = None
@@ -806,11 +806,12 @@ class BatchC2cPlan:
     def tuning(self):
         """(threads per block target, largest radix, lines per column tile,
         one launch per stage over all compile-time plans, persistent
-        workgroups rather than one workgroup per tile, fused plane stages)."""
-        b, r, c, m, p, q = (ctypes.c_int() for _ in range(6))
+        workgroups rather than one workgroup per tile, fused plane stages,
+        streams the launches run on)."""
+        b, r, c, m, p, q, k = (ctypes.c_int() for _ in range(7))
         _check(_lib.pfh_batch_tuning(self._h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(c), ctypes.byref(m),
-                                     ctypes.byref(p), ctypes.byref(q)))
-        return b.value, r.value, c.value, bool(m.value), bool(p.value), bool(q.value)
+                                     ctypes.byref(p), ctypes.byref(q), ctypes.byref(k)))
+        return b.value, r.value, c.value, bool(m.value), bool(p.value), bool(q.value), k.value
 
     @property
     def tuning_source(self) -> str:

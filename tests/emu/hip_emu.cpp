@@ -724,6 +724,13 @@ static void emulate_batch(const char *what, const StageArgs<T> *items, const int
     check_read("batch items", items, sizeof(StageArgs<T>) * nitems);
     check_read("batch prefix", prefix, sizeof(int64_t) * (nitems + 1));
     if (prefix[0] != 0 || prefix[nitems] != total) fail("%s: prefix ends", what);
+    if (g_trace) {
+        double bytes = 0;
+        for (int q = 0; q < nitems; ++q)
+            bytes += (double)items[q].nlines * (items[q].nin + items[q].nout) * 2.0 * sizeof(T);
+        std::printf("  %s items %d tiles %" PRId64 " threads %d grid %" PRId64 " lds %zu bytes %.0f\n", what, nitems,
+                    total, threads, grid, lds_bytes, bytes);
+    }
     for (int q = 0; q < nitems; ++q) {
         const StageArgs<T> &a = items[q];
         if (prefix[q + 1] - prefix[q] != a.ntiles) fail("%s: item %d tiles", what, q);
@@ -783,6 +790,13 @@ hipError_t launch_line_multi_batch(const StageArgs<T> *items, const int64_t *pre
     if (threads > 512) fail("line_multi_batch: %d threads", threads);
     if (grid <= 0 || grid > total) fail("line_multi_batch: grid");
     if (prefix[0] != 0 || prefix[nitems] != total) fail("line_multi_batch: prefix ends");
+    if (g_trace) {
+        double bytes = 0;
+        for (int q = 0; q < nitems; ++q)
+            bytes += (double)items[q].nlines * (items[q].nin + items[q].nout) * 2.0 * sizeof(T);
+        std::printf("  line_multi_batch items %d tiles %" PRId64 " threads %d grid %" PRId64 " lds %zu bytes %.0f\n",
+                    nitems, total, threads, grid, lds_bytes, bytes);
+    }
     for (int q = 0; q < nitems; ++q) {
         const StageArgs<T> &a = items[q];
         if (prefix[q + 1] - prefix[q] != a.ntiles) fail("line_multi_batch: item %d tiles", q);
@@ -857,6 +871,12 @@ hipError_t launch_plane_batch(const PlaneArgs<T> *items, const int64_t *prefix, 
     if (grid <= 0 || grid > total) fail("plane_batch: grid");
     if (prefix[0] != 0 || prefix[nitems] != total)
         fail("plane_batch: prefix ends %lld %lld %lld (nitems %d)", (long long)prefix[0], (long long)prefix[nitems], (long long)total, nitems);
+    if (g_trace) {
+        double bytes = 0;
+        for (int q = 0; q < nitems; ++q) bytes += (double)items[q].nplanes * items[q].nr * items[q].nc * 4.0 * sizeof(T);
+        std::printf("  plane_batch items %d planes %" PRId64 " threads %d grid %" PRId64 " lds %zu bytes %.0f\n", nitems,
+                    total, threads, grid, lds_bytes, bytes);
+    }
     for (int q = 0; q < nitems; ++q) {
         const PlaneArgs<T> &a = items[q];
         if (prefix[q + 1] - prefix[q] != a.nplanes) fail("plane_batch: item %d planes", q);
@@ -2020,7 +2040,7 @@ void tune_cache_cases(const char *path)
     setenv("PFH_TUNE_CACHE", path, 1);
     std::remove(path);
     struct Got {
-        int src = -1, t[6] = {};
+        int src = -1, t[7] = {};
     };
     auto create = [&](const std::vector<pfh_desc> &d, int tune) {
         Got g;
@@ -2030,12 +2050,12 @@ void tune_cache_cases(const char *path)
             return g;
         }
         pfh_batch_tuning_source(b, &g.src);
-        pfh_batch_tuning(b, &g.t[0], &g.t[1], &g.t[2], &g.t[3], &g.t[4], &g.t[5]);
+        pfh_batch_tuning(b, &g.t[0], &g.t[1], &g.t[2], &g.t[3], &g.t[4], &g.t[5], &g.t[6]);
         pfh_batch_destroy(b);
         return g;
     };
     auto same = [](const Got &a, const Got &b) {
-        for (int i = 0; i < 6; ++i)
+        for (int i = 0; i < 7; ++i)
             if (a.t[i] != b.t[i]) return false;
         return true;
     };
@@ -2077,7 +2097,7 @@ void tune_cache_cases(const char *path)
             std::fprintf(fp, "%s 300 8 0 1 1 1 2.0\n", key.c_str());      // not a power of two
             std::fprintf(fp, "%s 256 8 0 1 1 1 2.0 junk\n", key.c_str()); // trailing text
             std::fprintf(fp, "garbage line\n");
-            std::fprintf(fp, "%s %d 8 32 1 0 0 2.5\n", key.c_str(), alt_block);
+            std::fprintf(fp, "%s %d 8 32 1 0 0 1 2.5\n", key.c_str(), alt_block);
             std::fclose(fp);
             pfh_batch_tuning_cache_clear();
             Got g = create(ds, 1);

@@ -50,9 +50,10 @@ def test_small_batch_matches_oracle(dt, autotune):
     torch_dt = torch.complex64 if dt == np.complex64 else torch.complex128
     plan = pf.BatchC2cPlan(shapes, dtype=torch_dt, autotune=autotune, extra_shapes=[(3,)] * len(shapes))
     if autotune:
-        block, maxr, col, one, persistent, plane = plan.tuning()
+        block, maxr, col, one, persistent, plane, streams = plan.tuning()
         assert block in (128, 256, 512) and maxr in (8, 16) and col in (0, 32) and one in (True, False)
         assert persistent in (True, False) and not plane  # no length here has a plane plan
+        assert streams in (1, 2)
     srcs = [_tdev(x) for x in xs]
     for op in (O.FORWARD, O.INVERSE, O.BACKWARD):
         for use_graph in (False, True):
@@ -82,7 +83,7 @@ def test_tuning_cache_reuses_the_measured_tuning(tmp_path, monkeypatch):
         c = pf.BatchC2cPlan(shapes, autotune=True, extra_shapes=ex)
         assert c.tuning_source == "file" and c.tuning() == a.tuning()
         lines = path.read_text().splitlines()
-        assert len(lines) == 1 and lines[0].startswith("pfhbt2;arch=gfx950"), lines
+        assert len(lines) == 1 and lines[0].startswith("pfhbt3;arch=gfx950"), lines
         assert pf.BatchC2cPlan(shapes, autotune=False, extra_shapes=ex).tuning_source == "default"
         rng = np.random.default_rng(5)
         xs = _inputs(rng, shapes, 3, np.complex64)
@@ -90,6 +91,42 @@ def test_tuning_cache_reuses_the_measured_tuning(tmp_path, monkeypatch):
         for k, (s, x) in enumerate(zip(shapes, xs)):
             ref = O.transform(O.KIND_C2C, x.astype(np.complex128), O.FORWARD, shape=s, extra=3)
             assert rel_l2(_host(outs[k]), ref) <= tol(np.complex64), (k, s)
+    finally:
+        pf.clear_tuning_cache()
+
+
+@pytest.mark.parametrize("persistent", [1, 0])
+def test_two_streams_match_oracle(tmp_path, monkeypatch, persistent):
+    """Problems whose first pass is a plane stage run on a second stream,
+    concurrently with the others (a tuning record with two streams in the
+    cache file forces it): every operation, with and without a graph, against
+    the oracle and the per-problem plans, twice in a row."""
+    path = tmp_path / "batch_tuning.txt"
+    monkeypatch.setenv("PFH_TUNE_CACHE", str(path))
+    pf.clear_tuning_cache()
+    shapes = [(48, 60, 64), (128, 112, 20), (96, 60, 7), (160, 84, 5), (112, 96, 48), (160, 128, 48), (12, 7, 9),
+              (84, 16, 20)]
+    ex = [(3,)] * len(shapes)
+    try:
+        pf.BatchC2cPlan(shapes, autotune=True, extra_shapes=ex)
+        key = path.read_text().splitlines()[0].split(" ")[0]
+        with open(path, "a") as f:
+            f.write(f"{key} 256 8 0 1 {persistent} 1 2 0.0\n")
+        pf.clear_tuning_cache()
+        plan = pf.BatchC2cPlan(shapes, autotune=True, extra_shapes=ex)
+        assert plan.tuning_source == "file"
+        t = plan.tuning()
+        assert t[5] and t[6] == 2 and t[4] == bool(persistent), t
+        rng = np.random.default_rng(90 + persistent)
+        xs = _inputs(rng, shapes, 3, np.complex64)
+        srcs = [_tdev(x) for x in xs]
+        for op in (O.FORWARD, O.INVERSE, O.BACKWARD):
+            for use_graph in (False, True, True):
+                outs = plan.execute(op, srcs, use_graph=use_graph)
+                for k, (s, x) in enumerate(zip(shapes, xs)):
+                    got = _host(outs[k])
+                    ref = O.transform(O.KIND_C2C, x.astype(np.complex128), op, shape=s, extra=3)
+                    assert rel_l2(got, ref) <= tol(np.complex64), (op, use_graph, k, s)
     finally:
         pf.clear_tuning_cache()
 
