@@ -1734,6 +1734,14 @@ static void contract_cases()
     pfh_plan_destroy(A);
     pfh_plan_destroy(B);
     pfh_plan_destroy(R);
+    // batches refuse problems whose lines run as pieces, before any device work
+    {
+        std::vector<pfh_desc> bd = {route_desc(PFH_C2C, PFH_F32, {12, 10, 8}, {}),
+                                    route_desc(PFH_C2C, PFH_F32, {4, 3, 1009}, {})};
+        pfh_batch bt = nullptr;
+        expect("batch with a long line", pfh_batch_create(bd.data(), 2, 0, &bt), PFH_ERR_UNSUPPORTED);
+        if (bt) pfh_batch_destroy(bt);
+    }
     std::printf("contract: %s\n", g_fail == before_fail ? "ok" : "FAILED");
 }
 

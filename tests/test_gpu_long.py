@@ -172,3 +172,36 @@ def test_route_real_long_axis(gpu, kind, shape, r2r, dt):
         got = _host(plan.execute(op, _tdev(ys)))
         ref = O.transform(kind, ys.astype(np.complex128 if np.iscomplexobj(ys) else np.float64), op, **kw)
         assert rel_l2(got, ref) <= tol(dt), (shape, op)
+
+
+def test_in_place_long_lines(gpu):
+    """In-place operations whose stages run as pieces (the piece regions
+    follow the in-place buffer assignment) give the out-of-place results:
+    a C2cPlan with a 2^16 axis, LocalC2cPlan at 2^16 and 10007, and the
+    packed LocalR2cPlan at 2018."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(31)
+    plan = pf.C2cPlan((4, 3, 65536), dtype=torch.complex64)
+    x = torch.randn(plan.input_elements(), generator=g, device="cuda", dtype=torch.complex64)
+    want = _host(plan.forward(x))
+    data = x.clone()
+    got = _host(plan.forward_in_place(data))
+    assert rel_l2(got, want) <= tol(np.float32)
+    for n in (65536, 10007):
+        lp = pf.LocalC2cPlan(n, dtype=torch.complex64)
+        x = torch.randn(3 * n, generator=g, device="cuda", dtype=torch.complex64)
+        want = _host(lp.forward(x))
+        got = _host(lp.forward_in_place(x.clone()))
+        assert rel_l2(got, want) <= tol(np.float32), n
+    rp = pf.LocalR2cPlan(2018, dtype=torch.float32)
+    x = torch.randn(2 * 2018, generator=g, device="cuda", dtype=torch.float32)
+    want = _host(rp.forward(x))
+    # packed in-place storage (r2c.rs:532-623): the reals in, 2 x 1010
+    # complex values out, through the workspace
+    buf = torch.zeros(2 * 2 * 1010, device="cuda", dtype=torch.float32)
+    buf[: x.numel()] = x
+    # the workspace holds the piece regions, then a staged copy of the input
+    ws = torch.zeros(max(rp.workspace_bytes(op) for op in range(6)) + 2 * 2018 * 4 + 256, dtype=torch.uint8,
+                     device="cuda")
+    got = _host(rp._run_in_place(pf.FORWARD, buf, x.numel(), ws)).view(np.complex64)
+    assert rel_l2(got, want) <= tol(np.float32)

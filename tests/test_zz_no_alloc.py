@@ -37,6 +37,9 @@ CASES = [
                         dtype=torch.float64), torch.float64),
     (lambda: pf.MixedR2cPlan((6, 5, 8), [pf.R2rKind.DctII, pf.AxisTransform.FFT, pf.AxisTransform.RFFT],
                              dtype=torch.float32), torch.float32),
+    # a long axis (1009 points, prime): its stage runs as Bluestein pieces
+    # through workspace regions, here inside the static scratch
+    (lambda: pf.C2cPlan((3, 1009, 5), dtype=torch.complex64), torch.complex64),
 ]
 
 
