@@ -655,6 +655,27 @@ static pfh_status step_geometry(Step &st)
     return PFH_OK;
 }
 
+// Threads per line of a step's radix plan: every pass's butterflies (and
+// the pre-processing) covered, rounded up to a power of two.
+static int threads_per_line(const Step &st, int tbytes)
+{
+    bool wide_plan = false;
+    for (int R : st.radices) wide_plan = wide_plan || is_wide_radix(R);
+    const int emax = st.fast ? st.fem : kEmax;
+    int need = 1;
+    for (int R : st.radices) {
+        if (is_generic_radix(R)) {
+            need = std::max(need, ceil_div(st.m, emax));
+            continue;
+        }
+        const int MB = st.fast ? std::max(1, emax / R) : generic_mb(R, wide_plan, tbytes == 4);
+        need = std::max(need, ceil_div(st.m / R, MB));
+    }
+    if (mode_needs_pre(st.mode))
+        need = std::max(need, ceil_div(st.m, st.fast ? emax : generic_pre_e(wide_plan, tbytes == 4)));
+    return pow2ceil(need);
+}
+
 // Launch shape (tile, threads per line, LDS) of a step whose line grid is set.
 static pfh_status size_step(Step &st, const DeviceInfo &dev, int tbytes, const Tuning &tu)
 {
@@ -672,20 +693,8 @@ static pfh_status size_step(Step &st, const DeviceInfo &dev, int tbytes, const T
     if (tu.col_lines > 0) lt_col = tu.col_lines;
     bool wide_plan = false;
     for (int R : st.radices) wide_plan = wide_plan || is_wide_radix(R);
-    const int emax = st.fast ? st.fem : kEmax;
     const int max_threads = st.fast ? 512 : (wide_plan ? kMaxThreadsWide : kMaxThreads);
-    int need = 1;
-    for (int R : st.radices) {
-        if (is_generic_radix(R)) {
-            need = std::max(need, ceil_div(st.m, emax));
-            continue;
-        }
-        const int MB = st.fast ? std::max(1, emax / R) : generic_mb(R, wide_plan, tbytes == 4);
-        need = std::max(need, ceil_div(st.m / R, MB));
-    }
-    if (mode_needs_pre(st.mode))
-        need = std::max(need, ceil_div(st.m, st.fast ? emax : generic_pre_e(wide_plan, tbytes == 4)));
-    int tpl = pow2ceil(need);
+    int tpl = threads_per_line(st, tbytes);
     static const int block_env = env_int("PFH_BLOCK", 256);
     const int target = tu.block > 0 ? tu.block : block_env;
     int lt = (st.in_col || st.out_col) ? lt_col : std::max(1, target / tpl);
@@ -1511,11 +1520,32 @@ static pfh_status route_specs(const PlanImpl &P, int op_in, std::vector<StageSpe
 // most slab-pipelined pairs (pair_score), then the fewest workspace arrays.
 template <typename Fn>
 static bool assign_route(const std::vector<StageSpec> &specs, bool in_place, int64_t dst_capacity, Fn &&pair_score,
-                         std::vector<Layout> &X, std::vector<Layout> &Y, std::vector<int> &inb, std::vector<int> &outb,
-                         int64_t wbytes[2])
+                         const std::vector<int> &line_threads, std::vector<Layout> &X, std::vector<Layout> &Y,
+                         std::vector<int> &inb, std::vector<int> &outb, int64_t wbytes[2])
 {
     const int K = (int)specs.size();
     int best_w = 99, best_pairs = -1;
+    int64_t best_cost = INT64_MAX;
+    // Column cost of a layout choice: a stage that reads or writes a column
+    // side (its axis not the fastest in memory) moves runs of one element
+    // per line of its tile, and a tile holds at most 512 / (threads per line)
+    // lines (the compile-time kernels' bound). Runs shorter than a 128-byte
+    // cache line cost their shortfall: the 385-point DST-I lines (32
+    // threads, 16-line tiles) write 64-byte runs of reals, so their stage
+    // should read and write rows and leave the transposes to its neighbours.
+    auto column_cost = [&](const std::vector<Layout> &x, const std::vector<Layout> &y) {
+        int64_t c = 0;
+        for (int e = 0; e < K; ++e) {
+            const int a = specs[e].axis;
+            const int N = x[e].N;
+            if (N == 0 || a < 0 || x[e].shape[a] <= 1) continue;
+            const int t = e < (int)line_threads.size() ? std::max(1, line_threads[e]) : 1;
+            const int64_t lines = std::max(1, 512 / t);
+            if (x[e].perm[N - 1] != a) c += std::max<int64_t>(0, 128 - lines * x[e].elem);
+            if (y[e].perm[N - 1] != a) c += std::max<int64_t>(0, 128 - lines * y[e].elem);
+        }
+        return c;
+    };
     std::vector<Layout> bX, bY;
     std::vector<int> bin, bout;
     int64_t bw[2] = {0, 0};
@@ -1584,9 +1614,12 @@ static bool assign_route(const std::vector<StageSpec> &specs, bool in_place, int
         }
         const int nw = (wb[0] > 0 ? 1 : 0) + (wb[1] > 0 ? 1 : 0);
         const int pairs = pair_score(x, y);
-        if (pairs > best_pairs || (pairs == best_pairs && nw < best_w)) {
+        const int64_t cost = column_cost(x, y);
+        if (pairs > best_pairs || (pairs == best_pairs && nw < best_w) ||
+            (pairs == best_pairs && nw == best_w && cost < best_cost)) {
             best_pairs = pairs;
             best_w = nw;
+            best_cost = cost;
             bX = x;
             bY = y;
             bin = ib;
@@ -1702,7 +1735,19 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
     for (int e = 0; e < K; ++e)
         if (checked[e] && e_chk < 0) e_chk = e;
     if (e_chk >= 0 && !in_place) dst_cap = 0;
-    if (!assign_route(specs, in_place, dst_cap, pair_score, X, Y, ib, ob, wb))
+    // threads per line of every stage (its radix plan, as the stage will be
+    // planned), for the route's column-side choice
+    std::vector<int> line_threads(K, 1);
+    for (int e = 0; e < K; ++e) {
+        std::vector<double> scratch;
+        Step probe;
+        const pfh_error_info saved = g_err;
+        if (fill_transform(probe, specs[e], scratch, tb, P.tuning) == PFH_OK && !probe.long_only &&
+            !wants_pieces(probe))
+            line_threads[e] = threads_per_line(probe, tb);
+        g_err = saved;
+    }
+    if (!assign_route(specs, in_place, dst_cap, pair_score, line_threads, X, Y, ib, ob, wb))
         return set_err(PFH_ERR_PREPARATION_FAILED, "no buffer assignment");
     out.steps.clear();
     for (int r = 0; r < 3; ++r) out.x_bytes[r] = 0;
