@@ -1908,13 +1908,14 @@ void local_long_case(int kind, int r2r, int64_t n, int op, int64_t batch, double
 
 // A 3D C2C route with a long axis against the naive separable transform.
 template <typename T>
-void route_long_case(std::vector<size_t> shape, size_t extra, int op, double tol)
+void route_long_case(std::vector<size_t> shape, size_t extra, int op, double tol, bool permute = true)
 {
-    char name[96];
-    std::snprintf(name, sizeof name, "long route c2c %zux%zux%zu extra %zu op %d %s", shape[0], shape[1], shape[2],
-                  extra, op, sizeof(T) == 4 ? "f32" : "f64");
+    char name[112];
+    std::snprintf(name, sizeof name, "long route c2c %zux%zux%zu extra %zu op %d %s%s", shape[0], shape[1], shape[2],
+                  extra, op, sizeof(T) == 4 ? "f32" : "f64", permute ? "" : " strided");
     pfh_desc d = route_desc(PFH_C2C, sizeof(T) == 4 ? PFH_F32 : PFH_F64, shape,
                             extra > 1 ? std::vector<size_t>{extra} : std::vector<size_t>{});
+    d.permute_dims = permute ? 1 : 0;
     pfh_plan p = nullptr;
     if (pfh_plan_create(&d, &p) != PFH_OK) {
         pfh_error_info e;
@@ -2011,6 +2012,11 @@ void long_cases()
         route_long_case<double>({4, 3, 41}, 1, op, 1e-12);
     }
     route_long_case<float>({6, 37, 5}, 1, PFH_FORWARD, 2e-6);
+    // permute_dims = false: every stage's lines are strided in place
+    for (int op : {PFH_FORWARD, PFH_INVERSE}) {
+        route_long_case<double>({37, 6, 4}, 2, op, 1e-12, false);
+        route_long_case<double>({5, 48, 3}, 1, op, 1e-12, false);
+    }
     unsetenv("PFH_LONG_LINE_MAX");
     unsetenv("PFH_BLUESTEIN_PRIME");
     // real-size lines with the default thresholds: launch shapes, thread

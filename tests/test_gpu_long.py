@@ -205,3 +205,25 @@ def test_in_place_long_lines(gpu):
                      device="cuda")
     got = _host(rp._run_in_place(pf.FORWARD, buf, x.numel(), ws)).view(np.complex64)
     assert rel_l2(got, want) <= tol(np.float32)
+
+
+def test_long_axis_with_strided_lines(gpu):
+    """permute_dims = false (lines strided in place, no transposes): a
+    C2cPlan with a 2^16 axis and an R2rPlan with a 10007-point axis, forward
+    and inverse against the oracle."""
+    rng = np.random.default_rng(12)
+    shape = (4, 3, 65536)
+    x = uniform_complex(rng, shape, np.complex64).reshape(-1)
+    plan = pf.C2cPlan(shape, dtype=torch.complex64, permute_dims=False)
+    y = O.transform(O.KIND_C2C, x.astype(np.complex128), O.FORWARD, shape=shape, permute_dims=False)
+    assert rel_l2(_host(plan.forward(_tdev(x))), y) <= tol(np.float32)
+    back = O.transform(O.KIND_C2C, y, O.INVERSE, shape=shape, permute_dims=False)
+    assert rel_l2(_host(plan.inverse(_tdev(y.astype(np.complex64)))), back) <= tol(np.float32)
+    shape = (10007, 3, 2)
+    kinds = [O.DST1, O.DCT2, O.DHT]
+    x = uniform_real(rng, shape, np.float64).reshape(-1)
+    plan = pf.R2rPlan(shape, kinds, dtype=torch.float64, permute_dims=False)
+    y = O.transform(O.KIND_R2R, x, O.FORWARD, shape=shape, r2r=kinds, permute_dims=False)
+    assert rel_l2(_host(plan.forward(_tdev(x))), y) <= tol(np.float64)
+    back = O.transform(O.KIND_R2R, y, O.INVERSE, shape=shape, r2r=kinds, permute_dims=False)
+    assert rel_l2(_host(plan.inverse(_tdev(y))), back) <= tol(np.float64)
