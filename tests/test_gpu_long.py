@@ -227,3 +227,41 @@ def test_long_axis_with_strided_lines(gpu):
     assert rel_l2(_host(plan.forward(_tdev(x))), y) <= tol(np.float64)
     back = O.transform(O.KIND_R2R, y, O.INVERSE, shape=shape, r2r=kinds, permute_dims=False)
     assert rel_l2(_host(plan.inverse(_tdev(y))), back) <= tol(np.float64)
+
+
+def test_mixed_plans_with_a_long_axis(gpu):
+    """MixedC2cPlan with a 10007-point DCT-II axis on complex data (real and
+    imaginary parts as real lines, Bluestein pieces), and MixedR2cPlan whose
+    RFFT axis is 2018 points (half-length 1009, prime): forward, inverse
+    (the mixed endpoint check on the long C2R stage) and backward against the
+    oracle."""
+    F, R = pf.AxisTransform.FFT, pf.AxisTransform.RFFT
+    rng = np.random.default_rng(14)
+    shape, codes = (10007, 4, 3), [2, F, 9]
+    x = uniform_complex(rng, shape, np.complex128).reshape(-1)
+    plan = pf.MixedC2cPlan(shape, codes, dtype=torch.complex128)
+    y = O.transform(O.KIND_MIXED_C2C, x, O.FORWARD, shape=shape, r2r=codes)
+    assert rel_l2(_host(plan.forward(_tdev(x))), y) <= tol(np.float64)
+    for op in (O.INVERSE, O.BACKWARD):
+        ref = O.transform(O.KIND_MIXED_C2C, y, op, shape=shape, r2r=codes)
+        assert rel_l2(_host(plan.execute(op, _tdev(y))), ref) <= tol(np.float64), op
+    shape, codes = (4, 2018, 3), [F, R, 2]
+    x = uniform_real(rng, shape, np.float32).reshape(-1)
+    plan = pf.MixedR2cPlan(shape, codes, dtype=torch.float32)
+    y = O.transform(O.KIND_MIXED_R2C, x.astype(np.float64), O.FORWARD, shape=shape, r2r=codes)
+    got = _host(plan.forward(_tdev(x)))
+    assert rel_l2(got, y) <= tol(np.float32)
+    for op in (O.INVERSE, O.BACKWARD):
+        ref = O.transform(O.KIND_MIXED_R2C, got.astype(np.complex128), op, shape=shape, r2r=codes)
+        assert rel_l2(_host(plan.execute(op, _tdev(got))), ref) <= tol(np.float32), op
+
+
+def test_timing_entries_of_a_long_stage(gpu):
+    """*_with_timing (distributed.rs:290-325) on a plan with a long axis: one
+    entry per route stage, the long stage's pieces summed into its entry."""
+    plan = pf.C2cPlan((4, 3, 65536), dtype=torch.complex64)
+    x = torch.randn(plan.input_elements(), dtype=torch.complex64, device="cuda")
+    y, t = plan.forward_with_timing(x)
+    entries = t.entries()
+    assert len(entries) == 3 and all(ms > 0 for _, ms, _ in entries), entries
+    assert rel_l2(_host(y), _host(plan.forward(x))) == 0.0
