@@ -1378,18 +1378,15 @@ hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int e
                             int64_t grid, hipStream_t s)
 {
     if (grid <= 0) return hipSuccess;
-    // real-input R2R modes (f32): their own instantiation, ordinary accesses
+    // real-input R2R modes: their own instantiation, ordinary accesses
     const bool r2r = !(a.mode == M_C2C || a.mode == M_R2C || a.mode == M_C2R || a.mode == M_DHT);
 #define PFH_FAST_CASE(TT, A, B, C, E)                                                                           \
     if constexpr (std::is_same<T, TT>::value) if (r0 == A && r1 == B && r2 == C && em == E) {                               \
-        if constexpr (std::is_same<TT, float>::value) {                                                         \
-            if (r2r) {                                                                                          \
-                hipLaunchKernelGGL((k_line_fast<T, A, B, C, E, false, 2>), dim3((unsigned)grid), dim3(threads), \
-                                   lds_bytes, s, a);                                                            \
-                return hipGetLastError();                                                                       \
-            }                                                                                                   \
+        if (r2r) {                                                                                              \
+            hipLaunchKernelGGL((k_line_fast<T, A, B, C, E, false, 2>), dim3((unsigned)grid), dim3(threads),     \
+                               lds_bytes, s, a);                                                                \
+            return hipGetLastError();                                                                           \
         }                                                                                                       \
-        if (r2r) return hipErrorInvalidValue;                                                                   \
         if (a.flags & F_NT)                                                                                     \
             hipLaunchKernelGGL((k_line_fast<T, A, B, C, E, true>), dim3((unsigned)grid), dim3(threads), lds_bytes, \
                                s, a);                                                                           \
