@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 
 #include "pfh_kernels.hpp"
@@ -137,12 +138,13 @@ __device__ __forceinline__ cpx<T> hin(const StageArgs<T> &a, const cpx<T> *ln, i
     return v;
 }
 
-// FFT input value j of a real-input R2R mode, raw values read through rd.
-template <typename T, typename Rd>
+// FFT input value j of a real-input R2R mode, raw values read through rd
+// (MODE >= 0: the stage mode as a compile-time constant).
+template <typename T, typename Rd, int MODE = -1>
 __device__ __forceinline__ cpx<T> pre_real(const StageArgs<T> &a, const Rd &rd, const cpx<T> *tws, int j)
 {
     const int n = a.n;
-    switch (a.mode) {
+    switch (MODE >= 0 ? MODE : a.mode) {
     case M_DHT_ODD:
         return mk(xin(a, rd, j), (T)0);
     case M_DCT2: {
@@ -249,8 +251,8 @@ __device__ __forceinline__ T t_real_view(const cpx<T> *Z, int i, bool conj_resul
     return (i & 1) ? v.y : v.x;
 }
 
-// Real output y_k of a real-output mode.
-template <typename T, int MG>
+// Real output y_k of a real-output mode (MODE >= 0: compile-time mode).
+template <typename T, int MG, int MODE = -1>
 __device__ __forceinline__ T post_real(const StageArgs<T> &a, const cpx<T> *Z, const cpx<T> *tws, int k)
 {
     const int n = a.n, m = a.m;
@@ -260,7 +262,7 @@ __device__ __forceinline__ T post_real(const StageArgs<T> &a, const cpx<T> *Z, c
         if (a.mode == M_C2R) y = t_real_view(Z, kk, true);
         else y = L(Z, kk).x; // M_C2R_ODD
     } else {
-    switch (a.mode) {
+    switch (MODE >= 0 ? MODE : a.mode) {
     case M_DCT2: {
         cpx<T> V = kk <= m ? split(Z, tws + a.post0, m, kk) : conjg(split(Z, tws + a.post0, m, n - kk));
         y = (T)2 * cmul(tws[a.post1 + kk], V).x;
@@ -566,7 +568,7 @@ __device__ __forceinline__ void stage_load(const StageArgs<T> &a, cpx<T> *data, 
 
 // ---- post-processing + store: LDS -> global (output layout)
 // Line lo of the tile, points k0, k0 + tpl, ... : post-processing, LDS -> global.
-template <typename T, int MG, bool NT>
+template <typename T, int MG, bool NT, int MODE = -1>
 __device__ __forceinline__ void store_line(const StageArgs<T> &a, const cpx<T> *data, const cpx<T> *tws,
                                            int64_t tile, int lo, int k0, int tpl)
 {
@@ -653,7 +655,7 @@ __device__ __forceinline__ void store_line(const StageArgs<T> &a, const cpx<T> *
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         const int k = base + u * tpl;
-                        if (k < a.nout) v[u] = post_real<T, MG>(a, Z, tws, k);
+                        if (k < a.nout) v[u] = post_real<T, MG, MODE>(a, Z, tws, k);
                     }
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
@@ -913,7 +915,7 @@ __device__ __forceinline__ void fast_first(const StageArgs<T> &a, cpx<T> *ln, bo
 // LDS (reordering, odd / even extension, pre-twiddles), then, after every
 // thread of the tile has read its values, the radix-R0 butterflies write the
 // pass output over the raw line.
-template <typename T, int R, int EM, int M>
+template <typename T, int R, int EM, int M, int MODE = -1>
 __device__ __forceinline__ void fast_first_pre_lds(const StageArgs<T> &a, cpx<T> *ln, const cpx<T> *tws, int tl,
                                                    int tpl)
 {
@@ -926,7 +928,7 @@ __device__ __forceinline__ void fast_first_pre_lds(const StageArgs<T> &a, cpx<T>
         const int j = tl + tpl * b;
         if (j < NB) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = pre_real<T>(a, rd, tws, j + r * NB);
+            for (int r = 0; r < R; ++r) v[b][r] = pre_real<T, LdsReal<T>, MODE>(a, rd, tws, j + r * NB);
         }
     }
     __syncthreads();
@@ -1215,7 +1217,7 @@ __device__ __forceinline__ void fast_dht_store(const StageArgs<T> &a, const cpx<
 // modes, 2 = route stages of the real-input R2R modes (their gathers and
 // post-processing live in an instantiation of their own, so the other
 // route kernels keep their smaller code).
-template <typename T, int R0, int R1, int R2, int EM, bool NT, int KIND = 1>
+template <typename T, int R0, int R1, int R2, int EM, bool NT, int KIND = 1, int MODE = -1>
 __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, const cpx<T> *tws, cpx<T> *data)
 {
     constexpr int M = R0 * R1 * (R2 ? R2 : 1);
@@ -1236,14 +1238,14 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
         // 0.8-2.0 TB/s.)
         __syncthreads(); // the pre-processing reads the staged tables
         stage_load<T, NT>(a, data, tile, tid, l, act ? tl : (1 << 28));
-        fast_first_pre_lds<T, R0, EM, M>(a, ln, tws, tl, tpl);
+        fast_first_pre_lds<T, R0, EM, M, MODE>(a, ln, tws, tl, tpl);
         if constexpr (R2 != 0) fast_mid<T, R1, EM, M>(ln, tws + a.twoff[1], R0, tl, tpl);
         fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
         if (act) {
             // the generic kernel's store (column side: consecutive threads
             // on consecutive lines)
             const int lo = a.out_col ? tid % lt : l, k0 = a.out_col ? tid / lt : tl;
-            store_line<T, 2, NT>(a, data, tws, tile, lo, k0, tpl);
+            store_line<T, 2, NT, MODE>(a, data, tws, tile, lo, k0, tpl);
         }
         return;
     } else {
@@ -1282,7 +1284,7 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
     }
 }
 
-template <typename T, int R0, int R1, int R2, int EM, bool NT, int KIND = 1>
+template <typename T, int R0, int R1, int R2, int EM, bool NT, int KIND = 1, int MODE = -1>
 __global__ __launch_bounds__(512) void k_line_fast(const StageArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1296,7 +1298,7 @@ __global__ __launch_bounds__(512) void k_line_fast(const StageArgs<T> a)
         if (tile >= a.ntiles) return;
     }
     for (int i = tid; i < a.twn; i += nthr) tws[i] = a.tw[i];
-    fast_body<T, R0, R1, R2, EM, NT, KIND>(a, tile, tws, tws + a.twn);
+    fast_body<T, R0, R1, R2, EM, NT, KIND, MODE>(a, tile, tws, tws + a.twn);
 }
 
 // Persistent batched fast stage: the work list of k_line_batch, every item
@@ -1378,8 +1380,25 @@ hipError_t launch_line_fast(const StageArgs<T> &a, int r0, int r1, int r2, int e
                             int64_t grid, hipStream_t s)
 {
     if (grid <= 0) return hipSuccess;
-    // real-input R2R modes: their own instantiation, ordinary accesses
+    // real-input R2R modes: their own instantiation, ordinary accesses; the
+    // plans and modes of PFH_R2R_MODE_PLANS also with the mode as a
+    // compile-time constant
     const bool r2r = !(a.mode == M_C2C || a.mode == M_R2C || a.mode == M_C2R || a.mode == M_DHT);
+    static const bool mode_kernels = [] {
+        const char *e = std::getenv("PFH_R2R_MODE_KERNELS");
+        return e == nullptr || std::atoi(e) != 0;
+    }();
+    if (r2r && mode_kernels) {
+#define PFH_MODE_CASE(TT, A, B, C, E, MD)                                                                     \
+        if constexpr (std::is_same<T, TT>::value)                                                             \
+            if (r0 == A && r1 == B && r2 == C && em == E && a.mode == MD) {                                   \
+                hipLaunchKernelGGL((k_line_fast<T, A, B, C, E, false, 2, MD>), dim3((unsigned)grid),          \
+                                   dim3(threads), lds_bytes, s, a);                                           \
+                return hipGetLastError();                                                                     \
+            }
+        PFH_R2R_MODE_PLANS(PFH_MODE_CASE)
+#undef PFH_MODE_CASE
+    }
 #define PFH_FAST_CASE(TT, A, B, C, E)                                                                           \
     if constexpr (std::is_same<T, TT>::value) if (r0 == A && r1 == B && r2 == C && em == E) {                               \
         if (r2r) {                                                                                              \

@@ -224,6 +224,18 @@ struct PlaneArgs {
     X(double, 8, 3, 5, 8)       \
     X(double, 8, 8, 3, 8)
 
+// Real-input R2R stages with the stage mode also compiled in: the 128- and
+// 120-point plans of the 256- and 240-point DCT-II / DCT-III / DCT-IV lines
+// and the 385-point plan of the 384-point DST-I lines.
+#define PFH_R2R_MODE_PLANS(X)         \
+    X(float, 8, 16, 0, 16, M_DCT2)    \
+    X(float, 8, 16, 0, 16, M_DCT3)    \
+    X(float, 8, 3, 5, 16, M_DCT4)     \
+    X(float, 11, 7, 5, 22, M_DST1)    \
+    X(double, 4, 4, 8, 8, M_DCT2)     \
+    X(double, 4, 4, 8, 8, M_DCT3)     \
+    X(double, 8, 3, 5, 8, M_DCT4)
+
 #define PFH_FAST_PLANS(X)      \
     X(float, 8, 8, 0, 16)      \
     X(float, 8, 16, 0, 16)     \
