@@ -201,7 +201,7 @@ __device__ __forceinline__ cpx<T> pre_real(const StageArgs<T> &a, const Rd &rd, 
     }
 }
 
-template <typename T, int MG>
+template <typename T, int MG, int MODE = -1>
 __device__ __forceinline__ cpx<T> pre_value(const StageArgs<T> &a, const cpx<T> *ln, const cpx<T> *tws, int j)
 {
     const int n = a.n;
@@ -227,7 +227,7 @@ __device__ __forceinline__ cpx<T> pre_value(const StageArgs<T> &a, const cpx<T> 
         return L(ln, j);
     }
     } else {
-        return pre_real<T>(a, LdsReal<T>{ln}, tws, j);
+        return pre_real<T, LdsReal<T>, MODE>(a, LdsReal<T>{ln}, tws, j);
     }
 }
 
@@ -671,7 +671,7 @@ __device__ __forceinline__ void store_line(const StageArgs<T> &a, const cpx<T> *
 
 // Tile store, LDS -> global: the thread mapping of stage_load on the
 // output side.
-template <typename T, int MG, bool NT>
+template <typename T, int MG, bool NT, int MODE = -1>
 __device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> *data, const cpx<T> *tws,
                                             int64_t tile, int tid, int l, int tl)
 {
@@ -681,7 +681,7 @@ __device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> 
     for (int i = 0; i < cnt; ++i) {
         int k = k0;
         asm volatile("" : "+v"(k));
-        store_line<T, MG, NT>(a, data, tws, tile, lo + i * lstep, k, step);
+        store_line<T, MG, NT, MODE>(a, data, tws, tile, lo + i * lstep, k, step);
     }
 }
 
@@ -715,7 +715,7 @@ __device__ __forceinline__ void run_pass(int R, cpx<T> *ln, const cpx<T> *twp, i
 // One tile of a generic stage: lt lines, computed lc at a time. Threads
 // beyond lc * tpl (a batched launch sizes its blocks for the largest item)
 // only take part in the barriers.
-template <typename T, bool WIDE, int MG>
+template <typename T, bool WIDE, int MG, int MODE = -1>
 __device__ __forceinline__ void stage_body(const StageArgs<T> &a, int64_t tile, const cpx<T> *tws, cpx<T> *data,
                                            const int *radix, const int *twoff, const uint32_t *nsmag,
                                            const int *nssh)
@@ -748,7 +748,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T> &a, int64_t tile, 
 #pragma unroll
             for (int i = 0; i < E; ++i) {
                 const int j = tl + tpl * i;
-                if (j < a.m) z[i] = pre_value<T, MG>(a, ln, tws, j);
+                if (j < a.m) z[i] = pre_value<T, MG, MODE>(a, ln, tws, j);
             }
             __syncthreads();
 #pragma unroll
@@ -766,10 +766,10 @@ __device__ __forceinline__ void stage_body(const StageArgs<T> &a, int64_t tile, 
             Ns *= R;
         }
     }
-    stage_store<T, MG, false>(a, data, tws, tile, tid, l, tl);
+    stage_store<T, MG, false, MODE>(a, data, tws, tile, tid, l, tl);
 }
 
-template <typename T, bool WIDE, int MG>
+template <typename T, bool WIDE, int MG, int MODE = -1>
 __global__ __launch_bounds__(WIDE ? kMaxThreadsWide : kMaxThreads) void k_line_stage(const StageArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -797,7 +797,7 @@ __global__ __launch_bounds__(WIDE ? kMaxThreadsWide : kMaxThreads) void k_line_s
             s_nssh[p] = a.nssh[p];
         }
     }
-    stage_body<T, WIDE, MG>(a, tile, tws, data, s_radix, s_twoff, s_nsmag, s_nssh);
+    stage_body<T, WIDE, MG, MODE>(a, tile, tws, data, s_radix, s_twoff, s_nsmag, s_nssh);
 }
 
 // Persistent batched stage: the blocks of one launch walk a work list of
@@ -1480,13 +1480,21 @@ hipError_t set_fast_lds_limit(size_t bytes)
     PFH_FAST_ROUTE_PLANS(PFH_FAST_ATTR_ROUTE)
 #undef PFH_FAST_ATTR_ROUTE
 #define PFH_FAST_ATTR_R2R(TT, A, B, C, E)                                                                \
-    if constexpr (std::is_same<TT, float>::value) {                                                       \
+    {                                                                                                     \
         hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_line_fast<TT, A, B, C, E, false, 2>), bytes); \
         if (e != hipSuccess) return e;                                                                    \
     }
     PFH_FAST_PLANS(PFH_FAST_ATTR_R2R)
     PFH_FAST_ROUTE_PLANS(PFH_FAST_ATTR_R2R)
 #undef PFH_FAST_ATTR_R2R
+#define PFH_FAST_ATTR_MODE(TT, A, B, C, E, MD)                                                           \
+    {                                                                                                     \
+        hipError_t e =                                                                                    \
+            raise_lds_limit(reinterpret_cast<const void *>(&k_line_fast<TT, A, B, C, E, false, 2, MD>), bytes); \
+        if (e != hipSuccess) return e;                                                                    \
+    }
+    PFH_R2R_MODE_PLANS(PFH_FAST_ATTR_MODE)
+#undef PFH_FAST_ATTR_MODE
     return hipSuccess;
 }
 
@@ -1724,6 +1732,24 @@ hipError_t launch_line_stage(const StageArgs<T> &a, bool wide, int threads, size
                            a.mode == M_C2R_ODD)
                               ? 1
                               : 2);
+    // the odd-length extension modes (DST-I / DCT-I lines, whose FFT lengths
+    // n + 1 / n - 1 rarely have a compile-time plan) with the mode compiled in
+    static const bool mode_kernels = [] {
+        const char *e = std::getenv("PFH_R2R_MODE_KERNELS");
+        return e == nullptr || std::atoi(e) != 0;
+    }();
+    if (mg == 2 && mode_kernels) {
+#define PFH_STAGE_MODE_CASE(W, MD)                                                                               \
+        if (wide == W && a.mode == MD) {                                                                         \
+            hipLaunchKernelGGL((k_line_stage<T, W, 2, MD>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, a); \
+            return hipGetLastError();                                                                           \
+        }
+        PFH_STAGE_MODE_CASE(true, M_DST1)
+        PFH_STAGE_MODE_CASE(false, M_DST1)
+        PFH_STAGE_MODE_CASE(true, M_DCT1)
+        PFH_STAGE_MODE_CASE(false, M_DCT1)
+#undef PFH_STAGE_MODE_CASE
+    }
 #define PFH_STAGE_CASE(W, G)                                                                                     \
     if (wide == W && mg == G) {                                                                                  \
         hipLaunchKernelGGL((k_line_stage<T, W, G>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, a);       \
@@ -1855,6 +1881,14 @@ hipError_t set_line_stage_lds_limit(size_t bytes)
                          reinterpret_cast<const void *>(&k_line_stage<double, false, 2>),
                          reinterpret_cast<const void *>(&k_line_stage<double, true, 1>),
                          reinterpret_cast<const void *>(&k_line_stage<double, true, 2>),
+                         reinterpret_cast<const void *>(&k_line_stage<float, true, 2, M_DST1>),
+                         reinterpret_cast<const void *>(&k_line_stage<float, false, 2, M_DST1>),
+                         reinterpret_cast<const void *>(&k_line_stage<float, true, 2, M_DCT1>),
+                         reinterpret_cast<const void *>(&k_line_stage<float, false, 2, M_DCT1>),
+                         reinterpret_cast<const void *>(&k_line_stage<double, true, 2, M_DST1>),
+                         reinterpret_cast<const void *>(&k_line_stage<double, false, 2, M_DST1>),
+                         reinterpret_cast<const void *>(&k_line_stage<double, true, 2, M_DCT1>),
+                         reinterpret_cast<const void *>(&k_line_stage<double, false, 2, M_DCT1>),
                          reinterpret_cast<const void *>(&k_line_batch<float, true>),
                          reinterpret_cast<const void *>(&k_line_batch<float, false>),
                          reinterpret_cast<const void *>(&k_line_batch<double, true>),
