@@ -206,7 +206,7 @@ __device__ __forceinline__ cpx<T> pre_value(const StageArgs<T> &a, const cpx<T> 
 {
     const int n = a.n;
     if constexpr (MG == 1) {
-    switch (a.mode) {
+    switch (MODE >= 0 ? MODE : a.mode) {
     case M_R2C_ODD:
         return mk(xin(a, LdsReal<T>{ln}, j), (T)0);
     case M_C2R: {
@@ -327,12 +327,12 @@ __device__ __forceinline__ T post_real(const StageArgs<T> &a, const cpx<T> *Z, c
     return y * a.scale;
 }
 
-// Complex output value of a complex-output mode.
-template <typename T>
+// Complex output value of a complex-output mode (MODE >= 0: compile-time mode).
+template <typename T, int MODE = -1>
 __device__ __forceinline__ cpx<T> post_complex(const StageArgs<T> &a, const cpx<T> *Z, const cpx<T> *tws, int k)
 {
     cpx<T> v;
-    switch (a.mode) {
+    switch (MODE >= 0 ? MODE : a.mode) {
     case M_R2C: {
         v = split(Z, tws + a.post0, a.m, k);
         if (k == 0 || k == a.m) v.y = (T)0; // real-input DC / Nyquist are exactly real
@@ -640,7 +640,7 @@ __device__ __forceinline__ void store_line(const StageArgs<T> &a, const cpx<T> *
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         const int k = base + u * tpl;
-                        if (k < a.nout) v[u] = post_complex(a, Z, tws, k);
+                        if (k < a.nout) v[u] = post_complex<T, MODE>(a, Z, tws, k);
                     }
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
@@ -1750,6 +1750,16 @@ hipError_t launch_line_stage(const StageArgs<T> &a, bool wide, int threads, size
         PFH_STAGE_MODE_CASE(false, M_DCT1)
 #undef PFH_STAGE_MODE_CASE
     }
+    // C2C lines (lengths without a compile-time plan, the pieces of long
+    // lines) with the mode compiled in
+    if (mg == 1 && a.mode == M_C2C && mode_kernels) {
+        if (wide)
+            hipLaunchKernelGGL((k_line_stage<T, true, 1, M_C2C>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, a);
+        else
+            hipLaunchKernelGGL((k_line_stage<T, false, 1, M_C2C>), dim3((unsigned)grid), dim3(threads), lds_bytes, s,
+                               a);
+        return hipGetLastError();
+    }
 #define PFH_STAGE_CASE(W, G)                                                                                     \
     if (wide == W && mg == G) {                                                                                  \
         hipLaunchKernelGGL((k_line_stage<T, W, G>), dim3((unsigned)grid), dim3(threads), lds_bytes, s, a);       \
@@ -1881,6 +1891,10 @@ hipError_t set_line_stage_lds_limit(size_t bytes)
                          reinterpret_cast<const void *>(&k_line_stage<double, false, 2>),
                          reinterpret_cast<const void *>(&k_line_stage<double, true, 1>),
                          reinterpret_cast<const void *>(&k_line_stage<double, true, 2>),
+                         reinterpret_cast<const void *>(&k_line_stage<float, true, 1, M_C2C>),
+                         reinterpret_cast<const void *>(&k_line_stage<float, false, 1, M_C2C>),
+                         reinterpret_cast<const void *>(&k_line_stage<double, true, 1, M_C2C>),
+                         reinterpret_cast<const void *>(&k_line_stage<double, false, 1, M_C2C>),
                          reinterpret_cast<const void *>(&k_line_stage<float, true, 2, M_DST1>),
                          reinterpret_cast<const void *>(&k_line_stage<float, false, 2, M_DST1>),
                          reinterpret_cast<const void *>(&k_line_stage<float, true, 2, M_DCT1>),
