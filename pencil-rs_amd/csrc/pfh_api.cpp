@@ -382,7 +382,7 @@ static void choose_fast(Step &st, int tbytes)
         case 384: r0 = 16; r1 = 8; r2 = 3; em = 16; break;
         case 385: // real-input plan: DST-I lines of length 384 and the odd 385-point
                   // DCT-II / DCT-III / DST-II / DST-III / DHT lines (never C2C)
-            if (r2r) { r0 = 11; r1 = 7; r2 = 5; em = 22; }
+            if (r2r) { r0 = 11; r1 = 7; r2 = 5; em = 11; }
             break;
         default: break;
         }
@@ -390,6 +390,7 @@ static void choose_fast(Step &st, int tbytes)
         // route-only f64 plans (never C2C, so never a batched line)
         if (m == 120) { r0 = 8; r1 = 3; r2 = 5; em = 8; }
         if (m == 192) { r0 = 8; r1 = 8; r2 = 3; em = 8; }
+        if (m == 385 && r2r) { r0 = 11; r1 = 7; r2 = 5; em = 11; }
     }
     if (r0 == 0 && (m < 64 || (m & (m - 1)) != 0)) return;
     if (r0 != 0) {

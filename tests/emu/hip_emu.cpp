@@ -1405,7 +1405,11 @@ void route_cases(int prec)
     const std::vector<K> fast_r2r = {{{128, 120, 63}, {}, {PFH_DCT2, PFH_DCT4, PFH_DST1}},
                                      {{120, 64, 128}, {2}, {PFH_DST2, PFH_DCT3, PFH_DST4}},
                                      {{65, 6, 128}, {}, {PFH_DCT1, 0, PFH_DCT3}},
-                                     {{128, 127, 2}, {}, {PFH_DST3, PFH_DST1, 0}}};
+                                     {{128, 127, 2}, {}, {PFH_DST3, PFH_DST1, 0}},
+                                     // 384-point DST-I lines (rows, a column axis, an odd line count)
+                                     {{384, 3, 5}, {}, {PFH_DST1, 0, PFH_DCT2}},
+                                     {{5, 384, 7}, {2}, {0, PFH_DST1, 0}},
+                                     {{3, 384}, {}, {0, PFH_DST1}}};
     for (const K &c : fast_r2r) {
         char name[128];
         std::string sh;

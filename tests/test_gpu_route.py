@@ -333,6 +333,11 @@ R2R_CASES = [
     ((6, 385, 4), 1, [pf.R2rKind.DctIII, pf.R2rKind.DstII, None]),
     ((4, 6, 385), 1, [None, pf.R2rKind.Dht, pf.R2rKind.Dht]),
     ((384, 3, 386), 1, [pf.R2rKind.DstI, None, pf.R2rKind.DctI]),
+    # 384-point DST-I lines: an odd number of lines, a middle axis, an extra
+    # batch
+    ((384, 3, 5), 1, [pf.R2rKind.DstI, pf.R2rKind.DctII, None]),
+    ((5, 384, 7), 1, [None, pf.R2rKind.DstI, None]),
+    ((3, 384), 2, [pf.R2rKind.DstII, pf.R2rKind.DstI]),
 ]
 
 
