@@ -470,8 +470,49 @@ __device__ __forceinline__ void tile_mapping(const StageArgs<T> &a, int col, int
     }
 }
 
-// Line ll of the tile, points p0, p0 + step, ... : global -> LDS.
-template <typename T, bool NT>
+// Real-input R2R modes whose FFT input is a signed permutation of the line
+// (plus zeros): DST-I (odd extension of length 2n + 2), DCT-II (even
+// points forward, odd points backward) and DCT-IV (pairs x_2j, x_n-1-2j,
+// the pre-twiddle applied by the first pass). Their compile-time-mode
+// kernels put each loaded value straight into its FFT-input slot(s).
+template <int MODE>
+constexpr bool scatter_mode()
+{
+    return MODE == M_DST1 || MODE == M_DCT2 || MODE == M_DCT4;
+}
+
+// Point p of a raw real line into the FFT input of a scatter mode, after the
+// stage's input reversal / alternating sign / edge weights (the values the
+// gathered pre-processing reads through xin).
+template <typename T, int MODE>
+__device__ __forceinline__ void scatter_real(const StageArgs<T> &a, cpx<T> *lln, int p, T v)
+{
+    const int n = a.n;
+    int q = p;
+    if (a.flags & (F_PRE_REV | F_PRE_ALT | F_WEIGHTS)) {
+        if ((a.flags & F_PRE_ALT) && (p & 1)) v = -v;
+        if (a.flags & F_WEIGHTS) {
+            if (p == 0) v = v * a.prew0;
+            if (p == n - 1) v = v * a.prew1;
+        }
+        if (a.flags & F_PRE_REV) q = n - 1 - p;
+    }
+    if constexpr (MODE == M_DST1) {
+        // o_{q+1} = x_q, o_{2n+1-q} = -x_q (o_0 = o_{n+1} = 0)
+        LRset(lln, q + 1, v);
+        LRset(lln, 2 * n + 1 - q, -v);
+    } else if constexpr (MODE == M_DCT2) {
+        // v_i = x_2i (i < n / 2), v_i = x_{2(n-1-i)+1} (i >= n / 2)
+        LRset(lln, (q & 1) ? n - 1 - (q >> 1) : (q >> 1), v);
+    } else {
+        // z_j = (x_2j, x_{n-1-2j}): real slot 2j or 2j + 1
+        LRset(lln, (q & 1) ? n - q : q, v);
+    }
+}
+
+// Line ll of the tile, points p0, p0 + step, ... : global -> LDS (the FFT
+// input of a scatter mode when MODE is one).
+template <typename T, bool NT, int MODE = -1>
 __device__ __forceinline__ void load_line(const StageArgs<T> &a, cpx<T> *data, int64_t tile, int ll, int p0, int step)
 {
     const bool cin = complex_input(a.mode);
@@ -530,6 +571,12 @@ __device__ __forceinline__ void load_line(const StageArgs<T> &a, cpx<T> *data, i
             }
         } else {
             const T *s = reinterpret_cast<const T *>(a.src) + io;
+            if constexpr (MODE == M_DST1) {
+                if (p0 == 0) {
+                    LRset(lln, 0, (T)0);
+                    LRset(lln, a.n + 1, (T)0);
+                }
+            }
             for (int base = p0; base < a.nin; base += step * (2 * Regs<T>::gunroll)) {
                 T v[2 * Regs<T>::gunroll];
 #pragma unroll
@@ -540,7 +587,10 @@ __device__ __forceinline__ void load_line(const StageArgs<T> &a, cpx<T> *data, i
 #pragma unroll
                 for (int u = 0; u < 2 * Regs<T>::gunroll; ++u) {
                     const int p = base + u * step;
-                    if (p < a.nin) LRset(lln, p, v[u]);
+                    if (p < a.nin) {
+                        if constexpr (scatter_mode<MODE>()) scatter_real<T, MODE>(a, lln, p, v[u]);
+                        else LRset(lln, p, v[u]);
+                    }
                 }
             }
         }
@@ -550,7 +600,7 @@ __device__ __forceinline__ void load_line(const StageArgs<T> &a, cpx<T> *data, i
 // Tile load: global -> LDS. lc lines are computed at a time (lc * tpl
 // threads) and the tile holds lt = lc * rounds lines. Threads beyond
 // lc * tpl only join the barrier.
-template <typename T, bool NT>
+template <typename T, bool NT, int MODE = -1>
 __device__ __forceinline__ void stage_load(const StageArgs<T> &a, cpx<T> *data, int64_t tile, int tid, int l, int tl)
 {
     int ll, p0, step, lstep, cnt;
@@ -561,7 +611,7 @@ __device__ __forceinline__ void stage_load(const StageArgs<T> &a, cpx<T> *data, 
         // hoisting per-point work out of the loop (and spilling it)
         int p = p0;
         asm volatile("" : "+v"(p));
-        load_line<T, NT>(a, data, tile, ll + i * lstep, p, step);
+        load_line<T, NT, MODE>(a, data, tile, ll + i * lstep, p, step);
     }
     __syncthreads();
 }
@@ -979,6 +1029,35 @@ __device__ __forceinline__ void fast_mid(cpx<T> *ln, const cpx<T> *twp, int Ns, 
     __syncthreads();
 }
 
+// first pass (Ns = 1) in LDS, each input value j times pre[j] (the DCT-IV
+// pre-twiddle of a scattered line)
+template <typename T, int R, int EM, int M>
+__device__ __forceinline__ void fast_first_tw(cpx<T> *ln, const cpx<T> *pre, int tl, int tpl)
+{
+    constexpr int MB = fast_mb<EM, R>();
+    constexpr int NB = M / R;
+    cpx<T> v[MB][R];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int j = tl + tpl * b;
+        if (j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = cmul(L(ln, j + r * NB), pre[j + r * NB]);
+            Dft<T, R>::run(v[b]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int j = tl + tpl * b;
+        if (j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) Lset(ln, j * R + r, v[b][r]);
+        }
+    }
+    __syncthreads();
+}
+
 // last pass (Ns = M / R) from LDS straight to global memory: complex output
 // (C2C, conjugation and scale applied) or the real pairs of a C2R line
 // (y_2k = Re conj Z_k, y_2k+1 = Im conj Z_k).
@@ -1235,10 +1314,14 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
         // pre-processed inputs from there; generic post-processing at the
         // store. (Gathering straight from global memory, 4-byte loads at
         // the reorder's strides, made the 256x384x240 f32 R2R stages run at
-        // 0.8-2.0 TB/s.)
+        // 0.8-2.0 TB/s.) The compile-time scatter modes place the values
+        // at their FFT-input slots as they load, so the first pass reads
+        // complex values without the per-value index arithmetic.
         __syncthreads(); // the pre-processing reads the staged tables
-        stage_load<T, NT>(a, data, tile, tid, l, act ? tl : (1 << 28));
-        fast_first_pre_lds<T, R0, EM, M, MODE>(a, ln, tws, tl, tpl);
+        stage_load<T, NT, MODE>(a, data, tile, tid, l, act ? tl : (1 << 28));
+        if constexpr (MODE == M_DCT4) fast_first_tw<T, R0, EM, M>(ln, tws + a.pre0, tl, tpl);
+        else if constexpr (scatter_mode<MODE>()) fast_mid<T, R0, EM, M>(ln, tws, 1, tl, tpl);
+        else fast_first_pre_lds<T, R0, EM, M, MODE>(a, ln, tws, tl, tpl);
         if constexpr (R2 != 0) fast_mid<T, R1, EM, M>(ln, tws + a.twoff[1], R0, tl, tpl);
         fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
         if (act) {
