@@ -738,6 +738,26 @@ static pfh_status size_step(Step &st, const DeviceInfo &dev, int tbytes, const T
     st.threads = lc * tpl;
     st.lds_bytes = lds_of(lt);
     set_tiles(st, tu);
+    // Real column sides of the compile-time R2R (load and store) and DHT
+    // (store) stages: lines 2i and 2i + 1 of a tile are neighbouring
+    // elements when the fastest line digit has stride 1 and an even extent
+    // and every other line stride and the point stride are even; a lane
+    // then moves both with one 2-element access (PFH_CPAIR=0: one element).
+    static const int cpair_env = env_int("PFH_CPAIR", 1);
+    st.flags &= ~(F_CPAIR_IN | F_CPAIR_OUT);
+    const bool kind2 = st.mode != M_C2C && st.mode != M_R2C && st.mode != M_C2R && st.mode != M_DHT;
+    auto pairs_ok = [&](bool in) {
+        const LineGeom &g = st.g;
+        if (g.nd < 1 || g.ext[0] % 2 != 0 || (in ? g.is[0] : g.os[0]) != 1 || (in ? st.ips : st.ops) % 2 != 0)
+            return false;
+        for (int d = 1; d < g.nd; ++d)
+            if ((in ? g.is[d] : g.os[d]) % 2 != 0) return false;
+        return st.lt % 2 == 0;
+    };
+    if (cpair_env && st.fast && !st.long_only) {
+        if (kind2 && st.in_col && !st.in_cplx && pairs_ok(true)) st.flags |= F_CPAIR_IN;
+        if ((kind2 || st.mode == M_DHT) && st.out_col && !st.out_cplx && pairs_ok(false)) st.flags |= F_CPAIR_OUT;
+    }
     // Non-temporal global accesses (the compile-time radix plans only) once
     // the stage's input and output together exceed the 256 MB last-level
     // cache (PFH_NT_MB overrides the threshold, 0 turns them off): measured
@@ -2333,6 +2353,9 @@ static StageArgs<T> fill_args(const PlanImpl &P, const Step &s, void *const bufs
     a.ops = s.ops;
     a.mode = s.mode;
     a.flags = s.flags;
+    // paired column accesses need 2-element alignment of the array bases
+    if ((a.flags & F_CPAIR_IN) && reinterpret_cast<uintptr_t>(a.src) % (2 * sizeof(T)) != 0) a.flags &= ~F_CPAIR_IN;
+    if ((a.flags & F_CPAIR_OUT) && reinterpret_cast<uintptr_t>(a.dst) % (2 * sizeof(T)) != 0) a.flags &= ~F_CPAIR_OUT;
     a.n = s.n;
     a.nin = s.nin;
     a.nout = s.nout;

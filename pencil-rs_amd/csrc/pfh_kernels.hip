@@ -451,14 +451,22 @@ __device__ __forceinline__ void radix_pass_generic(cpx<T> *ln, const cpx<T> *roo
 // lines per round. Threads beyond lc * tpl get nothing.
 template <typename T>
 __device__ __forceinline__ void tile_mapping(const StageArgs<T> &a, int col, int tid, int l, int tl, int &ll, int &p0,
-                                             int &step, int &lstep, int &cnt)
+                                             int &step, int &lstep, int &cnt, bool pair = false)
 {
     const int lt = a.lt, lc = a.lc, tpl = a.tpl;
     if (col) {
         const int rows = lc * tpl / lt;
-        ll = tid % lt;
-        p0 = tid / lt;
-        step = rows;
+        if (pair) {
+            // lines 2i and 2i + 1 per lane (paired column accesses)
+            const int hl = lt >> 1;
+            ll = 2 * (tid % hl);
+            p0 = tid / hl;
+            step = 2 * rows;
+        } else {
+            ll = tid % lt;
+            p0 = tid / lt;
+            step = rows;
+        }
         lstep = 0;
         cnt = tid < rows * lt ? 1 : 0;
     } else {
@@ -569,6 +577,40 @@ __device__ __forceinline__ void load_line(const StageArgs<T> &a, cpx<T> *data, i
                     }
                 }
             }
+        } else if (a.in_col && (a.flags & F_CPAIR_IN)) {
+            // lines ll and ll + 1: neighbouring elements, one 2-element load
+            const cpx<T> *s2 = reinterpret_cast<const cpx<T> *>(reinterpret_cast<const T *>(a.src) + io);
+            const int64_t ips2 = a.ips >> 1;
+            cpx<T> *lln1 = lln + a.ls;
+            if constexpr (MODE == M_DST1) {
+                if (p0 == 0) {
+                    LRset(lln, 0, (T)0);
+                    LRset(lln, a.n + 1, (T)0);
+                    LRset(lln1, 0, (T)0);
+                    LRset(lln1, a.n + 1, (T)0);
+                }
+            }
+            for (int base = p0; base < a.nin; base += step * Regs<T>::gunroll) {
+                cpx<T> v[Regs<T>::gunroll];
+#pragma unroll
+                for (int u = 0; u < Regs<T>::gunroll; ++u) {
+                    const int p = base + u * step;
+                    if (p < a.nin) v[u] = gld<NT>(s2 + (int64_t)p * ips2);
+                }
+#pragma unroll
+                for (int u = 0; u < Regs<T>::gunroll; ++u) {
+                    const int p = base + u * step;
+                    if (p < a.nin) {
+                        if constexpr (scatter_mode<MODE>()) {
+                            scatter_real<T, MODE>(a, lln, p, v[u].x);
+                            scatter_real<T, MODE>(a, lln1, p, v[u].y);
+                        } else {
+                            LRset(lln, p, v[u].x);
+                            LRset(lln1, p, v[u].y);
+                        }
+                    }
+                }
+            }
         } else {
             const T *s = reinterpret_cast<const T *>(a.src) + io;
             if constexpr (MODE == M_DST1) {
@@ -604,7 +646,7 @@ template <typename T, bool NT, int MODE = -1>
 __device__ __forceinline__ void stage_load(const StageArgs<T> &a, cpx<T> *data, int64_t tile, int tid, int l, int tl)
 {
     int ll, p0, step, lstep, cnt;
-    tile_mapping(a, a.in_col, tid, l, tl, ll, p0, step, lstep, cnt);
+    tile_mapping(a, a.in_col, tid, l, tl, ll, p0, step, lstep, cnt, a.in_col && (a.flags & F_CPAIR_IN));
 #pragma unroll 1
     for (int i = 0; i < cnt; ++i) {
         // an opaque copy of the first point per line keeps the compiler from
@@ -732,6 +774,36 @@ __device__ __forceinline__ void stage_store(const StageArgs<T> &a, const cpx<T> 
         int k = k0;
         asm volatile("" : "+v"(k));
         store_line<T, MG, NT, MODE>(a, data, tws, tile, lo + i * lstep, k, step);
+    }
+}
+
+// Lines lo and lo + 1 of the tile (neighbouring output elements), points
+// k0, k0 + step, ...: real post-processing of both, one 2-element store.
+template <typename T, int MG, bool NT, int MODE = -1>
+__device__ __forceinline__ void store_line_pair(const StageArgs<T> &a, const cpx<T> *data, const cpx<T> *tws,
+                                                int64_t tile, int lo, int k0, int step)
+{
+    const int64_t line = tile * a.lt + lo;
+    if (line >= a.nlines) return;
+    int64_t io, oo;
+    line_offsets(a.g, line, io, oo);
+    const cpx<T> *Z0 = data + lo * a.ls;
+    const cpx<T> *Z1 = Z0 + a.ls;
+    cpx<T> *d = reinterpret_cast<cpx<T> *>(reinterpret_cast<T *>(a.dst) + oo);
+    const int64_t ops2 = a.ops >> 1;
+    constexpr int U = Regs<T>::gunroll;
+    for (int base = k0; base < a.nout; base += step * U) {
+        cpx<T> v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = base + u * step;
+            if (k < a.nout) v[u] = mk(post_real<T, MG, MODE>(a, Z0, tws, k), post_real<T, MG, MODE>(a, Z1, tws, k));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = base + u * step;
+            if (k < a.nout) gst<NT>(d + (int64_t)k * ops2, v[u]);
+        }
     }
 }
 
@@ -1252,6 +1324,41 @@ __device__ __forceinline__ void fast_dht_store(const StageArgs<T> &a, const cpx<
                                                int64_t tile, int tid, int l, int tl)
 {
     const int lt = a.lt, tpl = a.tpl;
+    const cpx<T> *w = tws + a.post0;
+    const int m = a.m;
+    constexpr int U = Regs<T>::unroll;
+    if (a.out_col && (a.flags & F_CPAIR_OUT)) {
+        // lines lo and lo + 1: neighbouring elements, one 2-element store
+        const int hl = lt >> 1;
+        const int lo = 2 * (tid % hl), k0 = tid / hl, step = 2 * tpl;
+        const int64_t line = tile * lt + lo;
+        if (line >= a.nlines) return;
+        int64_t io, oo;
+        line_offsets(a.g, line, io, oo);
+        const cpx<T> *Z0 = data + lo * a.ls;
+        const cpx<T> *Z1 = Z0 + a.ls;
+        cpx<T> *d = reinterpret_cast<cpx<T> *>(reinterpret_cast<T *>(a.dst) + oo);
+        const int64_t ops2 = a.ops >> 1;
+        for (int base = k0; base < a.nout; base += step * U) {
+            cpx<T> v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = base + u * step;
+                if (k < a.nout) {
+                    const bool low = k <= m;
+                    const int kk = low ? k : 2 * m - k;
+                    const cpx<T> x0 = split(Z0, w, m, kk), x1 = split(Z1, w, m, kk);
+                    v[u] = scal(low ? mk(x0.x - x0.y, x1.x - x1.y) : mk(x0.x + x0.y, x1.x + x1.y), a.scale);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = base + u * step;
+                if (k < a.nout) gst<NT>(d + (int64_t)k * ops2, v[u]);
+            }
+        }
+        return;
+    }
     int lo, k0;
     if (a.out_col) {
         lo = tid % lt;
@@ -1266,9 +1373,6 @@ __device__ __forceinline__ void fast_dht_store(const StageArgs<T> &a, const cpx<
     line_offsets(a.g, line, io, oo);
     const cpx<T> *Z = data + lo * a.ls;
     T *d = reinterpret_cast<T *>(a.dst) + oo;
-    const cpx<T> *w = tws + a.post0;
-    const int m = a.m;
-    constexpr int U = Regs<T>::unroll;
     for (int base = k0; base < a.nout; base += tpl * U) {
         T v[U];
 #pragma unroll
@@ -1326,9 +1430,14 @@ __device__ __forceinline__ void fast_body(const StageArgs<T> &a, int64_t tile, c
         fast_mid<T, RL, EM, M>(ln, tws + a.twoff[R2 ? 2 : 1], M / RL, tl, tpl);
         if (act) {
             // the generic kernel's store (column side: consecutive threads
-            // on consecutive lines)
-            const int lo = a.out_col ? tid % lt : l, k0 = a.out_col ? tid / lt : tl;
-            store_line<T, 2, NT, MODE>(a, data, tws, tile, lo, k0, tpl);
+            // on consecutive lines, or on consecutive line pairs)
+            if (a.out_col && (a.flags & F_CPAIR_OUT)) {
+                const int hl = lt >> 1;
+                store_line_pair<T, 2, NT, MODE>(a, data, tws, tile, 2 * (tid % hl), tid / hl, 2 * tpl);
+            } else {
+                const int lo = a.out_col ? tid % lt : l, k0 = a.out_col ? tid / lt : tl;
+                store_line<T, 2, NT, MODE>(a, data, tws, tile, lo, k0, tpl);
+            }
         }
         return;
     } else {

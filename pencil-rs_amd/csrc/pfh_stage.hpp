@@ -88,6 +88,10 @@ enum StageFlags : int {
     F_XIN = 8192,   // points whose full index reaches xvalid_in load as zero; xtab_in factor on load
     F_XOUT = 16384, // points whose full index reaches xvalid_out are not stored; xtab_out factor on store
     F_XTW = 32768,  // twiddle exp(-2 pi i d k / xtw_m) on the store of point k
+    // real column side whose neighbouring lines are neighbouring elements:
+    // a lane moves lines 2i and 2i + 1 of a point as one 2-element access
+    F_CPAIR_IN = 65536,
+    F_CPAIR_OUT = 131072,
 };
 
 struct LineGeom {
