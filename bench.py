@@ -375,7 +375,7 @@ def r2r_dht_round_trips(torch, pf, shape=(256, 384, 240), steps: int = 20):
 def long_lines(torch, pf, reps: int = 20):
     """Local C2C f32 forward on lines that run as pieces: 2^16 and 2^20
     (four-step split, two line-stage launches), 10007 (Bluestein, four
-    launches of length 20250 split into 135 x 150), against 8192-point lines
+    launches of length 20480 split into 128 x 160), against 8192-point lines
     (one launch), about 8.4M points each. GB/s counts one read and one write
     of the array (what a one-launch stage moves)."""
     out = []

@@ -986,15 +986,42 @@ static int64_t split_factor(int64_t m, const DeviceInfo &dev, int tb, const Tuni
     return best;
 }
 
-// Length of Bluestein's cyclic convolution for an FFT of length m: the
-// smallest 5-smooth M >= 2m - 1 that runs as one piece or a split.
+// Radix passes of a generic line stage of length len.
+static int radix_passes(int64_t len, int tb, const Tuning &tu)
+{
+    std::vector<int> r;
+    const int max_r = tu.max_radix ? tu.max_radix : (tb == 4 ? 16 : 8);
+    return radix_plan((int)len, r, max_r) ? (int)r.size() : 64;
+}
+
+// Length of Bluestein's cyclic convolution for an FFT of length m: a
+// 5-smooth M >= 2m - 1 that runs as one piece or a split, the one with the
+// fewest M x (radix passes of its pieces) among those up to 25% above the
+// smallest. For m = 10007: 20480 = 128 x 160 (2 + 3 passes) rather than
+// 20250 = 135 x 150 (4 + 4 passes).
 static int64_t bluestein_length(int64_t m, const DeviceInfo &dev, int tb, const Tuning &tu)
 {
-    for (int64_t M = 2 * m - 1; M <= 4 * m && M <= ((int64_t)1 << 25); ++M) {
+    int64_t best = 0;
+    double best_cost = 0;
+    const int64_t lo = 2 * m - 1;
+    for (int64_t M = lo; M <= 4 * m && M <= ((int64_t)1 << 25); ++M) {
+        if (best && (double)M > 1.25 * (double)lo) break;
         if (!five_smooth(M)) continue;
-        if (piece_fits(M, dev, tb, tu) || split_factor(M, dev, tb, tu) > 0) return M;
+        int passes = 0;
+        if (piece_fits(M, dev, tb, tu)) {
+            passes = radix_passes(M, tb, tu);
+        } else {
+            const int64_t n1 = split_factor(M, dev, tb, tu);
+            if (n1 <= 0) continue;
+            passes = radix_passes(n1, tb, tu) + radix_passes(M / n1, tb, tu);
+        }
+        const double cost = (double)M * passes;
+        if (!best || cost < best_cost) {
+            best = M;
+            best_cost = cost;
+        }
     }
-    return 0;
+    return best;
 }
 
 // Strides of one side of the long lines: the stage's line digits (is or os
@@ -2009,7 +2036,8 @@ static pfh_status build_local_op(PlanImpl &P, int op, OpPlan &out)
         // long lines run in chunks of lines whose piece regions take about
         // 256 MiB of the workspace (at least one line): each chunk is a
         // handful of launches (838 lines of 10007 points, f32, on MI355X:
-        // 0.72 ms in chunks of 828 lines, 0.81 ms in chunks of 207)
+        // 0.72 ms in chunks of 828 lines, 0.81 ms in chunks of 207, with
+        // M = 20250)
         const int64_t one = out.x_bytes[0] + out.x_bytes[1] + out.x_bytes[2];
         const int64_t chunk = std::max<int64_t>(1, ((int64_t)256 << 20) / std::max<int64_t>(one, 1));
         s.in.extra = s.out.extra = chunk;
