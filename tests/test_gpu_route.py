@@ -338,6 +338,14 @@ R2R_CASES = [
     ((384, 3, 5), 1, [pf.R2rKind.DstI, pf.R2rKind.DctII, None]),
     ((5, 384, 7), 1, [None, pf.R2rKind.DstI, None]),
     ((3, 384), 2, [pf.R2rKind.DstII, pf.R2rKind.DstI]),
+    # the compile-time DCT-II (n = 256) and DCT-IV (n = 240) kernels, whose
+    # loads scatter each value to its FFT-input slot, under the input
+    # reversal / alternating sign of DST-II / DST-IV and the edge weights of
+    # the DCT-III / DST-III adjoints
+    ((256, 3, 4), 1, [pf.R2rKind.DstII, None, None]),
+    ((4, 240, 3), 1, [None, pf.R2rKind.DstIV, None]),
+    ((3, 4, 256), 2, [None, None, pf.R2rKind.DctIII]),
+    ((256, 2, 3), 1, [pf.R2rKind.DstIII, None, pf.R2rKind.DctIV]),
 ]
 
 
