@@ -1888,12 +1888,11 @@ static pfh_status build_op(PlanImpl &P, int op, bool in_place, OpPlan &out)
             ep.planes = (real_len % 2 == 0) ? 2 : 1;
             {
                 // the plane elements are one per cache line when the reduction
-                // axis is the fastest (the C2R input): many small chunks keep
-                // enough loads in flight
-                // (the partial kernel holds two elements per thread: chunks
-                // of at most 512 elements for its 256 threads)
+                // axis is the fastest (the C2R input): every thread keeps its
+                // kEndpointPerThread loads in flight at once
                 const int64_t count = ep.outer * ep.stride;
-                int64_t nb = (count + 511) / 512;
+                const int64_t per = 256 * (int64_t)kEndpointPerThread;
+                int64_t nb = (count + per - 1) / per;
                 if (nb < 1) nb = 1;
                 ep.nblk = (int)nb;
                 P.partials = std::max<int64_t>(P.partials, ep.extra * ep.planes * nb);

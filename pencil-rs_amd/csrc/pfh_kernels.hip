@@ -1703,28 +1703,53 @@ __device__ __forceinline__ const cpx<T> *plane_ptr(const EndpointArgs &e, const 
     return reinterpret_cast<const cpx<T> *>(data) + batch * e.len + k * e.stride;
 }
 
-__device__ __forceinline__ double block_reduce_max(double v, double *sh)
+// Three block maxima in one pass (one pair of barriers); sh holds 3 x 4.
+__device__ __forceinline__ void block_reduce_max3(double &a, double &b, double &c, double *sh)
 {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) {
+        a = fmax(a, __shfl_xor(a, o, 64));
+        b = fmax(b, __shfl_xor(b, o, 64));
+        c = fmax(c, __shfl_xor(c, o, 64));
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
     __syncthreads();
-    if (lane == 0) sh[w] = v;
+    if (lane == 0) {
+        sh[w] = a;
+        sh[4 + w] = b;
+        sh[8 + w] = c;
+    }
     __syncthreads();
-    double r = sh[0];
-    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) r = fmax(r, sh[i]);
-    return r; // every thread gets the block result
+    a = sh[0];
+    b = sh[4];
+    c = sh[8];
+    for (int i = 1; i < nw; ++i) {
+        a = fmax(a, sh[i]);
+        b = fmax(b, sh[4 + i]);
+        c = fmax(c, sh[8 + i]);
+    }
 }
 
-__device__ __forceinline__ double block_reduce_sum(double v, double *sh)
+// Two block sums in one pass, each in a fixed order (xor butterflies across
+// the lanes, then the waves in index order).
+__device__ __forceinline__ void block_reduce_sum2(double &a, double &b, double *sh)
 {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
     __syncthreads();
-    if (lane == 0) sh[w] = v;
+    if (lane == 0) {
+        sh[w] = a;
+        sh[4 + w] = b;
+    }
     __syncthreads();
-    double r = sh[0];
-    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
-    return r;
+    a = sh[0];
+    b = sh[4];
+    for (int i = 1; i < nw; ++i) {
+        a += sh[i];
+        b += sh[4 + i];
+    }
 }
 
 // One chunk of one constrained plane: its max(|re|,|im|), max|im|, and the
@@ -1734,7 +1759,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const void *data, EndpointPartial *part,
                                                           int *gate)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[12];
     if (!e.keep_gate && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *gate = 0;
     const int64_t bp = blockIdx.y;
     const int64_t batch = bp / e.planes;
@@ -1744,10 +1769,10 @@ __global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const 
     const int64_t chunk = (count + gridDim.x - 1) / gridDim.x;
     const int64_t begin = (int64_t)blockIdx.x * chunk;
     const int64_t end = begin + chunk < count ? begin + chunk : count;
-    // a chunk is at most two elements per thread (the planner sizes the grid
-    // for it): both stay in registers for the two passes (maximum, then sums
+    // a chunk is at most K elements per thread (the planner sizes the grid
+    // for it): they stay in registers for the two passes (maximum, then sums
     // scaled by it)
-    constexpr int K = 2;
+    constexpr int K = kEndpointPerThread;
     cpx<T> v[K];
     bool have[K];
     double ma = 0.0, mi = 0.0;
@@ -1773,9 +1798,8 @@ __global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const 
         ma = fmax(ma, fmax(fabs(re), fabs(im)));
         mi = fmax(mi, fabs(im));
     }
-    ma = block_reduce_max(ma, sh);
-    mi = block_reduce_max(mi, sh);
-    const double anybad = block_reduce_max((double)bad, sh);
+    double anybad = (double)bad;
+    block_reduce_max3(ma, mi, anybad, sh);
     double sr = 0.0, si = 0.0;
     if (ma != 0.0) {
 #pragma unroll
@@ -1786,8 +1810,7 @@ __global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const 
             si += ni * ni;
         }
     }
-    sr = block_reduce_sum(sr, sh);
-    si = block_reduce_sum(si, sh);
+    block_reduce_sum2(sr, si, sh);
     if (threadIdx.x == 0) {
         EndpointPartial p;
         p.maxa = ma;
@@ -1810,7 +1833,7 @@ __global__ __launch_bounds__(256) void k_endpoint_partial(EndpointArgs e, const 
 __global__ __launch_bounds__(256) void k_endpoint_combine(EndpointArgs e, const EndpointPartial *part, int *gate,
                                                           int *status)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[12];
     const int64_t b = blockIdx.x;
     bool invalid = !isfinite(e.relative) || !isfinite(e.absolute);
     for (int p = 0; p < e.planes; ++p) {
@@ -1821,9 +1844,9 @@ __global__ __launch_bounds__(256) void k_endpoint_combine(EndpointArgs e, const 
             mi = fmax(mi, q[c].maxi);
             if (q[c].bad) bad = 1.0;
         }
-        const double M = block_reduce_max(ma, sh);
-        const double MI = block_reduce_max(mi, sh);
-        if (block_reduce_max(bad, sh) != 0.0) invalid = true;
+        block_reduce_max3(ma, mi, bad, sh);
+        const double M = ma, MI = mi;
+        if (bad != 0.0) invalid = true;
         double s0 = 0.0, s1 = 0.0;
         if (M != 0.0) {
             for (int c = threadIdx.x; c < e.nblk; c += blockDim.x) {
@@ -1832,8 +1855,7 @@ __global__ __launch_bounds__(256) void k_endpoint_combine(EndpointArgs e, const 
                 s1 += q[c].sim * f * f;
             }
         }
-        s0 = block_reduce_sum(s0, sh);
-        s1 = block_reduce_sum(s1, sh);
+        block_reduce_sum2(s0, s1, sh);
         if (!isfinite(M) || !isfinite(MI) || !isfinite(s0) || !isfinite(s1)) invalid = true;
         const bool abs_ok = MI <= e.absolute;
         const bool rel_ok = sqrt(s1) <= e.relative * sqrt(s0);

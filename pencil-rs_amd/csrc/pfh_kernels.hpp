@@ -21,6 +21,10 @@ constexpr int kStatusOffFirst = 8;   // u64: first failing local endpoint code
 constexpr int kStatusOffTag = 16;    // u64: tag of the plan the workspace is bound to (0 = unbound)
 constexpr int kStatusOffPoison = 24; // int: an in-place call failed after it started writing
 
+// Plane elements per thread of the endpoint partial kernel (256 threads):
+// a chunk holds at most 256 x kEndpointPerThread elements.
+constexpr int kEndpointPerThread = 8;
+
 // One chunk's endpoint statistics of an R2C boundary plane.
 struct EndpointPartial {
     double maxa; // max(|re|, |im|)

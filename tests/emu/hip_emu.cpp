@@ -925,8 +925,9 @@ hipError_t launch_endpoint_check(const EndpointArgs &e, const void *data, Endpoi
     ++g_all_launches;
     begin_launch();
     const int64_t count = e.outer * e.stride;
-    // the partial kernel covers at most 2 x 256 elements of a chunk
-    if (e.nblk < 1 || (count + e.nblk - 1) / e.nblk > 512) fail("endpoint chunk of more than 512 elements");
+    // the partial kernel covers at most kEndpointPerThread x 256 elements of a chunk
+    if (e.nblk < 1 || (count + e.nblk - 1) / e.nblk > 256 * kEndpointPerThread)
+        fail("endpoint chunk of more than %d elements", 256 * kEndpointPerThread);
     for (int64_t b = 0; b < e.extra; ++b)
         for (int p = 0; p < e.planes; ++p) {
             const int64_t k = p == 0 ? 0 : e.clen - 1;
