@@ -744,7 +744,7 @@ static pfh_status size_step(Step &st, const DeviceInfo &dev, int tbytes, const T
     // and every other line stride and the point stride are even; a lane
     // then moves both with one 2-element access (PFH_CPAIR=0: one element).
     static const int cpair_env = env_int("PFH_CPAIR", 1);
-    st.flags &= ~(F_CPAIR_IN | F_CPAIR_OUT);
+    st.flags &= ~(F_CPAIR_IN | F_CPAIR_OUT | F_RPAIR_IN | F_RPAIR_OUT);
     const bool kind2 = st.mode != M_C2C && st.mode != M_R2C && st.mode != M_C2R && st.mode != M_DHT;
     auto pairs_ok = [&](bool in) {
         const LineGeom &g = st.g;
@@ -757,6 +757,17 @@ static pfh_status size_step(Step &st, const DeviceInfo &dev, int tbytes, const T
     if (cpair_env && st.fast && !st.long_only) {
         if (kind2 && st.in_col && !st.in_cplx && pairs_ok(true)) st.flags |= F_CPAIR_IN;
         if ((kind2 || st.mode == M_DHT) && st.out_col && !st.out_cplx && pairs_ok(false)) st.flags |= F_CPAIR_OUT;
+        // rows of the R2R stages: an even number of points and even line
+        // offsets keep every point pair 2-element aligned
+        auto rows_ok = [&](bool in) {
+            const LineGeom &g = st.g;
+            if ((in ? st.ips : st.ops) != 1 || (in ? st.nin : st.nout) % 2 != 0) return false;
+            for (int d = 0; d < g.nd; ++d)
+                if ((in ? g.is[d] : g.os[d]) % 2 != 0) return false;
+            return true;
+        };
+        if (kind2 && !st.in_col && !st.in_cplx && rows_ok(true)) st.flags |= F_RPAIR_IN;
+        if (kind2 && !st.out_col && !st.out_cplx && rows_ok(false)) st.flags |= F_RPAIR_OUT;
     }
     // Non-temporal global accesses (the compile-time radix plans only) once
     // the stage's input and output together exceed the 256 MB last-level
@@ -2383,6 +2394,8 @@ static StageArgs<T> fill_args(const PlanImpl &P, const Step &s, void *const bufs
     // paired column accesses need 2-element alignment of the array bases
     if ((a.flags & F_CPAIR_IN) && reinterpret_cast<uintptr_t>(a.src) % (2 * sizeof(T)) != 0) a.flags &= ~F_CPAIR_IN;
     if ((a.flags & F_CPAIR_OUT) && reinterpret_cast<uintptr_t>(a.dst) % (2 * sizeof(T)) != 0) a.flags &= ~F_CPAIR_OUT;
+    if ((a.flags & F_RPAIR_IN) && reinterpret_cast<uintptr_t>(a.src) % (2 * sizeof(T)) != 0) a.flags &= ~F_RPAIR_IN;
+    if ((a.flags & F_RPAIR_OUT) && reinterpret_cast<uintptr_t>(a.dst) % (2 * sizeof(T)) != 0) a.flags &= ~F_RPAIR_OUT;
     a.n = s.n;
     a.nin = s.nin;
     a.nout = s.nout;

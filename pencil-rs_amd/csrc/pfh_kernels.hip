@@ -611,6 +611,36 @@ __device__ __forceinline__ void load_line(const StageArgs<T> &a, cpx<T> *data, i
                     }
                 }
             }
+        } else if (!a.in_col && (a.flags & F_RPAIR_IN)) {
+            // points 2i and 2i + 1 of the row in one 2-element load
+            const cpx<T> *s2 = reinterpret_cast<const cpx<T> *>(reinterpret_cast<const T *>(a.src) + io);
+            const int np = a.nin >> 1;
+            if constexpr (MODE == M_DST1) {
+                if (p0 == 0) {
+                    LRset(lln, 0, (T)0);
+                    LRset(lln, a.n + 1, (T)0);
+                }
+            }
+            for (int base = p0; base < np; base += step * Regs<T>::gunroll) {
+                cpx<T> v[Regs<T>::gunroll];
+#pragma unroll
+                for (int u = 0; u < Regs<T>::gunroll; ++u) {
+                    const int i = base + u * step;
+                    if (i < np) v[u] = gld<NT>(s2 + i);
+                }
+#pragma unroll
+                for (int u = 0; u < Regs<T>::gunroll; ++u) {
+                    const int i = base + u * step;
+                    if (i < np) {
+                        if constexpr (scatter_mode<MODE>()) {
+                            scatter_real<T, MODE>(a, lln, 2 * i, v[u].x);
+                            scatter_real<T, MODE>(a, lln, 2 * i + 1, v[u].y);
+                        } else {
+                            Lset(lln, i, v[u]);
+                        }
+                    }
+                }
+            }
         } else {
             const T *s = reinterpret_cast<const T *>(a.src) + io;
             if constexpr (MODE == M_DST1) {
@@ -738,6 +768,24 @@ __device__ __forceinline__ void store_line(const StageArgs<T> &a, const cpx<T> *
                     for (int u = 0; u < U; ++u) {
                         const int k = base + u * tpl;
                         if (k < a.nout) gst<NT>(d + (int64_t)k * a.ops, v[u]);
+                    }
+                }
+            } else if (MG == 2 && !a.out_col && (a.flags & F_RPAIR_OUT)) {
+                // outputs 2i and 2i + 1 of the row in one 2-element store
+                cpx<T> *d2 = reinterpret_cast<cpx<T> *>(reinterpret_cast<T *>(a.dst) + oo);
+                const int np = a.nout >> 1;
+                for (int base = k0; base < np; base += tpl * U) {
+                    cpx<T> v[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int i = base + u * tpl;
+                        if (i < np)
+                            v[u] = mk(post_real<T, MG, MODE>(a, Z, tws, 2 * i), post_real<T, MG, MODE>(a, Z, tws, 2 * i + 1));
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int i = base + u * tpl;
+                        if (i < np) gst<NT>(d2 + i, v[u]);
                     }
                 }
             } else {

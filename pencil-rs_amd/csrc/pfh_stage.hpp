@@ -92,6 +92,10 @@ enum StageFlags : int {
     // a lane moves lines 2i and 2i + 1 of a point as one 2-element access
     F_CPAIR_IN = 65536,
     F_CPAIR_OUT = 131072,
+    // real row side of even length with even line offsets: a lane moves
+    // points 2i and 2i + 1 of a line as one 2-element access
+    F_RPAIR_IN = 262144,
+    F_RPAIR_OUT = 524288,
 };
 
 struct LineGeom {
