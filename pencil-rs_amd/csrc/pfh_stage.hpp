@@ -226,7 +226,8 @@ struct PlaneArgs {
 // Compile-time plans for route stages only (the batched kernels, which
 // carry C2C lines, do not instantiate them): the 385-point FFT of a DST-I
 // line of length 384 (5 x 7 x 11, 11 values per thread: 64 threads a line;
-// 22 values and 32 threads took 5% (f32) and 16% (f64) longer on MI355X)
+// with 22 values and 32 threads the 256x384x240 R2R forward + inverse took
+// 2% (f32) and 16% (f64) longer on MI355X)
 // and the f64 half-length FFTs of 240- and 384-point R2C / C2R / DHT lines.
 #define PFH_FAST_ROUTE_PLANS(X) \
     X(float, 11, 7, 5, 11)      \
