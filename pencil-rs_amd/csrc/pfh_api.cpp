@@ -3185,21 +3185,36 @@ static pfh_status local_execute_impl(pfh_plan plan, int op, const void *src, siz
         }
         for (int64_t c0 = 0; c0 < batch; c0 += tmpl.long_lines) {
             const int64_t cnt = std::min<int64_t>(tmpl.long_lines, batch - c0);
-            OpPlan o = tmpl;
-            Step &s = o.steps[0];
-            s.in.extra = s.out.extra = cnt;
-            s.local_endpoint_check = false;
-            s.gated = c2r_check;
-            int64_t xb[3] = {0, 0, 0};
-            pfh_status st = step_geometry(s);
-            if (!st) st = expand_long(P, s, tb, xb);
-            if (st) return st;
-            for (int r = 0; r < 3; ++r)
-                if (xb[r] > tmpl.x_bytes[r]) return set_err(PFH_ERR_PREPARATION_FAILED, "long-line chunk regions");
+            // the chunk's pieces are planned once per (op, cnt): planning them
+            // (factorizations, the Bluestein length search, piece sizing)
+            // took about 0.2 ms of host time per call for 10007-point lines
+            std::shared_ptr<const OpPlan> oc;
+            {
+                std::lock_guard<std::mutex> lk(P.chunks->mu);
+                auto it = P.chunks->plans.find({op, cnt});
+                if (it != P.chunks->plans.end()) {
+                    oc = it->second;
+                } else {
+                    auto o = std::make_shared<OpPlan>(tmpl);
+                    Step &s = o->steps[0];
+                    s.in.extra = s.out.extra = cnt;
+                    s.local_endpoint_check = false;
+                    s.gated = c2r_check;
+                    int64_t xb[3] = {0, 0, 0};
+                    pfh_status st = step_geometry(s);
+                    if (!st) st = expand_long(P, s, tb, xb);
+                    if (st) return st;
+                    for (int r = 0; r < 3; ++r)
+                        if (xb[r] > tmpl.x_bytes[r])
+                            return set_err(PFH_ERR_PREPARATION_FAILED, "long-line chunk regions");
+                    P.chunks->plans[{op, cnt}] = o;
+                    oc = o;
+                }
+            }
             const char *cs = static_cast<const char *>(src) + c0 * lin * tmpl.in.elem;
             char *cd = static_cast<char *>(dst) + c0 * lout * tmpl.out.elem;
             void *bufs[4] = {const_cast<char *>(cs), cd, nullptr, nullptr};
-            st = run_op(P, o, bufs, wv, stream, nullptr);
+            pfh_status st = run_op(P, *oc, bufs, wv, stream, nullptr);
             if (st) return st;
         }
         if (c2r_check && !(flags & PFH_EXEC_DEFER_STATUS)) return collect_status(P, wv, stream);

@@ -7,6 +7,7 @@
 #include <map>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -185,6 +186,14 @@ struct PlanImpl {
     // local plans
     size_t line_len = 0;
     int r2r_kind = 0;
+    // local long-line plans: the expanded operation for a chunk of cnt
+    // lines, keyed (op, cnt), built on the first execute that runs that
+    // chunk size (executes may come from several host threads)
+    struct ChunkCache {
+        std::mutex mu;
+        std::map<std::pair<int, int64_t>, std::shared_ptr<const OpPlan>> plans;
+    };
+    std::shared_ptr<ChunkCache> chunks = std::make_shared<ChunkCache>();
     ~PlanImpl();
 };
 
