@@ -265,3 +265,24 @@ def test_timing_entries_of_a_long_stage(gpu):
     entries = t.entries()
     assert len(entries) == 3 and all(ms > 0 for _, ms, _ in entries), entries
     assert rel_l2(_host(y), _host(plan.forward(x))) == 0.0
+
+
+def test_local_long_line_chunks_reuse_their_plans(gpu):
+    """A local plan runs long lines in chunks of about 256 MiB of piece
+    regions (819 lines of 10007 points in f32) and plans a chunk's pieces
+    once per (operation, line count). 830 lines (a full chunk and 11 lines),
+    5 lines, the 830 lines again (every chunk plan reused) and 821 lines (a
+    new short chunk) must give every line bit-exactly as the other calls do,
+    since lines never interact; the first five lines match the oracle."""
+    n = 10007
+    rng = np.random.default_rng(2026)
+    x = uniform_complex(rng, (830, n), complex_of(np.float32)).reshape(-1)
+    plan = pf.LocalC2cPlan(n, dtype=torch.complex64)
+    xd = _tdev(x)
+    y830 = _host(plan.forward(xd))
+    y5 = _host(plan.forward(xd[: 5 * n]))
+    ref5 = O.c2c_lines(x[: 5 * n].reshape(5, n).astype(np.complex128), O.FORWARD).reshape(-1)
+    assert rel_l2(y5, ref5) <= tol(np.float32)
+    assert np.array_equal(y5, y830[: 5 * n])
+    assert np.array_equal(_host(plan.forward(xd)), y830)
+    assert np.array_equal(_host(plan.forward(xd[: 821 * n])), y830[: 821 * n])
